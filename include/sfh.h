@@ -1,0 +1,208 @@
+/*
+ * sfh.h - C ABI of the MI355X (gfx950) staggered-fermion HMC hot path.
+ *
+ * Device-resident replacement for the field-level functions of the
+ * reference's Fermions.h / Lattice.h.  Each entry point below names the
+ * reference function it replaces (file:line).  Host buffers use the
+ * reference's array-of-structures layouts:
+ *   links / momenta / forces : double[V*4*18], (s*4+mu)*18 + (3a+b)*2 + re/im
+ *                              (Lattice.cpp:135-151, Parameters.h:11,20)
+ *   spinors                  : double[V*6],    s*6 + 2c + re/im
+ *                              (Fermions.cpp:99-116, Parameters.h:19)
+ *   site s = ((n0*L1 + n1)*L2 + n2)*L3 + n3, time = direction 3 carrying the
+ *   antiperiodic fermion boundary (Fermions.cpp:20-94).
+ * On the device, fields live in a structure-of-arrays, site-parity-ordered
+ * layout owned by the library (see sfh_gauge / sfh_field).
+ *
+ * Nothing throws across this ABI.  Every call returns an sfh_status; the
+ * message of the last failure on a context is sfh_last_error().  A context
+ * owns one HIP stream; calls on one context are serialised by an internal
+ * lock, so one context per host thread gives concurrency.
+ */
+#ifndef SFH_H
+#define SFH_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SFH_ABI_VERSION 1
+
+/* Status codes.  The failure cases mirror the reference's exceptions:
+ * BREAKDOWN / NONFINITE_* / NOCONV  <- std::runtime_error in solve_CG_DDdagger
+ *                                      (Fermions.cpp:573-600, :609-614)
+ * NOT_HERMITIAN                      <- std::runtime_error in the link update
+ *                                      (Fermions.cpp:712-716, :752-756)
+ * EXP_QCUBED / EXP_NEG_TRACE         <- std::invalid_argument in
+ *                                      numerically_stable_matrix_exponential
+ *                                      (Fermions.cpp:848-857)                 */
+typedef enum {
+  SFH_OK = 0,
+  SFH_E_INVALID = 1,
+  SFH_E_NOMEM = 2,
+  SFH_E_HIP = 3,
+  SFH_E_NODEVICE = 4,
+  SFH_E_BREAKDOWN = 5,
+  SFH_E_NONFINITE_ALPHA = 6,
+  SFH_E_NONFINITE_BETA = 7,
+  SFH_E_NOCONV = 8,
+  SFH_E_NOT_HERMITIAN = 9,
+  SFH_E_EXP_QCUBED = 10,
+  SFH_E_EXP_NEG_TRACE = 11
+} sfh_status;
+
+/* Arithmetic of the CG inner iterations.  F64 follows the reference's
+ * recurrence (Fermions.cpp:553-617).  F32 and F16 run the inner iterations in
+ * low precision with FP64 reliable updates (true residual recomputed in FP64
+ * whenever the iterated residual drops by reliable_delta), FP64 global
+ * reductions, and stop on the FP64 true residual.  F16 stores spinors as
+ * 16-bit floats with a per-site FP32 max-norm and links as 16-bit floats,
+ * accumulating in FP32. */
+typedef enum { SFH_PREC_F64 = 0, SFH_PREC_F32 = 1, SFH_PREC_F16 = 2 } sfh_precision;
+
+/* Force selection for sfh_force / sfh_leapfrog (Fermions.cpp:640-659 full,
+ * :1114-1134 fermion only, :1278-1294 gauge only). */
+#define SFH_FORCE_GAUGE 1u
+#define SFH_FORCE_FERMION 2u
+
+typedef struct sfh_ctx sfh_ctx;
+typedef struct sfh_gauge sfh_gauge;
+typedef struct sfh_field sfh_field;
+
+typedef struct {
+  double mass;           /* fermion_mass, Parameters.h:18 (runtime here)      */
+  double tol;            /* relative residual, Parameters.h:23                */
+  int max_iter;          /* Parameters.h:24                                   */
+  int precision;         /* sfh_precision                                     */
+  double reliable_delta; /* mixed precision: reliable update when |r| drops by
+                            this factor since the last one (0 -> 0.1)         */
+  int compute_true_residual; /* F64: also report the FP64 true residual       */
+} sfh_cg_params;
+
+typedef struct {
+  int iterations;
+  int reliable_updates;
+  int status;
+  int _pad;
+  double rsq;             /* final iterated |r|^2                              */
+  double true_rel_resid;  /* ||phi - DDdag x|| / ||phi|| in FP64 (if computed) */
+  double seconds;         /* device time of the solve                          */
+  double hbm_bytes;       /* algorithmic HBM bytes of the solve                */
+} sfh_cg_stats;
+
+typedef struct {
+  double mass, tol;
+  int max_iter, precision;
+  double beta, c11, c12; /* Parameters.h:51-52; Wilson is c11=1, c12=0        */
+  double eps;            /* MD step, Parameters.h:21                          */
+  int nsteps;            /* number_of_steps_fermion, Parameters.h:22          */
+  unsigned which;        /* SFH_FORCE_GAUGE | SFH_FORCE_FERMION              */
+  int check_hermitian;   /* 1: production guard of Fermions.cpp:712-716       */
+  int _pad;
+} sfh_md_params;
+
+/* ---------------------------------------------------------------- context */
+int sfh_abi_version(void);
+const char* sfh_status_string(int status);
+int sfh_create(int device, sfh_ctx** out);
+int sfh_destroy(sfh_ctx* ctx);
+const char* sfh_last_error(const sfh_ctx* ctx);
+int sfh_synchronize(sfh_ctx* ctx);
+/* Kernel timing with HIP events on the context stream, by kernel family:
+ * "hop_f64", "hop_dot_f64" (the hop with the fused <p,v> reduction), and the
+ * same for f32 / f16, "cg_blas", "force", "link_update".  on = 0 stops
+ * timing, on = 1 times every launch, on = k > 1 times one launch in k per
+ * family.  Setting it resets the totals.  No synchronisation is added to the
+ * timed launches. */
+int sfh_set_profiling(sfh_ctx* ctx, int on);
+int sfh_kernel_time(sfh_ctx* ctx, const char* name, double* total_ms, long* launches);
+
+/* ---------------------------------------------------------------- gauge field
+ * Device-resident links for one lattice.  dims = {L0, L1, L2, L3}, all even,
+ * L3 = time.  Replaces Link_array (Lattice.h:11) held in host memory. */
+int sfh_gauge_create(sfh_ctx* ctx, const int dims[4], sfh_gauge** out);
+int sfh_gauge_destroy(sfh_ctx* ctx, sfh_gauge* g);
+int sfh_gauge_upload(sfh_ctx* ctx, sfh_gauge* g, const double* links);
+int sfh_gauge_download(sfh_ctx* ctx, const sfh_gauge* g, double* links);
+/* Seeded links U = exp(i eps H), H = sum_a r_a lambda_a/2, r_a ~ N(0,1)
+ * (Philox4x32-10 + Box-Muller, counter = link index), generated on the host
+ * so the same seed gives the same bytes everywhere.  eps = 0 gives a cold
+ * start (Lattice.cpp:172-186). */
+int sfh_random_links(const int dims[4], double eps, unsigned long long seed, double* links);
+/* Seeded complex Gaussian field, sigma per real part (Fermions.cpp:207-212
+ * uses 1/sqrt(2)), and seeded momenta P = sum_a p_a lambda_a/2
+ * (Fermions.cpp:342-360). */
+int sfh_random_spinor(const int dims[4], double sigma, unsigned long long seed, double* out);
+int sfh_random_momenta(const int dims[4], unsigned long long seed, double* out);
+
+/* ---------------------------------------------------------------- spinor fields
+ * Device-resident spinor for the lattice of g (6V doubles). */
+int sfh_field_create(sfh_ctx* ctx, const sfh_gauge* g, sfh_field** out);
+int sfh_field_destroy(sfh_ctx* ctx, sfh_field* f);
+int sfh_field_upload(sfh_ctx* ctx, sfh_field* f, const double* host);
+int sfh_field_download(sfh_ctx* ctx, const sfh_field* f, double* host);
+
+/* ---------------------------------------------------------------- fermion ops */
+/* out = D in (dagger=0) or D^dagger in (dagger=1), host buffers.
+ * Replaces the per-site loops of apply_dirac_operator_at_site_s /
+ * apply_dirac_dagger_at_site_s (Fermions.cpp:218-288) as used by
+ * generate_phi_field (:302-314) and solve_Y_field (:623-633). */
+int sfh_apply_d(sfh_ctx* ctx, const sfh_gauge* g, double mass, int dagger, const double* in, double* out);
+/* out = D (D^dagger in).  Replaces apply_DDdagger (Fermions.cpp:530-547). */
+int sfh_apply_ddag(sfh_ctx* ctx, const sfh_gauge* g, double mass, const double* in, double* out);
+/* Device-resident variants of the two above. */
+int sfh_apply_d_field(sfh_ctx* ctx, const sfh_gauge* g, double mass, int dagger, const sfh_field* in,
+                      sfh_field* out);
+/* Re<a,b> over n doubles, FP64 accumulation, fixed reduction order.
+ * Replaces field_inner_product_real (Fermions.cpp:521-528). */
+int sfh_inner(sfh_ctx* ctx, size_t n, const double* a, const double* b, double* out);
+
+/* Solve (D D^dagger) x = phi.  Replaces solve_CG_DDdagger / solve_X_field
+ * (Fermions.cpp:553-621).  stats may be NULL. */
+int sfh_cg(sfh_ctx* ctx, const sfh_gauge* g, const sfh_cg_params* p, const double* phi, double* x,
+           sfh_cg_stats* stats);
+int sfh_cg_field(sfh_ctx* ctx, const sfh_gauge* g, const sfh_cg_params* p, const sfh_field* phi,
+                 sfh_field* x, sfh_cg_stats* stats);
+
+/* ---------------------------------------------------------------- forces, MD */
+/* F = F_gauge(beta, c11, c12) + F_fermion(X, Y) per link, selected by which.
+ * Replaces compute_full_force_field (Fermions.cpp:640-659) and the
+ * fermion-only / gauge-only fields (:1114-1134, :1278-1294).  X and Y may be
+ * NULL when which excludes the fermion force. */
+int sfh_force(sfh_ctx* ctx, const sfh_gauge* g, double beta, double c11, double c12, const double* X,
+              const double* Y, unsigned which, double* F);
+/* Symanzik gauge action summed per link.  Replaces compute_action
+ * (Lattice.cpp:759-781). */
+int sfh_action(sfh_ctx* ctx, const sfh_gauge* g, double beta, double c11, double c12, double* out);
+/* Replaces compute_avg_plaq (Lattice.cpp:784-804), same normalisation. */
+int sfh_avg_plaq(sfh_ctx* ctx, const sfh_gauge* g, double* out);
+/* sum over links of ReTr(P^2).  Replaces compute_Tr_P2 (Fermions.cpp:662-678). */
+int sfh_tr_p2(sfh_ctx* ctx, const sfh_gauge* g, const double* P, double* out);
+/* U <- exp(i eps P) U on every device link (Fermions.cpp:703-723, exponential
+ * :843-902).  check_hermitian=1 applies the reference's guard. */
+int sfh_link_update(sfh_ctx* ctx, sfh_gauge* g, double eps, const double* P, int check_hermitian);
+/* exp(iQ) of one 3x3 matrix on the device (Fermions.cpp:843-902). */
+int sfh_exp_iq(sfh_ctx* ctx, const double* Q18, double* out18);
+
+/* Leapfrog with the links kept on the device: P half-kick, (nsteps-1) x
+ * {link update, kick}, link update, half-kick.  Replaces
+ * apply_initial_step_to_conjugate_field, apply_intermediate_update_... and
+ * apply_final_step (Fermions.cpp:684-776) and their eps/n_steps copies
+ * (:967-1028).  phi may be NULL when which excludes the fermion force;
+ * X_out (may be NULL) receives the final solve. */
+int sfh_leapfrog(sfh_ctx* ctx, sfh_gauge* g, const sfh_md_params* p, const double* phi, double* P_inout,
+                 double* X_out);
+/* One HMC update (Fermions.cpp:778-828): phi = D chi, H_i, leapfrog, H_f,
+ * Metropolis with the caller's uniform r.  H[0..6] = TrP2_i, Sg_i, Sf_i,
+ * TrP2_f, Sg_f, Sf_f, H_i - H_f.  On accept the device links take the end
+ * point; otherwise they are left unchanged. */
+int sfh_hmc_trajectory(sfh_ctx* ctx, sfh_gauge* g, const sfh_md_params* p, const double* chi, const double* P0,
+                       double r_uniform, double* H, int* accepted);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SFH_H */

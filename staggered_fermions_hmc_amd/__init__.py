@@ -1,0 +1,9 @@
+"""MI355X (gfx950) staggered-fermion HMC hot path.
+
+libsfh.so (HIP, built from csrc/) exposes the C ABI of include/sfh.h;
+``device`` wraps it in Python objects and ``fermions`` mirrors the reference's
+field-level Fermions.h / Lattice.h functions on top of it.
+"""
+from . import _abi  # noqa: F401
+
+__all__ = ["device", "fermions"]

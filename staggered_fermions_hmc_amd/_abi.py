@@ -1,0 +1,129 @@
+"""ctypes binding of libsfh.so, the C ABI declared in include/sfh.h.
+
+The library is the product: there is no fallback.  Loading fails loudly when
+libsfh.so is missing (build it with ``python -c "import __graft_entry__ as g;
+g.build()"`` or ``make -C staggered_fermions_hmc_amd/csrc``).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsfh.so")
+
+_D = ctypes.c_double
+_I = ctypes.c_int
+_U = ctypes.c_uint
+_P = ctypes.c_void_p
+_SZ = ctypes.c_size_t
+_ULL = ctypes.c_ulonglong
+
+SFH_OK = 0
+STATUS_NAMES = {
+    0: "SFH_OK", 1: "SFH_E_INVALID", 2: "SFH_E_NOMEM", 3: "SFH_E_HIP", 4: "SFH_E_NODEVICE",
+    5: "SFH_E_BREAKDOWN", 6: "SFH_E_NONFINITE_ALPHA", 7: "SFH_E_NONFINITE_BETA", 8: "SFH_E_NOCONV",
+    9: "SFH_E_NOT_HERMITIAN", 10: "SFH_E_EXP_QCUBED", 11: "SFH_E_EXP_NEG_TRACE",
+}
+PREC = {"f64": 0, "f32": 1, "f16": 2}
+FORCE_GAUGE = 1
+FORCE_FERMION = 2
+
+
+class CGParams(ctypes.Structure):
+    _fields_ = [("mass", _D), ("tol", _D), ("max_iter", _I), ("precision", _I), ("reliable_delta", _D),
+                ("compute_true_residual", _I)]
+
+
+class CGStats(ctypes.Structure):
+    _fields_ = [("iterations", _I), ("reliable_updates", _I), ("status", _I), ("_pad", _I), ("rsq", _D),
+                ("true_rel_resid", _D), ("seconds", _D), ("hbm_bytes", _D)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("_")}
+
+
+class MDParams(ctypes.Structure):
+    _fields_ = [("mass", _D), ("tol", _D), ("max_iter", _I), ("precision", _I), ("beta", _D), ("c11", _D),
+                ("c12", _D), ("eps", _D), ("nsteps", _I), ("which", _U), ("check_hermitian", _I), ("_pad", _I)]
+
+
+# Every exported symbol of include/sfh.h with its signature.
+SIGNATURES = {
+    "sfh_abi_version": (_I, []),
+    "sfh_status_string": (ctypes.c_char_p, [_I]),
+    "sfh_create": (_I, [_I, ctypes.POINTER(_P)]),
+    "sfh_destroy": (_I, [_P]),
+    "sfh_last_error": (ctypes.c_char_p, [_P]),
+    "sfh_synchronize": (_I, [_P]),
+    "sfh_set_profiling": (_I, [_P, _I]),
+    "sfh_kernel_time": (_I, [_P, ctypes.c_char_p, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_long)]),
+    "sfh_gauge_create": (_I, [_P, _P, ctypes.POINTER(_P)]),
+    "sfh_gauge_destroy": (_I, [_P, _P]),
+    "sfh_gauge_upload": (_I, [_P, _P, _P]),
+    "sfh_gauge_download": (_I, [_P, _P, _P]),
+    "sfh_random_links": (_I, [_P, _D, _ULL, _P]),
+    "sfh_random_spinor": (_I, [_P, _D, _ULL, _P]),
+    "sfh_random_momenta": (_I, [_P, _ULL, _P]),
+    "sfh_field_create": (_I, [_P, _P, ctypes.POINTER(_P)]),
+    "sfh_field_destroy": (_I, [_P, _P]),
+    "sfh_field_upload": (_I, [_P, _P, _P]),
+    "sfh_field_download": (_I, [_P, _P, _P]),
+    "sfh_apply_d": (_I, [_P, _P, _D, _I, _P, _P]),
+    "sfh_apply_ddag": (_I, [_P, _P, _D, _P, _P]),
+    "sfh_apply_d_field": (_I, [_P, _P, _D, _I, _P, _P]),
+    "sfh_inner": (_I, [_P, _SZ, _P, _P, ctypes.POINTER(_D)]),
+    "sfh_cg": (_I, [_P, _P, ctypes.POINTER(CGParams), _P, _P, ctypes.POINTER(CGStats)]),
+    "sfh_cg_field": (_I, [_P, _P, ctypes.POINTER(CGParams), _P, _P, ctypes.POINTER(CGStats)]),
+    "sfh_force": (_I, [_P, _P, _D, _D, _D, _P, _P, _U, _P]),
+    "sfh_action": (_I, [_P, _P, _D, _D, _D, ctypes.POINTER(_D)]),
+    "sfh_avg_plaq": (_I, [_P, _P, ctypes.POINTER(_D)]),
+    "sfh_tr_p2": (_I, [_P, _P, _P, ctypes.POINTER(_D)]),
+    "sfh_link_update": (_I, [_P, _P, _D, _P, _I]),
+    "sfh_exp_iq": (_I, [_P, _P, _P]),
+    "sfh_leapfrog": (_I, [_P, _P, ctypes.POINTER(MDParams), _P, _P, _P]),
+    "sfh_hmc_trajectory": (_I, [_P, _P, ctypes.POINTER(MDParams), _P, _P, _D, _P, ctypes.POINTER(_I)]),
+}
+
+_lib = None
+
+
+class SfhError(RuntimeError):
+    """A non-zero sfh_status.  ``status`` holds the code, the message is the
+    library's last-error text (the reference's exception text where one
+    exists)."""
+
+    def __init__(self, status, message):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+        self.message = message
+
+
+def lib():
+    """Load libsfh.so once; raise if it is absent (no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"libsfh.so not built at {LIB_PATH}: the HIP library is required")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def dims_arr(dims):
+    d = np.ascontiguousarray(np.asarray(dims, dtype=np.int32))
+    if d.shape != (4,):
+        raise ValueError("dims must have 4 entries (L0, L1, L2, T)")
+    return d

@@ -1,0 +1,87 @@
+// Drop-in replacement of the reference's field-level Fermions.h / Lattice.h
+// functions, implemented on the MI355X library through the C ABI (sfh.h).
+//
+// Same names, signatures, argument meaning, ownership and exceptions as the
+// reference (Fermions.h:31-70, :105, :130; Lattice.h:76-77):
+//   - inputs are const& host vectors in the reference layouts, results are
+//     returned by value with exact sizes 6V or 4V*18;
+//   - apply_DDdagger writes into the caller's pre-sized out_field;
+//   - the leapfrog stages mutate P_field and the links in place;
+//   - CG failures and the hermiticity guard throw std::runtime_error, the
+//     exponential's guards std::invalid_argument, with the reference's texts.
+// The reference's compile-time constants (Parameters.h:9-24, :51-52) are
+// runtime values here: set them once with sfh_compat::configure().
+// Each host thread gets its own library context (the reference calls these
+// from inside an OpenMP loop over coupling values, Main.cpp:174).
+#pragma once
+
+#include <vector>
+
+namespace sfh_compat {
+
+struct Config {
+  int Spatial_Size = 4;              // Parameters.h:9
+  int temporal_size = 4;             // Parameters.h:10
+  double fermion_mass = 1.0;         // Parameters.h:18
+  double epsilon = 1e-2;             // Parameters.h:21
+  int number_of_steps_fermion = 100; // Parameters.h:22
+  double tol = 1e-12;                // Parameters.h:23
+  int max_iter = 50000;              // Parameters.h:24
+  double c_11 = 5.0 / 3.0;           // Parameters.h:51
+  double c_12 = -1.0 / 12.0;         // Parameters.h:52
+  int precision = 0;                 // 0 FP64 (reference), 1 FP32 inner, 2 FP16 inner
+  int device = 0;
+  unsigned long long seed = 0x5f3759dfULL;  // Philox key of the field generators
+  bool verbose = true;               // print the H components like Fermions.cpp:798-804
+};
+
+void configure(const Config& c);
+const Config& config();
+int volume();
+
+}  // namespace sfh_compat
+
+using Link_array = std::vector<double>;
+
+struct TrajectoryResult {  // Fermions.h:87-91
+  double delta_H;
+  double initial_H;
+  double final_H;
+};
+
+std::vector<double> generate_phi_field(const Link_array& arr);
+std::vector<double> generate_conjugate_field_configuration();
+double field_inner_product_real(const std::vector<double>& A, const std::vector<double>& B);
+void apply_DDdagger(const std::vector<double>& in_field, const Link_array& arr, std::vector<double>& out_field);
+std::vector<double> solve_CG_DDdagger(const std::vector<double>& phi_field, const Link_array& arr);
+std::vector<double> solve_X_field(const std::vector<double>& phi_field, const Link_array& arr);
+std::vector<double> solve_Y_field(const std::vector<double>& X_field, const Link_array& arr);
+std::vector<double> compute_full_force_field(const std::vector<double>& X_field, const std::vector<double>& Y_field,
+                                             const Link_array& arr, double beta);
+std::vector<double> compute_fermion_only_force_field(const std::vector<double>& X_field,
+                                                     const std::vector<double>& Y_field, const Link_array& arr);
+std::vector<double> compute_gauge_only_force_field(const Link_array& arr, double beta);
+double compute_Tr_P2(const std::vector<double>& P_field);
+void apply_initial_step_to_conjugate_field(std::vector<double>& P_field, double beta,
+                                           const std::vector<double>& phi_field, const Link_array& arr);
+void apply_intermediate_update_to_gauge_field_and_conjugate_field(std::vector<double>& P_field, double beta,
+                                                                  const std::vector<double>& phi_field,
+                                                                  Link_array& arr);
+std::vector<double> apply_final_step(std::vector<double>& P_field, double beta, const std::vector<double>& phi_field,
+                                     Link_array& arr);
+void Monte_carlo_step(double initial_Tr_P2, double initial_phi_field_action, const std::vector<double>& P_field_final,
+                      const std::vector<double>& phi_field, double gauge_field_initial_action,
+                      Link_array& gauge_field_initial, Link_array& gauge_field_final,
+                      const std::vector<double>& X_field, double beta);
+void single_sweep_monte_carlo_update_WITH_FERMIONS(Link_array& gauge_field, double beta);
+
+// eps / n_steps copies of the stages (Fermions.h:131-141)
+void local_initial_half_step(std::vector<double>& P_field, double beta, const std::vector<double>& phi_field,
+                             const Link_array& arr, double eps);
+void local_leapfrog_steps(std::vector<double>& P_field, double beta, const std::vector<double>& phi_field,
+                          Link_array& arr, double eps, int n_steps);
+std::vector<double> local_final_step(std::vector<double>& P_field, double beta, const std::vector<double>& phi_field,
+                                     Link_array& arr, double eps);
+
+double compute_action(const Link_array& arr, double beta);
+double compute_avg_plaq(const Link_array& arr);

@@ -1,0 +1,416 @@
+// Gauge-side kernels: plaquette and rectangle staples, Symanzik gauge force,
+// staggered fermion force, the fused momentum kick, the SU(3) exponential and
+// link update, and the FP64 action / plaquette / Tr P^2 reductions.
+// All FP64, one thread per link.  The staple kernels are FP64-VALU-bound
+// (about 85 SU(3) products per link); the fermion force and link update are
+// HBM-bound.
+#include "internal.h"
+#include "reduce.h"
+
+namespace sfh {
+
+using M = mat3<double>;
+using C = cpx<double>;
+
+__device__ __forceinline__ int dindex(const Geom& g, const int n[4]) {
+  const int p = (n[0] + n[1] + n[2] + n[3]) & 1;
+  return p * g.Vh + cb_of(g, n);
+}
+
+struct Pos {
+  int n[4];
+  __device__ __forceinline__ Pos step(const Geom& g, int mu, int d) const {
+    Pos r = *this;
+    int v = r.n[mu] + d;
+    const int L = g.L[mu];
+    if (v < 0) v += L;
+    else if (v >= L) v -= L;
+    r.n[mu] = v;
+    return r;
+  }
+};
+
+__device__ __forceinline__ M ldU(const double2* __restrict__ U, const Geom& g, const Pos& x, int mu) {
+  const int i = dindex(g, x.n);
+  M m;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) {
+    const double2 v = U[(size_t)(mu * 9 + e) * g.V + i];
+    m.e[e] = {v.x, v.y};
+  }
+  return m;
+}
+__device__ __forceinline__ M ldL(const double2* __restrict__ U, const Geom& g, int i, int mu) {
+  M m;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) {
+    const double2 v = U[(size_t)(mu * 9 + e) * g.V + i];
+    m.e[e] = {v.x, v.y};
+  }
+  return m;
+}
+__device__ __forceinline__ void stL(double2* __restrict__ U, const Geom& g, int i, int mu, const M& m) {
+#pragma unroll
+  for (int e = 0; e < 9; ++e) U[(size_t)(mu * 9 + e) * g.V + i] = make_double2(m.e[e].re, m.e[e].im);
+}
+
+// Plaquette staple sum A1 (Lattice.cpp:249-313): ReTr(U_mu(n) A1) sums the six
+// plaquettes containing the link.
+__device__ M staple_sum(const double2* __restrict__ U, const Geom& g, const Pos& n, int mu) {
+  M sum = mzero<double>();
+#pragma unroll 1
+  for (int nu = 0; nu < 4; ++nu) {
+    if (nu == mu) continue;
+    const Pos a = n.step(g, nu, -1);
+    const Pos b = a.step(g, mu, +1);
+    const M lo = mmul_dn(ldU(U, g, b, nu), mmul_dn(ldU(U, g, a, mu), ldU(U, g, a, nu)));
+    sum = madd(sum, lo);
+    const Pos c = n.step(g, nu, +1);
+    const Pos d = n.step(g, mu, +1);
+    const M hi = mmul(ldU(U, g, d, nu), mmul_nd(mdag(ldU(U, g, c, mu)), ldU(U, g, n, nu)));
+    sum = madd(sum, hi);
+  }
+  return sum;
+}
+
+// Rectangle staple sum A2 (Lattice.cpp:315-499): the six 2x1 / 1x2 loops per
+// orthogonal direction that contain the link.
+__device__ M rect_sum(const double2* __restrict__ U, const Geom& g, const Pos& n, int mu) {
+  M sum = mzero<double>();
+#pragma unroll 1
+  for (int nu = 0; nu < 4; ++nu) {
+    if (nu == mu) continue;
+    const Pos nP = n.step(g, mu, +1), n2P = nP.step(g, mu, +1), nM = n.step(g, mu, -1);
+    const Pos nU = n.step(g, nu, +1), n2U = nU.step(g, nu, +1);
+    const Pos nD = n.step(g, nu, -1), n2D = nD.step(g, nu, -1);
+    M t;
+    {  // short side, below
+      const Pos a = nP.step(g, nu, -1), b = a.step(g, nu, -1);
+      t = mmul_nd(mdag(ldU(U, g, a, nu)), ldU(U, g, b, nu));
+      t = mmul_nd(t, ldU(U, g, n2D, mu));
+      t = mmul(t, ldU(U, g, n2D, nu));
+      t = mmul(t, ldU(U, g, nD, nu));
+      sum = madd(sum, t);
+    }
+    {  // short side, above
+      const Pos a = nP.step(g, nu, +1);
+      t = mmul(ldU(U, g, nP, nu), ldU(U, g, a, nu));
+      t = mmul_nd(t, ldU(U, g, n2U, mu));
+      t = mmul_nd(t, ldU(U, g, nU, nu));
+      t = mmul_nd(t, ldU(U, g, n, nu));
+      sum = madd(sum, t);
+    }
+    {  // long side, link first, below
+      const Pos a = n2P.step(g, nu, -1), b = nP.step(g, nu, -1);
+      t = mmul_nd(ldU(U, g, nP, mu), ldU(U, g, a, nu));
+      t = mmul_nd(t, ldU(U, g, b, mu));
+      t = mmul_nd(t, ldU(U, g, nD, mu));
+      t = mmul(t, ldU(U, g, nD, nu));
+      sum = madd(sum, t);
+    }
+    {  // long side, link second, above
+      const Pos a = nM.step(g, nu, +1);
+      t = mmul_nd(ldU(U, g, nP, nu), ldU(U, g, nU, mu));
+      t = mmul_nd(t, ldU(U, g, a, mu));
+      t = mmul_nd(t, ldU(U, g, nM, nu));
+      t = mmul(t, ldU(U, g, nM, mu));
+      sum = madd(sum, t);
+    }
+    {  // long side, link second, below
+      const Pos a = nP.step(g, nu, -1), b = nM.step(g, nu, -1);
+      t = mmul_nd(mdag(ldU(U, g, a, nu)), ldU(U, g, nD, mu));
+      t = mmul_nd(t, ldU(U, g, b, mu));
+      t = mmul(t, ldU(U, g, b, nu));
+      t = mmul(t, ldU(U, g, nM, mu));
+      sum = madd(sum, t);
+    }
+    {  // long side, link first, above
+      const Pos a = nP.step(g, nu, +1);
+      t = mmul(ldU(U, g, nP, mu), ldU(U, g, n2P, nu));
+      t = mmul_nd(t, ldU(U, g, a, mu));
+      t = mmul_nd(t, ldU(U, g, nU, mu));
+      t = mmul_nd(t, ldU(U, g, n, nu));
+      sum = madd(sum, t);
+    }
+  }
+  return sum;
+}
+
+__device__ __forceinline__ M traceless(const M& f) {
+  M r = f;
+  const C t = mtrace(f);
+  const C t3 = {t.re / 3.0, t.im / 3.0};
+  r.e[0] = r.e[0] - t3;
+  r.e[4] = r.e[4] - t3;
+  r.e[8] = r.e[8] - t3;
+  return r;
+}
+
+// -i s (A - A^dag) for a real s.
+__device__ __forceinline__ M mi_antiherm(const M& A, double s) {
+  M r;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const C d = A.e[3 * i + j] - conj(A.e[3 * j + i]);
+      r.e[3 * i + j] = {s * d.im, -s * d.re};
+    }
+  return r;
+}
+
+__device__ __forceinline__ void site_of(const Geom& g, int i, Pos& x) {
+  const int p = i >= g.Vh;
+  cb_coords(g, i - p * g.Vh, p, x.n);
+}
+
+// Total force on link (i, mu) and optional P -= coef F (Fermions.cpp:372-384,
+// :443-515, :640-659, :690-733).
+__global__ __launch_bounds__(128) void k_force(Geom g, const double2* __restrict__ U, const double2* __restrict__ X,
+                                               const double2* __restrict__ Y, double beta, double c11, double c12,
+                                               unsigned which, double2* __restrict__ F, double2* __restrict__ P,
+                                               double coef) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int mu = blockIdx.y;
+  if (i >= g.V) return;
+  Pos n;
+  site_of(g, i, n);
+  M f = mzero<double>();
+  const M u = ldL(U, g, i, mu);
+  if (which & SFH_FORCE_GAUGE) {
+    M A = mscale(c11, staple_sum(U, g, n, mu));
+    if (c12 != 0.0) A = madd(A, mscale(c12, rect_sum(U, g, n, mu)));
+    const M Q = mmul(u, A);
+    f = traceless(mi_antiherm(Q, beta / 12.0));
+  }
+  if (which & SFH_FORCE_FERMION) {
+    const Pos up = n.step(g, mu, +1);
+    const int j = dindex(g, up.n);
+    int es = 0;
+    for (int k = 0; k < mu; ++k) es += n.n[k];
+    double c = (es & 1) ? -0.25 : 0.25;
+    if (mu == 3 && n.n[3] == g.L[3] - 1) c = -c;
+    C x0[3], x1[3], y0[3], y1[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      double2 v = X[(size_t)a * g.V + i]; x0[a] = {v.x, v.y};
+      v = X[(size_t)a * g.V + j]; x1[a] = {v.x, v.y};
+      v = Y[(size_t)a * g.V + i]; y0[a] = {v.x, v.y};
+      v = Y[(size_t)a * g.V + j]; y1[a] = {v.x, v.y};
+    }
+    C uy[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      C s = u.e[3 * a] * y1[0];
+      cmac(s, u.e[3 * a + 1], y1[1]);
+      cmac(s, u.e[3 * a + 2], y1[2]);
+      uy[a] = s;
+    }
+    M af, ab0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        af.e[3 * a + b] = uy[a] * conj(x0[b]);
+        ab0.e[3 * a + b] = y0[a] * conj(x1[b]);
+      }
+    const M ab = mmul_nd(ab0, u);
+    const M ff = traceless(mi_antiherm(madd(af, ab), c));
+    f = madd(f, ff);
+  }
+  if (F) stL(F, g, i, mu, f);
+  if (P) {
+    M pm = ldL(P, g, i, mu);
+    pm = msub(pm, mscale(coef, f));
+    stL(P, g, i, mu, pm);
+  }
+}
+
+int force_field(sfh_ctx* c, const Geom& g, const double2* U, const double2* X, const double2* Y, double beta,
+                double c11, double c12, unsigned which, double2* F, double2* P, double coef) {
+  KTimer kt(c, "force");
+  dim3 grid((g.V + 127) / 128, 4);
+  hipLaunchKernelGGL(k_force, grid, dim3(128), 0, c->stream, g, U, X, Y, beta, c11, c12, which, F, P, coef);
+  return hip_check(c, hipGetLastError(), "force kernel launch");
+}
+
+// ------------------------------------------------------------------ reductions over links
+// kind 0: action, 1: plaquette sum ReTr(U A1)/3, 2: ReTr(P^2)
+__global__ __launch_bounds__(128) void k_link_sum(Geom g, const double2* __restrict__ U, double beta, double c11,
+                                                  double c12, int kind, double* partials, unsigned* cnt,
+                                                  double* out, int nb) {
+  double v = 0.0;
+  for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < 4 * g.V; l += nb * blockDim.x) {
+    const int i = l >> 2, mu = l & 3;
+    const M u = ldL(U, g, i, mu);
+    if (kind == 2) {
+      v += retr_mul(u, u);
+    } else {
+      Pos n;
+      site_of(g, i, n);
+      const double pl = retr_mul(u, staple_sum(U, g, n, mu));
+      if (kind == 1) {
+        v += pl / 3.0;
+      } else {
+        const double rc = c12 != 0.0 ? retr_mul(u, rect_sum(U, g, n, mu)) : 0.0;
+        v += -(beta / 3.0) * (c11 * (pl / 4.0) + c12 * (rc / 6.0));
+      }
+    }
+  }
+  grid_reduce(v, partials, cnt, nb, blockIdx.x, FinStore{out});
+}
+
+static double link_sum(sfh_ctx* c, const Geom& g, const double2* U, double beta, double c11, double c12, int kind) {
+  int nb = (4 * g.V + 127) / 128;
+  if (nb > kPartialSlot) nb = kPartialSlot;
+  KTimer kt(c, "force");
+  hipLaunchKernelGGL(k_link_sum, dim3(nb), dim3(128), 0, c->stream, g, U, beta, c11, c12, kind,
+                     c->d_partials + 2 * kPartialSlot, &c->d_state[1].cnt[CNT_AUX], &c->d_state[1].pv, nb);
+  hipMemcpyAsync(c->h_scalar, &c->d_state[1].pv, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (hip_check(c, hipStreamSynchronize(c->stream), "link reduction")) return NAN;
+  return *c->h_scalar;
+}
+
+double gauge_action(sfh_ctx* c, const Geom& g, const double2* U, double beta, double c11, double c12) {
+  return link_sum(c, g, U, beta, c11, c12, 0);
+}
+double avg_plaq(sfh_ctx* c, const Geom& g, const double2* U) {
+  return link_sum(c, g, U, 0, 0, 0, 1) / (4.0 * g.V);
+}
+double tr_p2(sfh_ctx* c, const Geom& g, const double2* P) { return link_sum(c, g, P, 0, 0, 0, 2); }
+
+// ------------------------------------------------------------------ exponential and link update
+// exp(iQ) for hermitian traceless Q by the Cayley-Hamilton closed form of
+// Morningstar & Peardon (hep-lat/0311018), with the guards and small-Q branch
+// of Fermions.cpp:843-902.  Returns a status code.
+__device__ int exp_iq(const M& Q, M& out) {
+  const M Q2 = mmul(Q, Q);
+  const M Q3 = mmul(Q2, Q);
+  const C t3 = mtrace(Q3);
+  if (fabs(t3.im) > 1e-10) return SFH_E_EXP_QCUBED;
+  double c0 = t3.re / 3.0;
+  const double c1 = mtrace(Q2).re / 2.0;
+  if (c1 < 0.0) return SFH_E_EXP_NEG_TRACE;
+  if (c1 < 1e-12) {
+    M r = meye<double>();
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      r.e[k].re += -Q.e[k].im - 0.5 * Q2.e[k].re;
+      r.e[k].im += Q.e[k].re - 0.5 * Q2.e[k].im;
+    }
+    out = r;
+    return SFH_OK;
+  }
+  const bool neg = c0 < 0.0;
+  c0 = fabs(c0);
+  const double c0max = 2.0 * sqrt(c1 * c1 * c1 / 27.0);
+  double ratio = c0max > 0.0 ? c0 / c0max : 1.0;
+  ratio = fmin(fmax(ratio, -1.0), 1.0);
+  const double theta = acos(ratio);
+  const double w = sqrt(c1) * sin(theta / 3.0);
+  const double u = sqrt(c1 / 3.0) * cos(theta / 3.0);
+  double s2u, c2u, su, cu;
+  sincos(2.0 * u, &s2u, &c2u);
+  sincos(u, &su, &cu);
+  const C e2u = {c2u, s2u}, emu = {cu, -su};
+  double xi;
+  if (fabs(w) <= 0.05) {
+    const double w2 = w * w;
+    xi = 1.0 - (w2 / 6.0) * (1.0 - (w2 / 20.0) * (1.0 - w2 / 42.0));
+  } else {
+    xi = sin(w) / w;
+  }
+  const double cw = cos(w);
+  const C h0 = (u * u - w * w) * e2u + emu * C{8.0 * u * u * cw, 2.0 * u * (3.0 * u * u + w * w) * xi};
+  const C h1 = (2.0 * u) * e2u - emu * C{2.0 * u * cw, -(3.0 * u * u - w * w) * xi};
+  const C h2 = e2u - emu * C{cw, 3.0 * u * xi};
+  const double den = 9.0 * u * u - w * w;
+  C f0 = {h0.re / den, h0.im / den}, f1 = {h1.re / den, h1.im / den}, f2 = {h2.re / den, h2.im / den};
+  if (neg) {
+    f0 = conj(f0);
+    f1 = {-f1.re, f1.im};
+    f2 = conj(f2);
+  }
+  M r;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) r.e[k] = f1 * Q.e[k] + f2 * Q2.e[k];
+  r.e[0] = r.e[0] + f0;
+  r.e[4] = r.e[4] + f0;
+  r.e[8] = r.e[8] + f0;
+  out = r;
+  return SFH_OK;
+}
+
+__global__ __launch_bounds__(128) void k_link_update(Geom g, double2* __restrict__ U, const double2* __restrict__ P,
+                                                     double eps, int check, int* status) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int mu = blockIdx.y;
+  if (i >= g.V) return;
+  const M q = mscale(eps, ldL(P, g, i, mu));
+  if (check) {
+    double hn = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int b = 0; b < 3; ++b) {
+        const C d = q.e[3 * a + b] - conj(q.e[3 * b + a]);
+        hn += d.re * d.re + d.im * d.im;
+      }
+    const C t = mtrace(q);
+    if (sqrt(hn) > 1e-10 || sqrt(t.re * t.re + t.im * t.im) > 1e-10) {
+      atomicMax(status, SFH_E_NOT_HERMITIAN);
+      return;
+    }
+  }
+  M e;
+  const int st = exp_iq(q, e);
+  if (st != SFH_OK) {
+    atomicMax(status, st);
+    return;
+  }
+  stL(U, g, i, mu, mmul(e, ldL(U, g, i, mu)));
+}
+
+int link_update(sfh_ctx* c, const Geom& g, double2* U, const double2* P, double eps, int check) {
+  int* d_status = &c->d_state[1].status;
+  hipMemsetAsync(d_status, 0, sizeof(int), c->stream);
+  {
+    KTimer kt(c, "link_update");
+    dim3 grid((g.V + 127) / 128, 4);
+    hipLaunchKernelGGL(k_link_update, grid, dim3(128), 0, c->stream, g, U, P, eps, check, d_status);
+  }
+  int* h = reinterpret_cast<int*>(c->h_scalar);
+  hipMemcpyAsync(h, d_status, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (int e = hip_check(c, hipStreamSynchronize(c->stream), "link update")) return e;
+  if (*h == SFH_E_NOT_HERMITIAN) return set_err(c, *h, "P_pre is not Hermitian or not traceless");
+  if (*h == SFH_E_EXP_QCUBED) return set_err(c, *h, "Error: Q cubed matrix has non-zero imaginary trace.");
+  if (*h == SFH_E_EXP_NEG_TRACE) return set_err(c, *h, "Error: Q squared matrix has negative trace.");
+  return SFH_OK;
+}
+
+__global__ void k_exp_one(const double* Q18, double* out18, int* status) {
+  M q, e;
+  for (int k = 0; k < 9; ++k) q.e[k] = {Q18[2 * k], Q18[2 * k + 1]};
+  *status = exp_iq(q, e);
+  for (int k = 0; k < 9; ++k) {
+    out18[2 * k] = e.e[k].re;
+    out18[2 * k + 1] = e.e[k].im;
+  }
+}
+
+int exp_iq_one(sfh_ctx* c, const double* Q18, double* out18) {
+  double* d = static_cast<double*>(scratch(c, "exp_one", 40 * sizeof(double)));
+  if (!d) return SFH_E_NOMEM;
+  int* dst = reinterpret_cast<int*>(d + 36);
+  hipMemcpyAsync(d, Q18, 18 * sizeof(double), hipMemcpyHostToDevice, c->stream);
+  hipLaunchKernelGGL(k_exp_one, dim3(1), dim3(1), 0, c->stream, d, d + 18, dst);
+  hipMemcpyAsync(out18, d + 18, 18 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  int* h = reinterpret_cast<int*>(c->h_scalar);
+  hipMemcpyAsync(h, dst, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (int e = hip_check(c, hipStreamSynchronize(c->stream), "exp")) return e;
+  if (*h == SFH_E_EXP_QCUBED) return set_err(c, *h, "Error: Q cubed matrix has non-zero imaginary trace.");
+  if (*h == SFH_E_EXP_NEG_TRACE) return set_err(c, *h, "Error: Q squared matrix has negative trace.");
+  return SFH_OK;
+}
+
+}  // namespace sfh

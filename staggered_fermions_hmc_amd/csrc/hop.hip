@@ -1,0 +1,424 @@
+// Staggered hop kernels for gfx950 and the layout converters around them.
+//
+// One launch applies the operator to the whole lattice, both parities:
+//   out(n) = a in(n) + (b/2) sum_mu eta_mu(n) [ bup(n,mu) U_mu(n) in(n+mu)
+//                                             - bdn(n,mu) U_mu(n-mu)^dag in(n-mu) ]
+// which is D (a=m, b=+1) or D^dagger (a=m, b=-1) of Fermions.cpp:218-288.
+// Each workgroup takes the same checkerboard range on both parities (first
+// half of its lanes even sites, second half odd), so a link loaded as the
+// forward link of one site is reused from cache as the backward link of its
+// neighbour.  Workgroups are remapped so each XCD sweeps a contiguous range.
+// The hop is HBM-bound (~0.9 flop/byte in FP64): no MFMA.
+#include "internal.h"
+#include "reduce.h"
+
+namespace sfh {
+
+struct Half {};
+
+template <class T> struct Prec;
+template <> struct Prec<double> { using R = double; using L = double2; using S = double2; };
+template <> struct Prec<float> { using R = float; using L = float2; using S = float2; };
+template <> struct Prec<Half> { using R = float; using L = __half2; using S = __half2; };
+
+__device__ __forceinline__ cpx<double> ldc(const double2* p) {
+  const double2 v = *p;
+  return {v.x, v.y};
+}
+__device__ __forceinline__ cpx<float> ldc(const float2* p) {
+  const float2 v = *p;
+  return {v.x, v.y};
+}
+__device__ __forceinline__ cpx<float> ldc(const __half2* p) {
+  const float2 v = __half22float2(*p);
+  return {v.x, v.y};
+}
+
+template <class T>
+__device__ __forceinline__ void load_link(const typename Prec<T>::L* __restrict__ U, int V, int mu, int i,
+                                          mat3<typename Prec<T>::R>& m) {
+#pragma unroll
+  for (int e = 0; e < 9; ++e) m.e[e] = ldc(U + (size_t)(mu * 9 + e) * V + i);
+}
+
+template <class T>
+__device__ __forceinline__ void load_spinor(const typename Prec<T>::S* __restrict__ f, const float* __restrict__ nrm,
+                                            int V, int i, vec3<typename Prec<T>::R>& v) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) v.c[c] = ldc(f + (size_t)c * V + i);
+  if constexpr (std::is_same<T, Half>::value) {
+    const float s = nrm[i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v.c[c] = s * v.c[c];
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void store_spinor(typename Prec<T>::S* __restrict__ f, float* __restrict__ nrm, int V,
+                                             int i, const vec3<typename Prec<T>::R>& v) {
+  if constexpr (std::is_same<T, Half>::value) {
+    float m = 0.f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) m = fmaxf(m, fmaxf(fabsf(v.c[c].re), fabsf(v.c[c].im)));
+    const float inv = m > 0.f ? 1.0f / m : 0.f;
+    nrm[i] = m;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f[(size_t)c * V + i] = __floats2half2_rn(v.c[c].re * inv, v.c[c].im * inv);
+  } else {
+    using S = typename Prec<T>::S;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f[(size_t)c * V + i] = S{v.c[c].re, v.c[c].im};
+  }
+}
+
+// acc += s * U v
+template <class R>
+__device__ __forceinline__ void acc_mv(vec3<R>& acc, const mat3<R>& U, const vec3<R>& v, R s) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    cpx<R> t = U.e[3 * a] * v.c[0];
+    cmac(t, U.e[3 * a + 1], v.c[1]);
+    cmac(t, U.e[3 * a + 2], v.c[2]);
+    acc.c[a].re += s * t.re;
+    acc.c[a].im += s * t.im;
+  }
+}
+// acc += s * U^dag v
+template <class R>
+__device__ __forceinline__ void acc_mdv(vec3<R>& acc, const mat3<R>& U, const vec3<R>& v, R s) {
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    cpx<R> t = conj(U.e[a]) * v.c[0];
+    cmacc(t, U.e[3 + a], v.c[1]);
+    cmacc(t, U.e[6 + a], v.c[2]);
+    acc.c[a].re += s * t.re;
+    acc.c[a].im += s * t.im;
+  }
+}
+
+template <class T, bool DOT>
+__global__ __launch_bounds__(256) void k_hop(Geom g, const typename Prec<T>::L* __restrict__ U,
+                                             const typename Prec<T>::S* __restrict__ in,
+                                             const float* __restrict__ in_norm, typename Prec<T>::S* __restrict__ out,
+                                             float* __restrict__ out_norm, typename Prec<T>::R a,
+                                             typename Prec<T>::R hb, const typename Prec<T>::S* __restrict__ dw,
+                                             const float* __restrict__ dw_norm, CGState* st, double* partials,
+                                             int nb) {
+  using R = typename Prec<T>::R;
+  if (st != nullptr && st->done) return;
+  const int lb = xcd_block(nb);
+  const int hw = blockDim.x >> 1;
+  const int p = threadIdx.x >= hw;
+  const int cb = lb * hw + (threadIdx.x - p * hw);
+  double dotv = 0.0;
+  if (cb < g.Vh) {
+    const int V = g.V;
+    const int i = p * g.Vh + cb;
+    const int q = (1 - p) * g.Vh;
+    int n[4];
+    cb_coords(g, cb, p, n);
+    vec3<R> acc;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc.c[c] = {R(0), R(0)};
+    int esum = 0;
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) {
+      const R eta = (esum & 1) ? R(-1) : R(1);
+      esum += n[mu];
+      int fw, bw;
+      R sf = eta, sb = eta;
+      if (mu == 3) {
+        const int h = n[3] >> 1;
+        const int up = n[3] + 1 == g.L[3] ? 0 : n[3] + 1;
+        const int dn = n[3] == 0 ? g.L[3] - 1 : n[3] - 1;
+        fw = cb - h + (up >> 1);
+        bw = cb - h + (dn >> 1);
+        if (n[3] == g.L[3] - 1) sf = -sf;
+        if (n[3] == 0) sb = -sb;
+      } else {
+        const int S = mu == 0 ? g.S0 : (mu == 1 ? g.S1 : g.S2);
+        const int L = g.L[mu];
+        fw = cb + (n[mu] + 1 == L ? -(L - 1) * S : S);
+        bw = cb + (n[mu] == 0 ? (L - 1) * S : -S);
+      }
+      mat3<R> Uf, Ub;
+      vec3<R> vf, vb;
+      load_link<T>(U, V, mu, i, Uf);
+      load_spinor<T>(in, in_norm, V, q + fw, vf);
+      load_link<T>(U, V, mu, q + bw, Ub);
+      load_spinor<T>(in, in_norm, V, q + bw, vb);
+      acc_mv(acc, Uf, vf, sf);
+      acc_mdv(acc, Ub, vb, -sb);
+    }
+    vec3<R> self, res;
+    load_spinor<T>(in, in_norm, V, i, self);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      res.c[c].re = a * self.c[c].re + hb * acc.c[c].re;
+      res.c[c].im = a * self.c[c].im + hb * acc.c[c].im;
+    }
+    store_spinor<T>(out, out_norm, V, i, res);
+    if constexpr (DOT) {
+      vec3<R> w;
+      load_spinor<T>(dw, dw_norm, V, i, w);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) dotv += (double)w.c[c].re * (double)res.c[c].re + (double)w.c[c].im * (double)res.c[c].im;
+    }
+  }
+  if constexpr (DOT) grid_reduce(dotv, partials, &st->cnt[CNT_PV], nb, lb, FinPV{st});
+}
+
+// Four lanes per site, one per direction mu: each lane does the forward and
+// backward hop of its direction, the four partial sums meet through two
+// cross-lane exchanges, and lane mu < 3 writes colour mu.  Four times the
+// waves of the one-lane form, for lattices too small to fill 256 CUs and
+// for more loads in flight per CU.  Lane layout in a wavefront:
+// lane = mu * 16 + site, so the 16 lanes of one direction load 16
+// consecutive sites (coalesced).
+template <class T, bool DOT>
+__global__ __launch_bounds__(256) void k_hop4(Geom g, const typename Prec<T>::L* __restrict__ U,
+                                              const typename Prec<T>::S* __restrict__ in,
+                                              const float* __restrict__ in_norm, typename Prec<T>::S* __restrict__ out,
+                                              float* __restrict__ out_norm, typename Prec<T>::R a,
+                                              typename Prec<T>::R hb, const typename Prec<T>::S* __restrict__ dw,
+                                              const float* __restrict__ dw_norm, CGState* st, double* partials,
+                                              int nb) {
+  using R = typename Prec<T>::R;
+  if (st != nullptr && st->done) return;
+  const int lb = xcd_block(nb);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int mu = lane >> 4, sl = lane & 15;
+  // 4 waves: waves 0,1 even sites, waves 2,3 odd sites, same cb range of 32
+  const int p = wave >> 1;
+  const int cb = lb * 32 + (wave & 1) * 16 + sl;
+  const bool valid = cb < g.Vh;
+  const int V = g.V;
+  const int i = p * g.Vh + (valid ? cb : 0);
+  const int q = (1 - p) * g.Vh;
+  int n[4];
+  cb_coords(g, valid ? cb : 0, p, n);
+  vec3<R> acc;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) acc.c[c] = {R(0), R(0)};
+  if (valid) {
+    const int es = (mu > 0 ? n[0] : 0) + (mu > 1 ? n[1] : 0) + (mu > 2 ? n[2] : 0);
+    R sf = (es & 1) ? R(-1) : R(1), sb = sf;
+    int fw, bw;
+    if (mu == 3) {
+      const int h = n[3] >> 1;
+      const int up = n[3] + 1 == g.L[3] ? 0 : n[3] + 1;
+      const int dn = n[3] == 0 ? g.L[3] - 1 : n[3] - 1;
+      fw = cb - h + (up >> 1);
+      bw = cb - h + (dn >> 1);
+      if (n[3] == g.L[3] - 1) sf = -sf;
+      if (n[3] == 0) sb = -sb;
+    } else {
+      const int S = mu == 0 ? g.S0 : (mu == 1 ? g.S1 : g.S2);
+      const int L = mu == 0 ? g.L[0] : (mu == 1 ? g.L[1] : g.L[2]);
+      const int nm = mu == 0 ? n[0] : (mu == 1 ? n[1] : n[2]);
+      fw = cb + (nm + 1 == L ? -(L - 1) * S : S);
+      bw = cb + (nm == 0 ? (L - 1) * S : -S);
+    }
+    mat3<R> Uf, Ub;
+    vec3<R> vf, vb;
+    load_link<T>(U, V, mu, i, Uf);
+    load_spinor<T>(in, in_norm, V, q + fw, vf);
+    load_link<T>(U, V, mu, q + bw, Ub);
+    load_spinor<T>(in, in_norm, V, q + bw, vb);
+    acc_mv(acc, Uf, vf, sf);
+    acc_mdv(acc, Ub, vb, -sb);
+  }
+  // sum over the four direction lanes (xor 16, xor 32); fixed order
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    R re = acc.c[c].re, im = acc.c[c].im;
+    re += __shfl_xor(re, 16, 64);
+    im += __shfl_xor(im, 16, 64);
+    re += __shfl_xor(re, 32, 64);
+    im += __shfl_xor(im, 32, 64);
+    acc.c[c] = {re, im};
+  }
+  double dotv = 0.0;
+  if constexpr (std::is_same<T, Half>::value) {
+    // FP16 output needs the whole site for its norm: lane mu == 0 stores it.
+    if (valid && mu == 0) {
+      vec3<R> self, res;
+      load_spinor<T>(in, in_norm, V, i, self);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        res.c[c].re = a * self.c[c].re + hb * acc.c[c].re;
+        res.c[c].im = a * self.c[c].im + hb * acc.c[c].im;
+      }
+      store_spinor<T>(out, out_norm, V, i, res);
+      if constexpr (DOT) {
+        vec3<R> w;
+        load_spinor<T>(dw, dw_norm, V, i, w);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          dotv += (double)w.c[c].re * (double)res.c[c].re + (double)w.c[c].im * (double)res.c[c].im;
+      }
+    }
+  } else {
+    if (valid && mu < 3) {
+      using S = typename Prec<T>::S;
+      const cpx<R> self = ldc(in + (size_t)mu * V + i);
+      const cpx<R> ac = mu == 0 ? acc.c[0] : (mu == 1 ? acc.c[1] : acc.c[2]);
+      const R re = a * self.re + hb * ac.re, im = a * self.im + hb * ac.im;
+      out[(size_t)mu * V + i] = S{re, im};
+      if constexpr (DOT) {
+        const cpx<R> w = ldc(dw + (size_t)mu * V + i);
+        dotv = (double)w.re * (double)re + (double)w.im * (double)im;
+      }
+    }
+  }
+  if constexpr (DOT) grid_reduce(dotv, partials, &st->cnt[CNT_PV], nb, lb, FinPV{st});
+}
+
+constexpr int kHopBlock = 256;
+
+static inline int hop_blocks(const Geom& g) { return (g.Vh + kHopBlock / 2 - 1) / (kHopBlock / 2); }
+static inline int hop4_blocks(const Geom& g) { return (g.Vh + 31) / 32; }
+
+// Variant 0: one lane per site; variant 1: four lanes per site.
+template <class T>
+static void hop_launch(hipStream_t s, int variant, const Geom& g, const typename Prec<T>::L* U,
+                       const typename Prec<T>::S* in, const float* in_norm, typename Prec<T>::S* out, float* out_norm,
+                       typename Prec<T>::R a, typename Prec<T>::R hb, const typename Prec<T>::S* dw,
+                       const float* dw_norm, CGState* st, double* partials) {
+  if (variant == 1) {
+    const int nb = hop4_blocks(g);
+    if (dw)
+      hipLaunchKernelGGL((k_hop4<T, true>), dim3(nb), dim3(256), 0, s, g, U, in, in_norm, out, out_norm, a, hb, dw,
+                         dw_norm, st, partials, nb);
+    else
+      hipLaunchKernelGGL((k_hop4<T, false>), dim3(nb), dim3(256), 0, s, g, U, in, in_norm, out, out_norm, a, hb,
+                         dw, dw_norm, st, partials, nb);
+  } else {
+    const int nb = hop_blocks(g);
+    if (dw)
+      hipLaunchKernelGGL((k_hop<T, true>), dim3(nb), dim3(kHopBlock), 0, s, g, U, in, in_norm, out, out_norm, a, hb,
+                         dw, dw_norm, st, partials, nb);
+    else
+      hipLaunchKernelGGL((k_hop<T, false>), dim3(nb), dim3(kHopBlock), 0, s, g, U, in, in_norm, out, out_norm, a,
+                         hb, dw, dw_norm, st, partials, nb);
+  }
+}
+
+static int g_hop_variant[3] = {-1, -1, -1};  // forced variant per precision, -1 = by size
+
+static int pick_variant(int prec, const Geom& g) {
+  if (g_hop_variant[prec] >= 0) return g_hop_variant[prec];
+  // four lanes per site when one lane per site gives fewer than 16 waves per CU
+  return g.V < 256 * 64 * 16 ? 1 : 0;
+}
+
+void set_hop_variant(int prec, int variant) {
+  if (prec >= 0 && prec < 3) g_hop_variant[prec] = variant;
+}
+int get_hop_variant(int prec, const Geom& g) { return pick_variant(prec, g); }
+
+void hop_f64(hipStream_t s, const Geom& g, const double2* U, const double2* in, double2* out, double a, double b,
+             const double2* dot_with, CGState* st, double* partials) {
+  hop_launch<double>(s, pick_variant(0, g), g, U, in, nullptr, out, nullptr, a, 0.5 * b, dot_with, nullptr, st,
+                     partials);
+}
+
+void hop_f32(hipStream_t s, const Geom& g, const float2* U, const float2* in, float2* out, float a, float b,
+             const float2* dot_with, CGState* st, double* partials) {
+  hop_launch<float>(s, pick_variant(1, g), g, U, in, nullptr, out, nullptr, a, 0.5f * b, dot_with, nullptr, st,
+                    partials);
+}
+
+void hop_f16(hipStream_t s, const Geom& g, const __half2* U, const __half2* in, const float* in_norm, __half2* out,
+             float* out_norm, float a, float b, const __half2* dot_with, const float* dot_norm, CGState* st,
+             double* partials) {
+  hop_launch<Half>(s, pick_variant(2, g), g, U, in, in_norm, out, out_norm, a, 0.5f * b, dot_with, dot_norm, st,
+                   partials);
+}
+
+// ------------------------------------------------------------------ layout converters
+// One thread per device site; the reference site is computed from (p, cb).
+__global__ void k_links_aos_to_soa(Geom g, const double* __restrict__ aos, double2* __restrict__ soa) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.V) return;
+  const int p = i >= g.Vh, cb = i - p * g.Vh;
+  const size_t s = (size_t)ref_site(g, cb, p);
+  for (int mu = 0; mu < 4; ++mu)
+    for (int e = 0; e < 9; ++e) {
+      const double* src = aos + (s * 4 + mu) * 18 + 2 * e;
+      soa[(size_t)(mu * 9 + e) * g.V + i] = make_double2(src[0], src[1]);
+    }
+}
+
+__global__ void k_links_soa_to_aos(Geom g, const double2* __restrict__ soa, double* __restrict__ aos) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.V) return;
+  const int p = i >= g.Vh, cb = i - p * g.Vh;
+  const size_t s = (size_t)ref_site(g, cb, p);
+  for (int mu = 0; mu < 4; ++mu)
+    for (int e = 0; e < 9; ++e) {
+      const double2 v = soa[(size_t)(mu * 9 + e) * g.V + i];
+      double* dst = aos + (s * 4 + mu) * 18 + 2 * e;
+      dst[0] = v.x;
+      dst[1] = v.y;
+    }
+}
+
+__global__ void k_spinor_aos_to_soa(Geom g, const double* __restrict__ aos, double2* __restrict__ soa) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.V) return;
+  const int p = i >= g.Vh, cb = i - p * g.Vh;
+  const size_t s = (size_t)ref_site(g, cb, p);
+  for (int c = 0; c < 3; ++c) soa[(size_t)c * g.V + i] = make_double2(aos[s * 6 + 2 * c], aos[s * 6 + 2 * c + 1]);
+}
+
+__global__ void k_spinor_soa_to_aos(Geom g, const double2* __restrict__ soa, double* __restrict__ aos) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.V) return;
+  const int p = i >= g.Vh, cb = i - p * g.Vh;
+  const size_t s = (size_t)ref_site(g, cb, p);
+  for (int c = 0; c < 3; ++c) {
+    const double2 v = soa[(size_t)c * g.V + i];
+    aos[s * 6 + 2 * c] = v.x;
+    aos[s * 6 + 2 * c + 1] = v.y;
+  }
+}
+
+__global__ void k_links_to_f32(size_t n, const double2* __restrict__ U, float2* __restrict__ u) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double2 v = U[i];
+  u[i] = make_float2((float)v.x, (float)v.y);
+}
+
+__global__ void k_links_to_f16(size_t n, const double2* __restrict__ U, __half2* __restrict__ u) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double2 v = U[i];
+  u[i] = __floats2half2_rn((float)v.x, (float)v.y);
+}
+
+static inline unsigned grid1(size_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+void links_aos_to_soa(hipStream_t s, const Geom& g, const double* aos, double2* soa) {
+  hipLaunchKernelGGL(k_links_aos_to_soa, dim3(grid1(g.V, 256)), dim3(256), 0, s, g, aos, soa);
+}
+void links_soa_to_aos(hipStream_t s, const Geom& g, const double2* soa, double* aos) {
+  hipLaunchKernelGGL(k_links_soa_to_aos, dim3(grid1(g.V, 256)), dim3(256), 0, s, g, soa, aos);
+}
+void spinor_aos_to_soa(hipStream_t s, const Geom& g, const double* aos, double2* soa) {
+  hipLaunchKernelGGL(k_spinor_aos_to_soa, dim3(grid1(g.V, 256)), dim3(256), 0, s, g, aos, soa);
+}
+void spinor_soa_to_aos(hipStream_t s, const Geom& g, const double2* soa, double* aos) {
+  hipLaunchKernelGGL(k_spinor_soa_to_aos, dim3(grid1(g.V, 256)), dim3(256), 0, s, g, soa, aos);
+}
+void links_to_f32(hipStream_t s, const Geom& g, const double2* U, float2* u32) {
+  const size_t n = (size_t)36 * g.V;
+  hipLaunchKernelGGL(k_links_to_f32, dim3(grid1(n, 256)), dim3(256), 0, s, n, U, u32);
+}
+void links_to_f16(hipStream_t s, const Geom& g, const double2* U, __half2* u16) {
+  const size_t n = (size_t)36 * g.V;
+  hipLaunchKernelGGL(k_links_to_f16, dim3(grid1(n, 256)), dim3(256), 0, s, n, U, u16);
+}
+
+}  // namespace sfh

@@ -1,0 +1,161 @@
+// Internal host-side structures and kernel entry points of libsfh.
+#pragma once
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/sfh.h"
+#include "common.h"
+
+namespace sfh {
+
+// Per-solve device scalars.  Written only by the last workgroup of a grid
+// reduction (or by the host between launches), read by later launches, so
+// kernels of one stream never race on them.
+struct CGState {
+  double rsq;       // iterated |r|^2
+  double rsq_prev;  // |r|^2 before the last update (for beta after a reliable update)
+  double alpha, beta, pv;
+  double target;    // tol^2 |phi|^2
+  double rmax;      // mixed precision: max |r| since the last reliable update
+  double delta;     // mixed precision: reliable-update factor
+  int iter, done, status, max_iter;
+  int pause_reason;  // mixed precision: 1 reliable update due, 2 iterated residual converged
+  int mixed;
+  unsigned cnt[8];  // grid-reduction arrival counters
+};
+
+enum { CNT_PV = 0, CNT_RR = 1, CNT_AUX = 2 };
+
+// Low-precision link / spinor storage (FP16 uses a per-site FP32 norm).
+struct LinksLP {
+  float2* u32 = nullptr;  // 4*9*V
+  __half2* u16 = nullptr; // 4*9*V
+};
+
+}  // namespace sfh
+
+struct sfh_field {
+  sfh::Geom g;
+  double2* d = nullptr;  // 3*V complex, device layout
+};
+
+struct sfh_gauge {
+  sfh::Geom g;
+  double2* U = nullptr;     // 4*9*V complex, device layout
+  sfh::LinksLP lp;
+  bool lp32_valid = false, lp16_valid = false;
+};
+
+struct sfh_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::recursive_mutex mu;
+  // grow-only device scratch, keyed by purpose
+  std::map<std::string, std::pair<void*, size_t>> scratch;
+  // pinned host staging for scalars
+  sfh::CGState* h_state = nullptr;
+  sfh::CGState* d_state = nullptr;
+  double* d_partials = nullptr;  // reduction partials (8 slots x 65536)
+  double* h_scalar = nullptr;
+  bool profiling = false;
+  int prof_every = 1;   // time one launch in prof_every per kernel name
+  std::map<std::string, long> prof_seen;
+  std::map<std::string, std::pair<double, long>> ktime;
+  // event pool for kernel timing: pending (name, start, stop) resolved lazily
+  std::vector<hipEvent_t> ev_pool;
+  struct Pending { std::string name; hipEvent_t a, b; };
+  std::vector<Pending> pending;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;  // whole-solve timing
+};
+
+namespace sfh {
+
+constexpr int kPartialSlot = 65536;  // max workgroups per reduction slot
+
+// ------------------------------------------------------------------ helpers
+int set_err(sfh_ctx* c, int code, const std::string& msg);
+int hip_check(sfh_ctx* c, hipError_t e, const char* what);
+void* scratch(sfh_ctx* c, const std::string& key, size_t bytes);
+
+// Profiling: brackets the launches of its scope with a pair of events on the
+// context stream when profiling is on (no synchronisation); the durations are
+// accumulated by resolve_timers().
+struct KTimer {
+  sfh_ctx* c;
+  const char* name;
+  hipEvent_t a = nullptr;
+  KTimer(sfh_ctx* c_, const char* n);
+  ~KTimer();
+};
+void resolve_timers(sfh_ctx* c);
+
+// ------------------------------------------------------------------ layout (hop.hip)
+void links_aos_to_soa(hipStream_t s, const Geom& g, const double* aos, double2* soa);
+void links_soa_to_aos(hipStream_t s, const Geom& g, const double2* soa, double* aos);
+void spinor_aos_to_soa(hipStream_t s, const Geom& g, const double* aos, double2* soa);
+void spinor_soa_to_aos(hipStream_t s, const Geom& g, const double2* soa, double* aos);
+void links_to_f32(hipStream_t s, const Geom& g, const double2* U, float2* u32);
+void links_to_f16(hipStream_t s, const Geom& g, const double2* U, __half2* u16);
+
+// ------------------------------------------------------------------ hop (hop.hip)
+// out = a*in + (b/2) * sum_mu eta [bup U psi(n+mu) - bdn U^dag psi(n-mu)]
+// If dot_with != nullptr, also reduces Re<dot_with, out> into st (pv, alpha).
+void hop_f64(hipStream_t s, const Geom& g, const double2* U, const double2* in, double2* out, double a, double b,
+             const double2* dot_with, CGState* st, double* partials);
+void hop_f32(hipStream_t s, const Geom& g, const float2* U, const float2* in, float2* out, float a, float b,
+             const float2* dot_with, CGState* st, double* partials);
+void hop_f16(hipStream_t s, const Geom& g, const __half2* U, const __half2* in, const float* in_norm,
+             __half2* out, float* out_norm, float a, float b, const __half2* dot_with, const float* dot_norm,
+             CGState* st, double* partials);
+
+// Hop kernel form: 0 one lane per site, 1 four lanes per site (one per
+// direction); -1 restores the size-based default.  prec: 0 f64, 1 f32, 2 f16.
+void set_hop_variant(int prec, int variant);
+int get_hop_variant(int prec, const Geom& g);
+
+// ------------------------------------------------------------------ BLAS (blas.hip)
+void dot_f64(hipStream_t s, const double* a, const double* b, size_t n, double* partials, unsigned* cnt,
+             double* result);
+// CG update kernels for each storage precision.
+void cg_xr_f64(hipStream_t s, const Geom& g, double2* x, double2* r, const double2* p, const double2* v,
+               CGState* st, double* partials);
+void cg_p_f64(hipStream_t s, const Geom& g, double2* p, const double2* r, CGState* st);
+void cg_xr_f32(hipStream_t s, const Geom& g, float2* x, float2* r, const float2* p, const float2* v,
+               CGState* st, double* partials);
+void cg_p_f32(hipStream_t s, const Geom& g, float2* p, const float2* r, CGState* st);
+void cg_xr_f16(hipStream_t s, const Geom& g, float2* x, __half2* r, float* rn, const __half2* p, const float* pn,
+               const __half2* v, const float* vn, CGState* st, double* partials);
+void cg_p_f16(hipStream_t s, const Geom& g, __half2* p, float* pn, const __half2* r, const float* rn,
+              CGState* st);
+// mixed-precision support
+void f64_to_f32(hipStream_t s, const double2* in, float2* out, size_t n);
+void f64_to_f16(hipStream_t s, const Geom& g, const double2* in, __half2* out, float* norm);
+void acc_f32_into_f64(hipStream_t s, double2* xh, float2* xl, size_t n);  // xh += xl; xl = 0
+void xpay_f64(hipStream_t s, double2* y, const double2* x, double a, size_t n);  // y = x + a*y
+void axpy_f64(hipStream_t s, double2* y, const double2* x, double a, size_t n);  // y += a*x
+void sub_f64(hipStream_t s, double2* out, const double2* a, const double2* b, size_t n);  // out = a - b
+
+// ------------------------------------------------------------------ gauge (gauge.hip)
+int force_field(sfh_ctx* c, const Geom& g, const double2* U, const double2* X, const double2* Y, double beta,
+                double c11, double c12, unsigned which, double2* F, double2* P, double coef);
+double gauge_action(sfh_ctx* c, const Geom& g, const double2* U, double beta, double c11, double c12);
+double avg_plaq(sfh_ctx* c, const Geom& g, const double2* U);
+double tr_p2(sfh_ctx* c, const Geom& g, const double2* P);
+int link_update(sfh_ctx* c, const Geom& g, double2* U, const double2* P, double eps, int check);
+int exp_iq_one(sfh_ctx* c, const double* Q18, double* out18);
+
+// ------------------------------------------------------------------ solver (solver.cpp)
+int cg_solve(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* p, const double2* phi, double2* x,
+             sfh_cg_stats* stats);
+int apply_ddag_dev(sfh_ctx* c, const sfh_gauge* gauge, double mass, const double2* in, double2* out);
+
+// host generators (gen.cpp)
+void gen_links(const int* dims, double eps, unsigned long long seed, double* out);
+void gen_spinor(const int* dims, double sigma, unsigned long long seed, double* out);
+void gen_momenta(const int* dims, unsigned long long seed, double* out);
+
+}  // namespace sfh

@@ -1,0 +1,106 @@
+// Deterministic grid reductions: every workgroup writes its partial sum, the
+// last workgroup to arrive (arrival counted with one device-scope atomic)
+// sums all partials in index order and runs the finisher.  The result does
+// not depend on arrival order, only on the grid shape, so it is reproducible
+// run to run and independent of what else shares the GPU.
+#pragma once
+
+#include "common.h"
+#include "internal.h"
+
+namespace sfh {
+
+template <class Fin>
+__device__ __forceinline__ void grid_reduce(double v, double* partials, unsigned* counter, int nblocks, int slot,
+                                            Fin fin) {
+  const double bs = block_sum(v);
+  __shared__ int is_last;
+  if (threadIdx.x == 0) {
+    partials[slot] = bs;
+    __threadfence();
+    const unsigned t = atomicAdd(counter, 1u);
+    is_last = (t == unsigned(nblocks - 1));
+  }
+  __syncthreads();
+  if (is_last) {
+    __threadfence();
+    double s = 0.0;
+    for (int k = threadIdx.x; k < nblocks; k += blockDim.x)
+      s += __hip_atomic_load(&partials[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s = block_sum(s);
+    if (threadIdx.x == 0) {
+      fin(s);
+      __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __threadfence();
+    }
+  }
+}
+
+// Finisher of <p, DDdag p>: alpha = rsq / pv with the reference's breakdown
+// checks (Fermions.cpp:573-586).
+struct FinPV {
+  CGState* st;
+  __device__ void operator()(double pv) const {
+    st->pv = pv;
+    if (!isfinite(pv) || fabs(pv) < 1e-30) {
+      st->status = SFH_E_BREAKDOWN;
+      st->done = 1;
+      return;
+    }
+    const double a = st->rsq / pv;
+    if (!isfinite(a)) {
+      st->status = SFH_E_NONFINITE_ALPHA;
+      st->done = 1;
+      return;
+    }
+    st->alpha = a;
+  }
+};
+
+// Finisher of |r|^2 after x += alpha p, r -= alpha v: beta, convergence and
+// the iteration budget (Fermions.cpp:568, :594-600, :609-614).  In mixed
+// precision it also pauses the solve when a reliable update is due or the
+// iterated residual meets the target, for the host to run the FP64 update.
+struct FinRR {
+  CGState* st;
+  __device__ void operator()(double rr) const {
+    const double b = rr / st->rsq;
+    st->iter += 1;
+    st->rsq_prev = st->rsq;
+    st->rsq = rr;
+    if (!isfinite(b)) {
+      st->status = SFH_E_NONFINITE_BETA;
+      st->done = 1;
+      return;
+    }
+    st->beta = b;
+    if (st->mixed) {
+      const double r = sqrt(rr);
+      if (rr < st->target) {
+        st->pause_reason = 2;
+        st->done = 1;
+      } else if (r < st->delta * st->rmax) {
+        st->pause_reason = 1;
+        st->done = 1;
+      } else if (st->iter >= st->max_iter) {
+        st->status = SFH_E_NOCONV;
+        st->done = 1;
+      }
+      if (r > st->rmax) st->rmax = r;
+      return;
+    }
+    if (rr < st->target) {
+      st->done = 1;
+    } else if (st->iter >= st->max_iter) {
+      st->status = SFH_E_NOCONV;
+      st->done = 1;
+    }
+  }
+};
+
+struct FinStore {
+  double* out;
+  __device__ void operator()(double s) const { *out = s; }
+};
+
+}  // namespace sfh

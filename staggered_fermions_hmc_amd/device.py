@@ -1,0 +1,266 @@
+"""Object wrapper over the C ABI: a context (one HIP stream), device-resident
+gauge fields and spinor fields.  All host arrays are float64 in the
+reference's AoS layouts (links V*4*18, spinors V*6)."""
+import ctypes
+
+import numpy as np
+
+from . import _abi
+from ._abi import SfhError, f64, ptr
+
+# Reference exception classes by status (Fermions.cpp: std::runtime_error for
+# CG and the hermiticity guard, std::invalid_argument for the exponential).
+_EXC = {5: RuntimeError, 6: RuntimeError, 7: RuntimeError, 8: RuntimeError, 9: RuntimeError,
+        10: ValueError, 11: ValueError, 1: ValueError}
+
+
+class SolverError(SfhError):
+    pass
+
+
+def _raise(status, msg):
+    base = _EXC.get(status)
+    if base is None:
+        raise SfhError(status, msg)
+    cls = type(f"Sfh{base.__name__}", (SfhError, base), {})
+    raise cls(status, msg)
+
+
+class Context:
+    def __init__(self, device=0):
+        self._lib = _abi.lib()
+        h = ctypes.c_void_p()
+        st = self._lib.sfh_create(int(device), ctypes.byref(h))
+        if st:
+            raise SfhError(st, "sfh_create failed (is a GPU visible?)")
+        self.h = h
+
+    def check(self, st):
+        if st:
+            msg = self._lib.sfh_last_error(self.h)
+            _raise(st, msg.decode() if msg else "")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.sfh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def synchronize(self):
+        self.check(self._lib.sfh_synchronize(self.h))
+
+    def set_profiling(self, on):
+        self.check(self._lib.sfh_set_profiling(self.h, int(bool(on))))
+
+    def kernel_time(self, name):
+        ms = ctypes.c_double()
+        n = ctypes.c_long()
+        self.check(self._lib.sfh_kernel_time(self.h, name.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    def inner(self, a, b):
+        a = f64(a); b = f64(b)
+        assert a.size == b.size
+        out = ctypes.c_double()
+        self.check(self._lib.sfh_inner(self.h, a.size, ptr(a), ptr(b), ctypes.byref(out)))
+        return out.value
+
+    def exp_iq(self, Q18):
+        Q = f64(Q18)
+        out = np.empty(18)
+        self.check(self._lib.sfh_exp_iq(self.h, ptr(Q), ptr(out)))
+        return out
+
+
+class Gauge:
+    """Device-resident links for one lattice (dims = L0, L1, L2, T)."""
+
+    def __init__(self, ctx, dims, links=None):
+        self.ctx = ctx
+        self.dims = tuple(int(d) for d in dims)
+        self.V = int(np.prod(self.dims))
+        h = ctypes.c_void_p()
+        ctx.check(ctx._lib.sfh_gauge_create(ctx.h, ptr(_abi.dims_arr(self.dims)), ctypes.byref(h)))
+        self.h = h
+        if links is not None:
+            self.upload(links)
+
+    def close(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            self.ctx._lib.sfh_gauge_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, links):
+        a = f64(links)
+        assert a.size == self.V * 72
+        self.ctx.check(self.ctx._lib.sfh_gauge_upload(self.ctx.h, self.h, ptr(a)))
+
+    def download(self):
+        out = np.empty(self.V * 72)
+        self.ctx.check(self.ctx._lib.sfh_gauge_download(self.ctx.h, self.h, ptr(out)))
+        return out
+
+    def _spinor(self, a):
+        a = f64(a)
+        assert a.size == self.V * 6, (a.size, self.V * 6)
+        return a
+
+    def apply_d(self, mass, psi, dagger=False):
+        a = self._spinor(psi)
+        out = np.empty_like(a)
+        self.ctx.check(self.ctx._lib.sfh_apply_d(self.ctx.h, self.h, mass, int(bool(dagger)), ptr(a), ptr(out)))
+        return out
+
+    def apply_ddag(self, mass, psi):
+        a = self._spinor(psi)
+        out = np.empty_like(a)
+        self.ctx.check(self.ctx._lib.sfh_apply_ddag(self.ctx.h, self.h, mass, ptr(a), ptr(out)))
+        return out
+
+    def cg(self, phi, mass, tol=1e-12, max_iter=50000, precision="f64", reliable_delta=0.1,
+           true_residual=False):
+        a = self._spinor(phi)
+        x = np.empty_like(a)
+        prm = _abi.CGParams(mass, tol, max_iter, _abi.PREC[precision], reliable_delta, int(bool(true_residual)))
+        stats = _abi.CGStats()
+        self.ctx.check(self.ctx._lib.sfh_cg(self.ctx.h, self.h, ctypes.byref(prm), ptr(a), ptr(x),
+                                            ctypes.byref(stats)))
+        return x, stats.as_dict()
+
+    def cg_field(self, phi_field, x_field, mass, tol=1e-12, max_iter=50000, precision="f64", reliable_delta=0.1,
+                 true_residual=False):
+        prm = _abi.CGParams(mass, tol, max_iter, _abi.PREC[precision], reliable_delta, int(bool(true_residual)))
+        stats = _abi.CGStats()
+        self.ctx.check(self.ctx._lib.sfh_cg_field(self.ctx.h, self.h, ctypes.byref(prm), phi_field.h,
+                                                  x_field.h, ctypes.byref(stats)))
+        return stats.as_dict()
+
+    def force(self, beta, c11, c12, X=None, Y=None, which=3):
+        out = np.empty(self.V * 72)
+        Xa = self._spinor(X) if X is not None else None
+        Ya = self._spinor(Y) if Y is not None else None
+        self.ctx.check(self.ctx._lib.sfh_force(self.ctx.h, self.h, beta, c11, c12,
+                                               ptr(Xa) if Xa is not None else None,
+                                               ptr(Ya) if Ya is not None else None, int(which), ptr(out)))
+        return out
+
+    def action(self, beta, c11, c12):
+        out = ctypes.c_double()
+        self.ctx.check(self.ctx._lib.sfh_action(self.ctx.h, self.h, beta, c11, c12, ctypes.byref(out)))
+        return out.value
+
+    def avg_plaq(self):
+        out = ctypes.c_double()
+        self.ctx.check(self.ctx._lib.sfh_avg_plaq(self.ctx.h, self.h, ctypes.byref(out)))
+        return out.value
+
+    def tr_p2(self, P):
+        a = f64(P)
+        assert a.size == self.V * 72
+        out = ctypes.c_double()
+        self.ctx.check(self.ctx._lib.sfh_tr_p2(self.ctx.h, self.h, ptr(a), ctypes.byref(out)))
+        return out.value
+
+    def link_update(self, eps, P, check=True):
+        a = f64(P)
+        assert a.size == self.V * 72
+        self.ctx.check(self.ctx._lib.sfh_link_update(self.ctx.h, self.h, eps, ptr(a), int(bool(check))))
+
+    def md_params(self, *, mass=0.0, tol=1e-12, max_iter=50000, precision="f64", beta=0.0, c11=1.0, c12=0.0,
+                  eps=0.01, nsteps=100, which=3, check=True):
+        return _abi.MDParams(mass, tol, max_iter, _abi.PREC[precision], beta, c11, c12, eps, nsteps, which,
+                             int(bool(check)), 0)
+
+    def leapfrog(self, P, phi=None, **kw):
+        prm = self.md_params(**kw)
+        Pa = f64(P).copy()
+        X = np.zeros(self.V * 6)
+        ph = self._spinor(phi) if phi is not None else None
+        self.ctx.check(self.ctx._lib.sfh_leapfrog(self.ctx.h, self.h, ctypes.byref(prm),
+                                                  ptr(ph) if ph is not None else None, ptr(Pa), ptr(X)))
+        return Pa, X
+
+    def hmc_trajectory(self, chi, P0, r_uniform, **kw):
+        prm = self.md_params(**kw)
+        c = self._spinor(chi)
+        P = f64(P0)
+        H = np.zeros(7)
+        acc = ctypes.c_int()
+        self.ctx.check(self.ctx._lib.sfh_hmc_trajectory(self.ctx.h, self.h, ctypes.byref(prm), ptr(c), ptr(P),
+                                                        float(r_uniform), ptr(H), ctypes.byref(acc)))
+        return H, bool(acc.value)
+
+
+class Field:
+    """Device-resident spinor for the lattice of a Gauge."""
+
+    def __init__(self, gauge, host=None):
+        self.ctx = gauge.ctx
+        self.V = gauge.V
+        h = ctypes.c_void_p()
+        self.ctx.check(self.ctx._lib.sfh_field_create(self.ctx.h, gauge.h, ctypes.byref(h)))
+        self.h = h
+        if host is not None:
+            self.upload(host)
+
+    def upload(self, host):
+        a = f64(host)
+        assert a.size == self.V * 6
+        self.ctx.check(self.ctx._lib.sfh_field_upload(self.ctx.h, self.h, ptr(a)))
+
+    def download(self):
+        out = np.empty(self.V * 6)
+        self.ctx.check(self.ctx._lib.sfh_field_download(self.ctx.h, self.h, ptr(out)))
+        return out
+
+    def close(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            self.ctx._lib.sfh_field_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def random_links(dims, eps, seed):
+    out = np.empty(int(np.prod(dims)) * 72)
+    st = _abi.lib().sfh_random_links(ptr(_abi.dims_arr(dims)), float(eps), int(seed), ptr(out))
+    if st:
+        raise SfhError(st, "sfh_random_links")
+    return out
+
+
+def random_spinor(dims, seed, sigma=np.sqrt(0.5)):
+    out = np.empty(int(np.prod(dims)) * 6)
+    st = _abi.lib().sfh_random_spinor(ptr(_abi.dims_arr(dims)), float(sigma), int(seed), ptr(out))
+    if st:
+        raise SfhError(st, "sfh_random_spinor")
+    return out
+
+
+def random_momenta(dims, seed):
+    out = np.empty(int(np.prod(dims)) * 72)
+    st = _abi.lib().sfh_random_momenta(ptr(_abi.dims_arr(dims)), int(seed), ptr(out))
+    if st:
+        raise SfhError(st, "sfh_random_momenta")
+    return out

@@ -1,0 +1,81 @@
+"""CPU checks of the C ABI library: it loads, exports every entry point that
+include/sfh.h declares, and its host-side generators produce the documented
+distributions.  No GPU compute is called here."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from staggered_fermions_hmc_amd import _abi
+from staggered_fermions_hmc_amd import device as dev
+import fields as F
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "sfh.h")
+
+
+def declared_symbols():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(sfh_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_and_binding_agree():
+    assert set(declared_symbols()) == set(_abi.SIGNATURES)
+
+
+def test_library_exports_every_declared_symbol():
+    L = _abi.lib()
+    for name in declared_symbols():
+        assert hasattr(L, name), name
+    assert L.sfh_abi_version() == 1
+    assert L.sfh_status_string(8) == b"CG did not converge"
+
+
+def test_random_links_are_su3_and_seeded():
+    dims = (4, 4, 4, 6)
+    U = dev.random_links(dims, 0.3, seed=7)
+    assert np.array_equal(U, dev.random_links(dims, 0.3, seed=7))
+    assert not np.array_equal(U, dev.random_links(dims, 0.3, seed=8))
+    M = F.flat_to_mat(U).reshape(-1, 3, 3)
+    assert np.abs(M @ M.conj().transpose(0, 2, 1) - np.eye(3)).max() < 1e-14
+    assert np.abs(np.linalg.det(M) - 1).max() < 1e-14
+    # U = exp(i eps H): recover eps H from the eigenphases and check it is
+    # distributed as sum_a r_a lambda_a / 2 with r_a ~ N(0, 1).
+    w, v = np.linalg.eig(M)
+    th = np.angle(w)
+    Hs = np.einsum("lij,lj,ljk->lik", v, th, np.linalg.inv(v)) / 0.3
+    r = np.stack([np.einsum("lij,ji->l", Hs, F.GELL_MANN[a]).real for a in range(8)], axis=1)
+    assert abs(r.mean()) < 0.05 and abs(r.std() - 1.0) < 0.05
+
+
+def test_cold_links_for_zero_eps():
+    U = dev.random_links((2, 2, 2, 2), 0.0, seed=1)
+    assert np.array_equal(U, F.cold_links((2, 2, 2, 2)))
+
+
+def test_random_spinor_and_momenta_statistics():
+    dims = (8, 8, 8, 8)
+    s = dev.random_spinor(dims, seed=3)
+    assert abs(s.mean()) < 0.01 and abs(s.std() - np.sqrt(0.5)) < 0.01
+    P = dev.random_momenta(dims, seed=4)
+    Pm = F.flat_to_mat(P).reshape(-1, 3, 3)
+    assert np.abs(Pm - Pm.conj().transpose(0, 2, 1)).max() == 0.0
+    assert np.abs(np.einsum("lii->l", Pm)).max() < 1e-15
+    # Tr P^2 = (1/2) sum_a p_a^2 -> mean 4 per link
+    assert abs(np.einsum("lij,lji->l", Pm, Pm).real.mean() - 4.0) < 0.05
+
+
+def test_context_without_gpu_reports_no_device():
+    import ctypes
+    h = ctypes.c_void_p()
+    st = _abi.lib().sfh_create(0, ctypes.byref(h))
+    if st == 0:
+        _abi.lib().sfh_destroy(h)
+        pytest.skip("a GPU is visible")
+    assert st in (3, 4)
+
+
+def test_invalid_dims_rejected():
+    out = np.empty(3 * 2 * 2 * 2 * 72)
+    assert _abi.lib().sfh_random_links(_abi.ptr(_abi.dims_arr((3, 2, 2, 2))), 0.1, 1, _abi.ptr(out)) == 1

@@ -1,0 +1,295 @@
+"""Parity of the HIP path (through the C ABI) with the FP64 CPU oracle on the
+same seeded inputs, plus size-independent properties at the benchmark sizes.
+
+Tolerances: the FP64 kernels agree with the oracle to rounding (1e-13
+relative elementwise for D, forces, exp); the CG solution to the solve
+tolerance; mixed precision to its FP64 true residual."""
+import numpy as np
+import pytest
+
+from oracle import oracle as oc
+from staggered_fermions_hmc_amd import device as dev
+import fields as F
+
+pytestmark = pytest.mark.gpu
+
+DIMS_SET = [(4, 4, 4, 4), (4, 4, 4, 8), (6, 4, 4, 2), (2, 2, 2, 4), (8, 6, 4, 4), (12, 4, 6, 8)]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = dev.Context(0)
+    yield c
+    c.close()
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(1e-300, np.abs(b).max())
+
+
+# ---------------------------------------------------------------- layout
+@pytest.mark.parametrize("dims", DIMS_SET)
+def test_layout_round_trip_exact(ctx, dims):
+    U = dev.random_links(dims, 0.5, seed=1)
+    g = dev.Gauge(ctx, dims, U)
+    assert np.array_equal(g.download(), U)
+    s = dev.random_spinor(dims, seed=2)
+    f = dev.Field(g, s)
+    assert np.array_equal(f.download(), s)
+
+
+# ---------------------------------------------------------------- Dirac operator
+@pytest.mark.parametrize("dims", DIMS_SET)
+@pytest.mark.parametrize("dagger", [False, True])
+def test_dslash_matches_oracle(ctx, dims, dagger):
+    m = 0.13
+    U = dev.random_links(dims, 0.4, seed=3)
+    psi = dev.random_spinor(dims, seed=4)
+    g = dev.Gauge(ctx, dims, U)
+    out = g.apply_d(m, psi, dagger=dagger)
+    ref = oc.apply_d(dims, m, U, psi, dagger=dagger)
+    assert rel(out, ref) < 1e-14
+
+
+@pytest.mark.parametrize("dims", [(4, 4, 4, 4), (8, 8, 8, 8)])
+def test_ddag_matches_oracle(ctx, dims):
+    U = dev.random_links(dims, 0.3, seed=5)
+    psi = dev.random_spinor(dims, seed=6)
+    g = dev.Gauge(ctx, dims, U)
+    assert rel(g.apply_ddag(0.05, psi), oc.apply_ddag(dims, 0.05, U, psi)) < 1e-14
+
+
+def test_dslash_delta_source_stencil(ctx):
+    """The oracle's hand-derived stencil cases, on the device."""
+    dims = (4, 4, 4, 4)
+    U = F.random_links(dims, 0.6, seed=5)
+    g = dev.Gauge(ctx, dims, U)
+    for n0 in [(0, 0, 0, 3), (1, 1, 1, 0)]:
+        psi = np.zeros((256, 3), complex)
+        psi[F.site_index(dims, np.array(n0)), 1] = 0.3 - 0.8j
+        flat = F.vec_to_flat(psi)
+        for dg in (False, True):
+            assert rel(g.apply_d(0.37, flat, dagger=dg), oc.apply_d(dims, 0.37, U, flat, dagger=dg)) < 1e-15
+
+
+def test_free_field_closed_form_large(ctx):
+    """U = 1 at 16^3 x 32: D D^dag of an antiperiodic plane wave is
+    (m^2 + sum sin^2 p) times it."""
+    dims = (16, 16, 16, 32)
+    m = 0.01
+    k = (3, 5, 1, 7)
+    p = np.array([2 * np.pi * k[i] / dims[i] for i in range(3)] + [(2 * k[3] + 1) * np.pi / dims[3]])
+    c = F.coords(dims)
+    psi = np.exp(1j * (c @ p))[:, None] * np.array([0.6, -0.2j, 0.1 + 0.3j])[None, :]
+    g = dev.Gauge(ctx, dims, F.cold_links(dims))
+    out = F.flat_to_vec(g.apply_ddag(m, F.vec_to_flat(psi)))
+    lam = m * m + np.sum(np.sin(p) ** 2)
+    assert np.abs(out - lam * psi).max() < 1e-13
+
+
+def test_hop_antihermitian_at_32_4(ctx):
+    dims = (32, 32, 32, 32)
+    U = dev.random_links(dims, 0.3, seed=9)
+    g = dev.Gauge(ctx, dims, U)
+    a = dev.random_spinor(dims, seed=10)
+    b = dev.random_spinor(dims, seed=11)
+    Ka = g.apply_d(0.0, a)
+    Kb = g.apply_d(0.0, b)
+    ca, cb = F.flat_to_vec(a), F.flat_to_vec(b)
+    lhs = np.vdot(ca, F.flat_to_vec(Kb))
+    rhs = -np.vdot(F.flat_to_vec(Ka), cb)
+    assert abs(lhs - rhs) < 1e-10 * np.sqrt(a @ a * (b @ b))
+    # D^dag = eps D eps
+    es = np.repeat(F.eps_sign(dims), 6)
+    assert rel(g.apply_d(0.01, a, dagger=True), es * g.apply_d(0.01, es * a)) < 1e-14
+
+
+# ---------------------------------------------------------------- inner product
+def test_inner_product(ctx):
+    a = np.random.default_rng(1).standard_normal(100003)
+    b = np.random.default_rng(2).standard_normal(100003)
+    assert abs(ctx.inner(a, b) - oc.inner(a, b)) < 1e-10
+
+
+# ---------------------------------------------------------------- CG
+@pytest.mark.parametrize("dims,m", [((4, 4, 4, 4), 0.1), ((8, 8, 8, 8), 0.05), ((6, 4, 4, 2), 0.2)])
+def test_cg_f64_matches_oracle(ctx, dims, m):
+    U = dev.random_links(dims, 0.3, seed=12)
+    chi = dev.random_spinor(dims, seed=13)
+    phi = oc.apply_d(dims, m, U, chi)
+    g = dev.Gauge(ctx, dims, U)
+    x, st = g.cg(phi, m, tol=1e-10, precision="f64", true_residual=True)
+    xr, it, rsq, _ = oc.cg(dims, m, U, phi, tol=1e-10)
+    assert abs(st["iterations"] - it) <= 1
+    assert rel(x, xr) < 1e-9
+    assert st["true_rel_resid"] < 2e-10
+    # pseudofermion identity Y = D^dag X = chi
+    assert rel(g.apply_d(m, x, dagger=True), chi) < 1e-8
+
+
+@pytest.mark.parametrize("prec", ["f32", "f16"])
+@pytest.mark.parametrize("dims,m", [((4, 4, 4, 4), 0.1), ((8, 8, 8, 8), 0.05)])
+def test_cg_mixed_reaches_true_residual(ctx, dims, m, prec):
+    U = dev.random_links(dims, 0.3, seed=14)
+    phi = oc.apply_d(dims, m, U, dev.random_spinor(dims, seed=15))
+    g = dev.Gauge(ctx, dims, U)
+    x, st = g.cg(phi, m, tol=1e-10, precision=prec)
+    r = phi - oc.apply_ddag(dims, m, U, x)
+    true = np.sqrt(r @ r / (phi @ phi))
+    assert true < 1e-10, (true, st)
+    assert abs(st["true_rel_resid"] - true) < 1e-11
+    xr, *_ = oc.cg(dims, m, U, phi, tol=1e-12)
+    assert rel(x, xr) < 1e-8
+    assert st["reliable_updates"] >= 1
+
+
+def test_cg_error_paths(ctx):
+    dims = (4, 4, 4, 4)
+    U = dev.random_links(dims, 0.3, seed=16)
+    g = dev.Gauge(ctx, dims, U)
+    phi = dev.random_spinor(dims, seed=17)
+    x, st = g.cg(np.zeros_like(phi), 0.1)
+    assert st["iterations"] == 0 and not x.any()
+    with pytest.raises(RuntimeError, match="failed to converge after 3 iterations"):
+        g.cg(phi, 0.1, max_iter=3)
+    bad = phi.copy()
+    bad[7] = np.nan
+    with pytest.raises(RuntimeError, match="CG breakdown"):
+        g.cg(bad, 0.1)
+    # the context stays usable after an error
+    x, st = g.cg(phi, 0.1, tol=1e-10)
+    assert st["iterations"] > 0
+
+
+def test_cg_16_4_both_precisions(ctx):
+    """The 16^4, m=0.05 solve: FP64 and FP32-inner reach a true 1e-10."""
+    dims = (16, 16, 16, 16)
+    m = 0.05
+    U = dev.random_links(dims, 0.3, seed=21)
+    g = dev.Gauge(ctx, dims, U)
+    phi = dev.random_spinor(dims, seed=22)
+    x64, s64 = g.cg(phi, m, tol=1e-10, precision="f64", true_residual=True)
+    x32, s32 = g.cg(phi, m, tol=1e-10, precision="f32")
+    x16, s16 = g.cg(phi, m, tol=1e-10, precision="f16")
+    assert s64["true_rel_resid"] < 2e-10
+    assert s32["true_rel_resid"] < 1e-10
+    assert s16["true_rel_resid"] < 1e-10
+    assert rel(x32, x64) < 1e-8 and rel(x16, x64) < 1e-8
+
+
+# ---------------------------------------------------------------- forces / actions
+@pytest.mark.parametrize("dims", [(4, 4, 4, 4), (6, 4, 4, 2), (4, 6, 8, 4)])
+@pytest.mark.parametrize("c11,c12", [(1.0, 0.0), (5.0 / 3.0, -1.0 / 12.0), (0.0, 1.0)])
+def test_gauge_force_and_action_match_oracle(ctx, dims, c11, c12):
+    U = dev.random_links(dims, 0.5, seed=23)
+    g = dev.Gauge(ctx, dims, U)
+    Fg = g.force(4.0, c11, c12, which=1)
+    Fr = oc.force(dims, U, None, None, 4.0, c11, c12, which=1)
+    assert np.abs(Fg - Fr).max() < 1e-13 * max(1.0, np.abs(Fr).max())
+    S = g.action(4.0, c11, c12)
+    assert abs(S - oc.action(dims, U, 4.0, c11, c12)) < 1e-11 * max(1.0, abs(S))
+    assert abs(g.avg_plaq() - oc.avg_plaq(dims, U)) < 1e-13
+
+
+def test_fermion_and_full_force_match_oracle(ctx):
+    dims = (4, 4, 4, 8)
+    m = 0.1
+    U = dev.random_links(dims, 0.4, seed=24)
+    X = dev.random_spinor(dims, seed=25)
+    Y = dev.random_spinor(dims, seed=26)
+    g = dev.Gauge(ctx, dims, U)
+    for which in (2, 3):
+        Fd = g.force(5.6, 5 / 3, -1 / 12, X, Y, which=which)
+        Fr = oc.force(dims, U, X, Y, 5.6, 5 / 3, -1 / 12, which=which)
+        assert np.abs(Fd - Fr).max() < 1e-13 * max(1.0, np.abs(Fr).max())
+
+
+def test_cold_known_answers_on_device(ctx):
+    dims = (4, 4, 4, 4)
+    g = dev.Gauge(ctx, dims, F.cold_links(dims))
+    V = 256
+    assert abs(g.action(4.1, 5 / 3, -1 / 12) - (-4.1 * (6 * V * 5 / 3 + 12 * V * (-1 / 12)))) < 1e-10
+    assert abs(g.avg_plaq() - 6.0) < 1e-13
+    assert np.abs(g.force(4.1, 5 / 3, -1 / 12, which=1)).max() < 1e-14
+
+
+def test_tr_p2(ctx):
+    dims = (4, 4, 4, 6)
+    P = dev.random_momenta(dims, seed=27)
+    g = dev.Gauge(ctx, dims)
+    assert abs(g.tr_p2(P) - oc.tr_p2(dims, P)) < 1e-10
+
+
+# ---------------------------------------------------------------- exp and link update
+def test_exp_matches_oracle(ctx):
+    rng = np.random.default_rng(28)
+    for scale in (1e-7, 1e-3, 0.05, 0.7, 3.0):
+        for _ in range(5):
+            Q = F.mat_to_flat(scale * F.random_algebra(rng, ()))
+            assert np.abs(ctx.exp_iq(Q) - oc.exp_iq(Q)).max() < 1e-14
+    for Q in [np.diag([0.4, 0.4, -0.8]), np.zeros((3, 3)), np.diag([-0.4, -0.4, 0.8])]:
+        Qf = F.mat_to_flat(Q.astype(complex))
+        assert np.abs(ctx.exp_iq(Qf) - oc.exp_iq(Qf)).max() < 1e-15
+    with pytest.raises(ValueError, match="imaginary trace"):
+        ctx.exp_iq(F.mat_to_flat(np.diag([0.5j, 0.1, -0.1])))
+
+
+def test_link_update_matches_oracle(ctx):
+    dims = (4, 4, 4, 4)
+    U = dev.random_links(dims, 0.3, seed=29)
+    P = dev.random_momenta(dims, seed=30)
+    g = dev.Gauge(ctx, dims, U)
+    g.link_update(0.1, P)
+    assert np.abs(g.download() - oc.link_update(dims, 0.1, P, U)).max() < 1e-14
+    bad = P.copy()
+    bad[0] += 1.0
+    with pytest.raises(RuntimeError, match="not Hermitian"):
+        g.link_update(0.1, bad)
+
+
+# ---------------------------------------------------------------- integrator
+@pytest.mark.parametrize("which", [1, 2, 3])
+def test_leapfrog_matches_oracle(ctx, which):
+    dims = (4, 4, 4, 4)
+    m = 0.2
+    U = dev.random_links(dims, 0.3, seed=31)
+    P = dev.random_momenta(dims, seed=32)
+    phi = oc.apply_d(dims, m, U, dev.random_spinor(dims, seed=33))
+    kw = dict(beta=4.0, c11=5 / 3, c12=-1 / 12, eps=0.05, nsteps=4, which=which, tol=1e-12)
+    Ur, Pr, Xr = oc.leapfrog(dims, m, U, P, phi, **kw)
+    g = dev.Gauge(ctx, dims, U)
+    Pd, Xd = g.leapfrog(P, phi, mass=m, **kw)
+    assert np.abs(g.download() - Ur).max() < 1e-11
+    assert np.abs(Pd - Pr).max() < 1e-9 * max(1.0, np.abs(Pr).max())
+    if which & 2:
+        assert rel(Xd, Xr) < 1e-8
+
+
+def test_leapfrog_reversible_on_device(ctx):
+    dims = (8, 8, 8, 8)
+    m = 0.1
+    U0 = dev.random_links(dims, 0.3, seed=34)
+    P0 = dev.random_momenta(dims, seed=35)
+    g = dev.Gauge(ctx, dims, U0)
+    phi = g.apply_d(m, dev.random_spinor(dims, seed=36))
+    kw = dict(mass=m, beta=5.6, c11=1.0, c12=0.0, eps=0.05, nsteps=5, which=3, tol=1e-13)
+    P1, _ = g.leapfrog(P0, phi, **kw)
+    P2, _ = g.leapfrog(-P1, phi, **kw)
+    assert np.abs(g.download() - U0).max() < 1e-10
+    assert np.abs(-P2 - P0).max() < 1e-9
+
+
+def test_hmc_trajectory_matches_oracle(ctx):
+    dims = (4, 4, 4, 4)
+    m = 0.1
+    U = dev.random_links(dims, 0.3, seed=37)
+    P = dev.random_momenta(dims, seed=38)
+    chi = dev.random_spinor(dims, seed=39)
+    kw = dict(beta=5.6, c11=1.0, c12=0.0, eps=0.1, nsteps=10, tol=1e-12)
+    Ur, Hr, accr = oc.hmc_trajectory(dims, m, U, chi, P, 0.3, **kw)
+    g = dev.Gauge(ctx, dims, U)
+    H, acc = g.hmc_trajectory(chi, P, 0.3, mass=m, **kw)
+    assert np.abs(H[:6] - Hr[:6]).max() < 1e-8 * np.abs(Hr[:6]).max()
+    assert abs(H[6] - Hr[6]) < 1e-7
+    assert acc == accr
+    assert np.abs(g.download() - Ur).max() < 1e-10
