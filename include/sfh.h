@@ -119,6 +119,17 @@ int sfh_synchronize(sfh_ctx* ctx);
 int sfh_set_profiling(sfh_ctx* ctx, int on);
 int sfh_kernel_time(sfh_ctx* ctx, const char* name, double* total_ms, long* launches);
 
+/* Hop kernel form, per precision (0 f64, 1 f32, 2 f16): 0 = one lane per
+ * site, 1 = four lanes per site (one per direction), -1 = choose by lattice
+ * size (the default).  Process-wide.  No counterpart in the reference: a
+ * tuning knob of this build. */
+int sfh_set_hop_variant(int precision, int variant);
+/* Times each hop form on g's lattice (n launches each, HIP events on the
+ * context stream), fixes the faster one for that precision and returns it
+ * with its average launch time in microseconds.  No counterpart in the
+ * reference. */
+int sfh_tune_hop(sfh_ctx* ctx, const sfh_gauge* g, int precision, int n, int* variant, double* us);
+
 /* ---------------------------------------------------------------- gauge field
  * Device-resident links for one lattice.  dims = {L0, L1, L2, L3}, all even,
  * L3 = time.  Replaces Link_array (Lattice.h:11) held in host memory. */
@@ -178,6 +189,11 @@ int sfh_force(sfh_ctx* ctx, const sfh_gauge* g, double beta, double c11, double 
 int sfh_action(sfh_ctx* ctx, const sfh_gauge* g, double beta, double c11, double c12, double* out);
 /* Replaces compute_avg_plaq (Lattice.cpp:784-804), same normalisation. */
 int sfh_avg_plaq(sfh_ctx* ctx, const sfh_gauge* g, double* out);
+/* Polyakov loop trace Tr prod_t U_3(x,y,z,t) at every spatial site, complex
+ * (re, im) pairs at [(x*L1 + y)*L2 + z].  Replaces precompute_polyakov_grid /
+ * poly_loop_at_spat_coord (Lattice.cpp:849-884); the correlators of
+ * Lattice.cpp:888-948 are cheap host sums over this grid. */
+int sfh_polyakov_grid(sfh_ctx* ctx, const sfh_gauge* g, double* out);
 /* sum over links of ReTr(P^2).  Replaces compute_Tr_P2 (Fermions.cpp:662-678). */
 int sfh_tr_p2(sfh_ctx* ctx, const sfh_gauge* g, const double* P, double* out);
 /* U <- exp(i eps P) U on every device link (Fermions.cpp:703-723, exponential

@@ -614,3 +614,27 @@ int oc_set_threads(int n) {
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// Polyakov loop traces, Lattice.cpp:849-884: Tr prod_{t=0}^{T-1} U_3(x,y,z,t)
+// (right multiplication from t = 0), at [(x*L1 + y)*L2 + z] as (re, im).
+int oc_polyakov_grid(const int* dims, const double* U, double* out) {
+  Geo g(dims);
+  const int ns = g.L[0] * g.L[1] * g.L[2];
+#pragma omp parallel for
+  for (int k = 0; k < ns; ++k) {
+    const int x = k / (g.L[1] * g.L[2]), y = (k / g.L[2]) % g.L[1], z = k % g.L[2];
+    M3 w = eye3();
+    for (int t = 0; t < g.L[3]; ++t) {
+      const int n[4] = {x, y, z, t};
+      w = mul(w, load_link(U, g.index(n), 3));
+    }
+    const cd tr = trace(w);
+    out[2 * k] = tr.real();
+    out[2 * k + 1] = tr.imag();
+  }
+  return OK;
+}
+
+}  // extern "C"

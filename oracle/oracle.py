@@ -49,6 +49,7 @@ def lib():
             "oc_leapfrog": (_I, [_P, _D, _D, _I, _D, _D, _D, _D, _I, _I, _I, _P, _P, _P, _P]),
             "oc_hmc_trajectory": (_I, [_P, _D, _D, _I, _D, _D, _D, _D, _I, _P, _P, _D, _P, _P, _P]),
             "oc_set_threads": (_I, [_I]),
+            "oc_polyakov_grid": (_I, [_P, _P, _P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -196,3 +197,10 @@ def hmc_trajectory(dims, mass, U, chi, P, r_uniform, *, beta, c11, c12, eps, nst
     if st:
         raise OracleError(st, "hmc_trajectory")
     return U, H, bool(acc.value)
+
+
+def polyakov_grid(dims, U):
+    d = _dims(dims); U = _f64(U)
+    out = np.empty(2 * int(d[0] * d[1] * d[2]))
+    lib().oc_polyakov_grid(_p(d), _p(U), _p(out))
+    return out.view(np.complex128)

@@ -58,6 +58,8 @@ SIGNATURES = {
     "sfh_synchronize": (_I, [_P]),
     "sfh_set_profiling": (_I, [_P, _I]),
     "sfh_kernel_time": (_I, [_P, ctypes.c_char_p, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_long)]),
+    "sfh_set_hop_variant": (_I, [_I, _I]),
+    "sfh_tune_hop": (_I, [_P, _P, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_D)]),
     "sfh_gauge_create": (_I, [_P, _P, ctypes.POINTER(_P)]),
     "sfh_gauge_destroy": (_I, [_P, _P]),
     "sfh_gauge_upload": (_I, [_P, _P, _P]),
@@ -79,6 +81,7 @@ SIGNATURES = {
     "sfh_action": (_I, [_P, _P, _D, _D, _D, ctypes.POINTER(_D)]),
     "sfh_avg_plaq": (_I, [_P, _P, ctypes.POINTER(_D)]),
     "sfh_tr_p2": (_I, [_P, _P, _P, ctypes.POINTER(_D)]),
+    "sfh_polyakov_grid": (_I, [_P, _P, _P]),
     "sfh_link_update": (_I, [_P, _P, _D, _P, _I]),
     "sfh_exp_iq": (_I, [_P, _P, _P]),
     "sfh_leapfrog": (_I, [_P, _P, ctypes.POINTER(MDParams), _P, _P, _P]),

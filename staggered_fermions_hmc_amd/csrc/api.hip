@@ -284,6 +284,69 @@ int sfh_kernel_time(sfh_ctx* c, const char* name, double* total_ms, long* launch
   return SFH_OK;
 }
 
+int sfh_set_hop_variant(int precision, int variant) {
+  if (precision < 0 || precision > 2 || variant < -1 || variant > 1) return SFH_E_INVALID;
+  set_hop_variant(precision, variant);
+  return SFH_OK;
+}
+
+int sfh_tune_hop(sfh_ctx* c, const sfh_gauge* g, int precision, int n, int* variant, double* us) {
+  if (!c || !g || precision < 0 || precision > 2 || n < 1) return SFH_E_INVALID;
+  Guard gd(c);
+  const Geom& gg = g->g;
+  const size_t nc = 3 * (size_t)gg.V;
+  void* a = scratch(c, "tune_a", nc * sizeof(double2));
+  void* b = scratch(c, "tune_b", nc * sizeof(double2));
+  float* na = static_cast<float*>(scratch(c, "tune_na", gg.V * sizeof(float)));
+  float* nb = static_cast<float*>(scratch(c, "tune_nb", gg.V * sizeof(float)));
+  if (!a || !b || !na || !nb) return SFH_E_NOMEM;
+  sfh_gauge* gm = const_cast<sfh_gauge*>(g);
+  if (precision == 1 && !gm->lp32_valid) {
+    if (!gm->lp.u32 && hipMalloc(&gm->lp.u32, 36 * (size_t)gg.V * sizeof(float2)) != hipSuccess)
+      return set_err(c, SFH_E_NOMEM, "FP32 link copy");
+    links_to_f32(c->stream, gg, gm->U, gm->lp.u32);
+    gm->lp32_valid = true;
+  }
+  if (precision == 2 && !gm->lp16_valid) {
+    if (!gm->lp.u16 && hipMalloc(&gm->lp.u16, 36 * (size_t)gg.V * sizeof(__half2)) != hipSuccess)
+      return set_err(c, SFH_E_NOMEM, "FP16 link copy");
+    links_to_f16(c->stream, gg, gm->U, gm->lp.u16);
+    gm->lp16_valid = true;
+  }
+  hipMemsetAsync(a, 0, nc * sizeof(double2), c->stream);
+  hipMemsetAsync(na, 0, gg.V * sizeof(float), c->stream);
+  double best_us = 1e30;
+  int best = 0;
+  for (int v = 0; v < 2; ++v) {
+    set_hop_variant(precision, v);
+    auto launch = [&]() {
+      if (precision == 0)
+        hop_f64(c->stream, gg, g->U, (const double2*)a, (double2*)b, 0.1, 1.0, nullptr, nullptr, nullptr);
+      else if (precision == 1)
+        hop_f32(c->stream, gg, g->lp.u32, (const float2*)a, (float2*)b, 0.1f, 1.0f, nullptr, nullptr, nullptr);
+      else
+        hop_f16(c->stream, gg, g->lp.u16, (const __half2*)a, na, (__half2*)b, nb, 0.1f, 1.0f, nullptr, nullptr,
+                nullptr, nullptr);
+    };
+    launch();
+    hipEventRecord(c->ev0, c->stream);
+    for (int k = 0; k < n; ++k) launch();
+    hipEventRecord(c->ev1, c->stream);
+    if (int e = hip_check(c, hipEventSynchronize(c->ev1), "hop tuning")) return e;
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    const double t = 1e3 * ms / n;
+    if (t < best_us) {
+      best_us = t;
+      best = v;
+    }
+  }
+  set_hop_variant(precision, best);
+  if (variant) *variant = best;
+  if (us) *us = best_us;
+  return SFH_OK;
+}
+
 // ---------------------------------------------------------------- gauge
 int sfh_gauge_create(sfh_ctx* c, const int dims[4], sfh_gauge** out) {
   if (!c || !out) return SFH_E_INVALID;
@@ -497,6 +560,12 @@ int sfh_avg_plaq(sfh_ctx* c, const sfh_gauge* g, double* out) {
   Guard gd(c);
   *out = avg_plaq(c, g->g, g->U);
   return std::isnan(*out) ? SFH_E_HIP : SFH_OK;
+}
+
+int sfh_polyakov_grid(sfh_ctx* c, const sfh_gauge* g, double* out) {
+  if (!c || !g || !out) return SFH_E_INVALID;
+  Guard gd(c);
+  return polyakov_grid(c, g->g, g->U, out);
 }
 
 int sfh_tr_p2(sfh_ctx* c, const sfh_gauge* g, const double* P, double* out) {

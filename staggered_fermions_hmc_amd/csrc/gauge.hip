@@ -279,6 +279,37 @@ double avg_plaq(sfh_ctx* c, const Geom& g, const double2* U) {
 }
 double tr_p2(sfh_ctx* c, const Geom& g, const double2* P) { return link_sum(c, g, P, 0, 0, 0, 2); }
 
+// ------------------------------------------------------------------ Polyakov loops
+// Tr prod_t U_3(x, y, z, t), right-multiplied in t order from t = 0
+// (Lattice.cpp:849-866), one thread per spatial site; out is complex
+// [(x*L1 + y)*L2 + z] as in precompute_polyakov_grid (Lattice.cpp:870-884).
+__global__ void k_polyakov(Geom g, const double2* __restrict__ U, double2* __restrict__ out) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ns = g.L[0] * g.L[1] * g.L[2];
+  if (s >= ns) return;
+  Pos x;
+  x.n[2] = s % g.L[2];
+  x.n[1] = (s / g.L[2]) % g.L[1];
+  x.n[0] = s / (g.L[2] * g.L[1]);
+  x.n[3] = 0;
+  M w = ldU(U, g, x, 3);
+  for (int t = 1; t < g.L[3]; ++t) {
+    x.n[3] = t;
+    w = mmul(w, ldU(U, g, x, 3));
+  }
+  const C tr = mtrace(w);
+  out[s] = make_double2(tr.re, tr.im);
+}
+
+int polyakov_grid(sfh_ctx* c, const Geom& g, const double2* U, double* out) {
+  const int ns = g.L[0] * g.L[1] * g.L[2];
+  double2* d = static_cast<double2*>(scratch(c, "polyakov", ns * sizeof(double2)));
+  if (!d) return SFH_E_NOMEM;
+  hipLaunchKernelGGL(k_polyakov, dim3((ns + 127) / 128), dim3(128), 0, c->stream, g, U, d);
+  hipMemcpyAsync(out, d, ns * sizeof(double2), hipMemcpyDeviceToHost, c->stream);
+  return hip_check(c, hipStreamSynchronize(c->stream), "polyakov");
+}
+
 // ------------------------------------------------------------------ exponential and link update
 // exp(iQ) for hermitian traceless Q by the Cayley-Hamilton closed form of
 // Morningstar & Peardon (hep-lat/0311018), with the guards and small-Q branch

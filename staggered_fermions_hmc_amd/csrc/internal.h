@@ -147,6 +147,7 @@ double avg_plaq(sfh_ctx* c, const Geom& g, const double2* U);
 double tr_p2(sfh_ctx* c, const Geom& g, const double2* P);
 int link_update(sfh_ctx* c, const Geom& g, double2* U, const double2* P, double eps, int check);
 int exp_iq_one(sfh_ctx* c, const double* Q18, double* out18);
+int polyakov_grid(sfh_ctx* c, const Geom& g, const double2* U, double* out);
 
 // ------------------------------------------------------------------ solver (solver.cpp)
 int cg_solve(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* p, const double2* phi, double2* x,

@@ -61,7 +61,7 @@ class Context:
         self.check(self._lib.sfh_synchronize(self.h))
 
     def set_profiling(self, on):
-        self.check(self._lib.sfh_set_profiling(self.h, int(bool(on))))
+        self.check(self._lib.sfh_set_profiling(self.h, int(on)))
 
     def kernel_time(self, name):
         ms = ctypes.c_double()
@@ -161,6 +161,15 @@ class Gauge:
                                                ptr(Ya) if Ya is not None else None, int(which), ptr(out)))
         return out
 
+    def tune_hop(self, precision="f64", n=20):
+        """Time the hop forms on this lattice, keep the faster; returns
+        (variant, average microseconds)."""
+        v = ctypes.c_int()
+        us = ctypes.c_double()
+        self.ctx.check(self.ctx._lib.sfh_tune_hop(self.ctx.h, self.h, _abi.PREC[precision], int(n), ctypes.byref(v),
+                                                  ctypes.byref(us)))
+        return v.value, us.value
+
     def action(self, beta, c11, c12):
         out = ctypes.c_double()
         self.ctx.check(self.ctx._lib.sfh_action(self.ctx.h, self.h, beta, c11, c12, ctypes.byref(out)))
@@ -170,6 +179,12 @@ class Gauge:
         out = ctypes.c_double()
         self.ctx.check(self.ctx._lib.sfh_avg_plaq(self.ctx.h, self.h, ctypes.byref(out)))
         return out.value
+
+    def polyakov_grid(self):
+        """Tr prod_t U_3 at each spatial site, complex [(x*L1 + y)*L2 + z]."""
+        out = np.empty(2 * self.dims[0] * self.dims[1] * self.dims[2])
+        self.ctx.check(self.ctx._lib.sfh_polyakov_grid(self.ctx.h, self.h, ptr(out)))
+        return out.view(np.complex128)
 
     def tr_p2(self, P):
         a = f64(P)
@@ -240,6 +255,13 @@ class Field:
             self.close()
         except Exception:
             pass
+
+
+def set_hop_variant(precision, variant):
+    """0 one lane per site, 1 four lanes per site, -1 by size (default)."""
+    st = _abi.lib().sfh_set_hop_variant(_abi.PREC[precision], int(variant))
+    if st:
+        raise SfhError(st, "sfh_set_hop_variant")
 
 
 def random_links(dims, eps, seed):

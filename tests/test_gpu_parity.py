@@ -59,6 +59,30 @@ def test_ddag_matches_oracle(ctx, dims):
     assert rel(g.apply_ddag(0.05, psi), oc.apply_ddag(dims, 0.05, U, psi)) < 1e-14
 
 
+@pytest.mark.parametrize("dims", [(4, 4, 4, 4), (6, 4, 4, 2), (8, 6, 4, 4), (16, 16, 16, 16)])
+@pytest.mark.parametrize("prec", ["f64", "f32", "f16"])
+def test_hop_forms_agree(ctx, dims, prec):
+    """The one-lane and four-lane hop forms give the same CG solves."""
+    m = 0.1
+    U = dev.random_links(dims, 0.3, seed=41)
+    phi = dev.random_spinor(dims, seed=42)
+    g = dev.Gauge(ctx, dims, U)
+    out = []
+    try:
+        for v in (0, 1):
+            dev.set_hop_variant(prec, v)
+            out.append(g.cg(phi, m, tol=1e-10, precision=prec)[0])
+            if prec == "f64":
+                out.append(g.apply_d(m, phi))
+    finally:
+        dev.set_hop_variant(prec, -1)
+    if prec == "f64":
+        assert rel(out[1], out[3]) < 1e-14  # D: the two forms differ only in summation order
+        assert rel(out[0], out[2]) < 1e-9
+    else:
+        assert rel(out[0], out[1]) < 1e-8
+
+
 def test_dslash_delta_source_stencil(ctx):
     """The oracle's hand-derived stencil cases, on the device."""
     dims = (4, 4, 4, 4)
@@ -213,6 +237,13 @@ def test_cold_known_answers_on_device(ctx):
     assert np.abs(g.force(4.1, 5 / 3, -1 / 12, which=1)).max() < 1e-14
 
 
+@pytest.mark.parametrize("dims", [(4, 4, 4, 4), (4, 6, 8, 6), (16, 16, 16, 32)])
+def test_polyakov_grid_matches_oracle(ctx, dims):
+    U = dev.random_links(dims, 0.4, seed=44)
+    g = dev.Gauge(ctx, dims, U)
+    assert np.abs(g.polyakov_grid() - oc.polyakov_grid(dims, U)).max() < 1e-12
+
+
 def test_tr_p2(ctx):
     dims = (4, 4, 4, 6)
     P = dev.random_momenta(dims, seed=27)
@@ -245,6 +276,29 @@ def test_link_update_matches_oracle(ctx):
     bad[0] += 1.0
     with pytest.raises(RuntimeError, match="not Hermitian"):
         g.link_update(0.1, bad)
+
+
+def test_tune_hop_picks_a_form(ctx):
+    dims = (8, 8, 8, 8)
+    g = dev.Gauge(ctx, dims, dev.random_links(dims, 0.3, seed=43))
+    try:
+        for prec in ("f64", "f32", "f16"):
+            v, us = g.tune_hop(prec, n=5)
+            assert v in (0, 1) and us > 0
+    finally:
+        for prec in ("f64", "f32", "f16"):
+            dev.set_hop_variant(prec, -1)
+
+
+def test_dropin_selftest_binary():
+    """The C++ drop-in layer passes the reference-style checks."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "staggered_fermions_hmc_amd",
+                       "compat", "dropin_selftest")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=240)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
 
 
 # ---------------------------------------------------------------- integrator

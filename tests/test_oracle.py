@@ -317,6 +317,25 @@ def test_forces_gauge_covariant():
     assert np.abs(Fgm - expect).max() < 1e-11
 
 
+def test_polyakov_grid_against_numpy_walk():
+    """Lattice.cpp:849-884: product of the T time links at each spatial site,
+    right-multiplied from t = 0; cold links give Tr 1 = 3."""
+    dims = (4, 6, 4, 8)
+    U = F.random_links(dims, 0.5, seed=60)
+    Um = F.flat_to_mat(U)
+    ref = np.empty(dims[0] * dims[1] * dims[2], complex)
+    for k, (x, y, z) in enumerate(np.ndindex(*dims[:3])):
+        W = np.eye(3, dtype=complex)
+        for t in range(dims[3]):
+            W = W @ Um[F.site_index(dims, np.array([x, y, z, t])), 3]
+        ref[k] = np.trace(W)
+    assert np.abs(oc.polyakov_grid(dims, U) - ref).max() < 1e-13
+    assert np.abs(oc.polyakov_grid(dims, F.cold_links(dims)) - 3.0).max() == 0.0
+    # gauge invariant
+    Ug, _ = F.gauge_transform(dims, U, None, seed=61)
+    assert np.abs(oc.polyakov_grid(dims, Ug) - ref).max() < 1e-12
+
+
 def test_tr_p2():
     P = F.random_momenta(DIMS4, 30)
     Pm = F.flat_to_mat(P)
