@@ -177,6 +177,50 @@ int sfh_cg(sfh_ctx* ctx, const sfh_gauge* g, const sfh_cg_params* p, const doubl
 int sfh_cg_field(sfh_ctx* ctx, const sfh_gauge* g, const sfh_cg_params* p, const sfh_field* phi,
                  sfh_field* x, sfh_cg_stats* stats);
 
+/* Batched solves: many independent systems (each its own lattice, mass and
+ * 1-8 right-hand sides sharing its links) advanced together, one launch per
+ * CG phase for the whole batch, chunks of iterations replayed as a HIP graph.
+ * Each system's result does not depend on what else is in the batch.
+ * precision: SFH_PREC_F32 (FP32 inner + FP64 reliable updates, stops on the
+ * FP64 true residual) or SFH_PREC_F64 (the reference recurrence).  stats
+ * (may be NULL) receives one entry per right-hand side, systems in order.
+ * No counterpart in the reference (it solves one system at a time,
+ * Fermions.cpp:553-617). */
+typedef struct {
+  const sfh_gauge* gauge;
+  double mass;
+  int nrhs;
+  int _pad;
+  const sfh_field* const* rhs; /* nrhs device fields on gauge's lattice */
+  sfh_field* const* x;         /* nrhs device fields receiving the solutions */
+} sfh_batch_system;
+int sfh_cg_batch(sfh_ctx* ctx, int nsys, const sfh_batch_system* systems, const sfh_cg_params* p,
+                 sfh_cg_stats* stats);
+
+/* ---------------------------------------------------------------- solve service
+ * Continuous-batching CG service for concurrent HMC chains (mixed precision:
+ * FP32 inner, FP64 reliable updates, stop on the FP64 true residual).
+ * A chain registers its device-resident links once with a pool of up to 8
+ * solves in flight; the pool's memory is reserved at registration and a
+ * chain that would exceed mem_budget (0 = unlimited) is refused with
+ * SFH_E_NOMEM.  Submitted requests queue FIFO and are admitted as their
+ * chain's slots free up; a worker thread advances every solve in flight
+ * together, a chunk of iterations at a time, so requests that arrive while
+ * others run join at the next chunk boundary.  Each result equals the
+ * solve alone (bit-identical).  stats.seconds is the latency from submit to
+ * completion.  The links of a registered chain must not change while it is
+ * registered.  No counterpart in the reference. */
+typedef struct sfh_service sfh_service;
+int sfh_service_create(int device, size_t mem_budget, double tol, int max_iter, sfh_service** out);
+int sfh_service_add_chain(sfh_service* s, const sfh_gauge* g, double mass, int max_inflight, int* chain);
+int sfh_service_submit(sfh_service* s, int chain, const sfh_field* rhs, sfh_field* x, long long* ticket);
+/* Waits for one ticket (returns its status; SFH_E_HIP on timeout). */
+int sfh_service_wait(sfh_service* s, long long ticket, double timeout_s, sfh_cg_stats* stats);
+/* Collects up to max completed requests (waiting up to timeout_s for one). */
+int sfh_service_poll(sfh_service* s, double timeout_s, int max, long long* tickets, sfh_cg_stats* stats, int* n);
+const char* sfh_service_last_error(const sfh_service* s);
+int sfh_service_destroy(sfh_service* s);
+
 /* ---------------------------------------------------------------- forces, MD */
 /* F = F_gauge(beta, c11, c12) + F_fermion(X, Y) per link, selected by which.
  * Replaces compute_full_force_field (Fermions.cpp:640-659) and the

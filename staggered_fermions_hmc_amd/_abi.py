@@ -48,6 +48,11 @@ class MDParams(ctypes.Structure):
                 ("c12", _D), ("eps", _D), ("nsteps", _I), ("which", _U), ("check_hermitian", _I), ("_pad", _I)]
 
 
+class BatchSystem(ctypes.Structure):
+    _fields_ = [("gauge", _P), ("mass", _D), ("nrhs", _I), ("_pad", _I), ("rhs", ctypes.POINTER(_P)),
+                ("x", ctypes.POINTER(_P))]
+
+
 # Every exported symbol of include/sfh.h with its signature.
 SIGNATURES = {
     "sfh_abi_version": (_I, []),
@@ -77,6 +82,15 @@ SIGNATURES = {
     "sfh_inner": (_I, [_P, _SZ, _P, _P, ctypes.POINTER(_D)]),
     "sfh_cg": (_I, [_P, _P, ctypes.POINTER(CGParams), _P, _P, ctypes.POINTER(CGStats)]),
     "sfh_cg_field": (_I, [_P, _P, ctypes.POINTER(CGParams), _P, _P, ctypes.POINTER(CGStats)]),
+    "sfh_cg_batch": (_I, [_P, _I, ctypes.POINTER(BatchSystem), ctypes.POINTER(CGParams), ctypes.POINTER(CGStats)]),
+    "sfh_service_create": (_I, [_I, _SZ, _D, _I, ctypes.POINTER(_P)]),
+    "sfh_service_add_chain": (_I, [_P, _P, _D, _I, ctypes.POINTER(_I)]),
+    "sfh_service_submit": (_I, [_P, _I, _P, _P, ctypes.POINTER(ctypes.c_longlong)]),
+    "sfh_service_wait": (_I, [_P, ctypes.c_longlong, _D, ctypes.POINTER(CGStats)]),
+    "sfh_service_poll": (_I, [_P, _D, _I, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(CGStats),
+                              ctypes.POINTER(_I)]),
+    "sfh_service_last_error": (ctypes.c_char_p, [_P]),
+    "sfh_service_destroy": (_I, [_P]),
     "sfh_force": (_I, [_P, _P, _D, _D, _D, _P, _P, _U, _P]),
     "sfh_action": (_I, [_P, _P, _D, _D, _D, ctypes.POINTER(_D)]),
     "sfh_avg_plaq": (_I, [_P, _P, ctypes.POINTER(_D)]),

@@ -300,19 +300,7 @@ int sfh_tune_hop(sfh_ctx* c, const sfh_gauge* g, int precision, int n, int* vari
   float* na = static_cast<float*>(scratch(c, "tune_na", gg.V * sizeof(float)));
   float* nb = static_cast<float*>(scratch(c, "tune_nb", gg.V * sizeof(float)));
   if (!a || !b || !na || !nb) return SFH_E_NOMEM;
-  sfh_gauge* gm = const_cast<sfh_gauge*>(g);
-  if (precision == 1 && !gm->lp32_valid) {
-    if (!gm->lp.u32 && hipMalloc(&gm->lp.u32, 36 * (size_t)gg.V * sizeof(float2)) != hipSuccess)
-      return set_err(c, SFH_E_NOMEM, "FP32 link copy");
-    links_to_f32(c->stream, gg, gm->U, gm->lp.u32);
-    gm->lp32_valid = true;
-  }
-  if (precision == 2 && !gm->lp16_valid) {
-    if (!gm->lp.u16 && hipMalloc(&gm->lp.u16, 36 * (size_t)gg.V * sizeof(__half2)) != hipSuccess)
-      return set_err(c, SFH_E_NOMEM, "FP16 link copy");
-    links_to_f16(c->stream, gg, gm->U, gm->lp.u16);
-    gm->lp16_valid = true;
-  }
+  if (int e = ensure_lp(c, const_cast<sfh_gauge*>(g), precision)) return e;
   hipMemsetAsync(a, 0, nc * sizeof(double2), c->stream);
   hipMemsetAsync(na, 0, gg.V * sizeof(float), c->stream);
   double best_us = 1e30;
@@ -524,6 +512,29 @@ int sfh_cg_field(sfh_ctx* c, const sfh_gauge* g, const sfh_cg_params* p, const s
   Guard gd(c);
   if (phi->g.V != g->g.V || x->g.V != g->g.V) return set_err(c, SFH_E_INVALID, "field / gauge size mismatch");
   return cg_solve(c, const_cast<sfh_gauge*>(g), p, phi->d, x->d, stats);
+}
+
+int sfh_cg_batch(sfh_ctx* c, int nsys, const sfh_batch_system* systems, const sfh_cg_params* p,
+                 sfh_cg_stats* stats) {
+  if (!c || nsys < 1 || !systems || !cg_params_ok(p)) return SFH_E_INVALID;
+  Guard gd(c);
+  std::vector<BatchSys> bs(nsys);
+  std::vector<std::vector<const double2*>> rhs(nsys);
+  std::vector<std::vector<double2*>> xs(nsys);
+  for (int k = 0; k < nsys; ++k) {
+    const sfh_batch_system& s = systems[k];
+    if (!s.gauge || s.nrhs < 1 || s.nrhs > 8 || !s.rhs || !s.x || !std::isfinite(s.mass))
+      return set_err(c, SFH_E_INVALID, "batched system " + std::to_string(k) + ": bad gauge, mass or nrhs");
+    for (int j = 0; j < s.nrhs; ++j) {
+      if (!s.rhs[j] || !s.x[j] || s.rhs[j] == s.x[j] || s.rhs[j]->g.V != s.gauge->g.V ||
+          s.x[j]->g.V != s.gauge->g.V)
+        return set_err(c, SFH_E_INVALID, "batched system " + std::to_string(k) + ": bad field");
+      rhs[k].push_back(s.rhs[j]->d);
+      xs[k].push_back(s.x[j]->d);
+    }
+    bs[k] = BatchSys{s.gauge, s.mass, s.nrhs, rhs[k].data(), xs[k].data()};
+  }
+  return cg_batch(c, nsys, bs.data(), p, stats);
 }
 
 // ---------------------------------------------------------------- forces, MD

@@ -1,0 +1,114 @@
+// Batched CG engine (batch.hip): device tables and the host-side engine
+// shared by sfh_cg_batch and the solve service.
+#pragma once
+
+#include <chrono>
+#include <vector>
+
+#include "hopcore.h"
+#include "internal.h"
+
+namespace sfh {
+
+constexpr int kMaxRhs = 8;  // right-hand sides in flight per system
+
+template <class T> struct BSys {
+  Geom g;
+  const typename Prec<T>::L* U;
+  typename Prec<T>::R mass;
+  int nslot;
+  int nb_hop;
+  int slot[kMaxRhs];
+};
+
+template <class T> struct BSlot {
+  CGState* st;
+  typename Prec<T>::S *x, *r, *p, *t, *v;
+  double* part;
+  int V;
+};
+
+// T = float: FP32 inner iterations with FP64 reliable updates; T = double:
+// the reference recurrence in FP64.
+template <class T> class Engine {
+ public:
+  using S = typename Prec<T>::S;
+  using R = typename Prec<T>::R;
+  static constexpr bool kMixed = std::is_same<T, float>::value;
+
+  struct Done {
+    long long ticket;
+    int slot;
+    sfh_cg_stats stats;
+    CGState state;
+  };
+
+  // use_graph: replay each chunk as a captured HIP graph (best when the set
+  // of active solves is stable for many chunks, as in a fixed batch);
+  // otherwise launch the chunk's kernels directly.
+  Engine(sfh_ctx* c, double tol, int max_iter, double delta, int chunk, bool use_graph = true);
+  ~Engine();
+  Engine(const Engine&) = delete;
+  Engine& operator=(const Engine&) = delete;
+
+  // Registers a system (device links, mass) with a pool of `pool` slots of
+  // solve memory; returns its id, or minus a status.
+  int add_system(const sfh_gauge* gauge, double mass, int pool);
+  int free_slots(int sys) const;
+  // Starts a solve of (D D^dag) x = b on system sys in a free slot: returns
+  // the slot or -1 when the pool is full.  b and x are device spinors.
+  int admit(int sys, const double2* b, double2* x, long long ticket);
+  // One chunk of iterations for every active slot, then reliable updates
+  // and completions.
+  int step();
+  int active() const;
+  std::vector<Done> take_done();
+  size_t bytes() const { return bytes_; }
+  long chunks() const { return chunks_; }
+  static size_t slot_bytes(int V);
+
+ private:
+  struct SysH {
+    const sfh_gauge* gauge;
+    double mass;
+    const typename Prec<T>::L* U;
+    std::vector<int> pool;
+  };
+  struct SlotH {
+    int sys = 0;
+    bool used = false, finished = false;
+    int status = SFH_OK, rupd = 0;
+    double phi_sq = 0.0, rsq_true = 0.0;
+    const double2* b = nullptr;
+    double2* xh = nullptr;
+    double2* r64 = nullptr;
+    long long ticket = 0;
+    std::chrono::steady_clock::time_point t_admit;
+    CGState last{};
+    BSlot<T> dev{};
+  };
+  int rebuild();
+
+  sfh_ctx* c_;
+  double tol_;
+  int max_iter_;
+  double delta_;
+  int chunk_;
+  bool use_graph_;
+  void launch_chunk();
+  std::vector<SysH> sys_;
+  std::vector<SlotH> slots_;
+  std::vector<int> done_;
+  std::vector<void*> owned_;
+  size_t bytes_ = 0;
+  bool dirty_ = true;
+  int nbh_ = 0, nbb_ = 0;
+  long chunks_ = 0;
+  BSys<T>* dsys_ = nullptr;
+  BSlot<T>* dslot_ = nullptr;
+  int2 *dmh_ = nullptr, *dmb_ = nullptr;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t gexec_ = nullptr;
+};
+
+}  // namespace sfh

@@ -9,92 +9,11 @@
 // forward link of one site is reused from cache as the backward link of its
 // neighbour.  Workgroups are remapped so each XCD sweeps a contiguous range.
 // The hop is HBM-bound (~0.9 flop/byte in FP64): no MFMA.
+#include "hopcore.h"
 #include "internal.h"
 #include "reduce.h"
 
 namespace sfh {
-
-struct Half {};
-
-template <class T> struct Prec;
-template <> struct Prec<double> { using R = double; using L = double2; using S = double2; };
-template <> struct Prec<float> { using R = float; using L = float2; using S = float2; };
-template <> struct Prec<Half> { using R = float; using L = __half2; using S = __half2; };
-
-__device__ __forceinline__ cpx<double> ldc(const double2* p) {
-  const double2 v = *p;
-  return {v.x, v.y};
-}
-__device__ __forceinline__ cpx<float> ldc(const float2* p) {
-  const float2 v = *p;
-  return {v.x, v.y};
-}
-__device__ __forceinline__ cpx<float> ldc(const __half2* p) {
-  const float2 v = __half22float2(*p);
-  return {v.x, v.y};
-}
-
-template <class T>
-__device__ __forceinline__ void load_link(const typename Prec<T>::L* __restrict__ U, int V, int mu, int i,
-                                          mat3<typename Prec<T>::R>& m) {
-#pragma unroll
-  for (int e = 0; e < 9; ++e) m.e[e] = ldc(U + (size_t)(mu * 9 + e) * V + i);
-}
-
-template <class T>
-__device__ __forceinline__ void load_spinor(const typename Prec<T>::S* __restrict__ f, const float* __restrict__ nrm,
-                                            int V, int i, vec3<typename Prec<T>::R>& v) {
-#pragma unroll
-  for (int c = 0; c < 3; ++c) v.c[c] = ldc(f + (size_t)c * V + i);
-  if constexpr (std::is_same<T, Half>::value) {
-    const float s = nrm[i];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) v.c[c] = s * v.c[c];
-  }
-}
-
-template <class T>
-__device__ __forceinline__ void store_spinor(typename Prec<T>::S* __restrict__ f, float* __restrict__ nrm, int V,
-                                             int i, const vec3<typename Prec<T>::R>& v) {
-  if constexpr (std::is_same<T, Half>::value) {
-    float m = 0.f;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) m = fmaxf(m, fmaxf(fabsf(v.c[c].re), fabsf(v.c[c].im)));
-    const float inv = m > 0.f ? 1.0f / m : 0.f;
-    nrm[i] = m;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) f[(size_t)c * V + i] = __floats2half2_rn(v.c[c].re * inv, v.c[c].im * inv);
-  } else {
-    using S = typename Prec<T>::S;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) f[(size_t)c * V + i] = S{v.c[c].re, v.c[c].im};
-  }
-}
-
-// acc += s * U v
-template <class R>
-__device__ __forceinline__ void acc_mv(vec3<R>& acc, const mat3<R>& U, const vec3<R>& v, R s) {
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    cpx<R> t = U.e[3 * a] * v.c[0];
-    cmac(t, U.e[3 * a + 1], v.c[1]);
-    cmac(t, U.e[3 * a + 2], v.c[2]);
-    acc.c[a].re += s * t.re;
-    acc.c[a].im += s * t.im;
-  }
-}
-// acc += s * U^dag v
-template <class R>
-__device__ __forceinline__ void acc_mdv(vec3<R>& acc, const mat3<R>& U, const vec3<R>& v, R s) {
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    cpx<R> t = conj(U.e[a]) * v.c[0];
-    cmacc(t, U.e[3 + a], v.c[1]);
-    cmacc(t, U.e[6 + a], v.c[2]);
-    acc.c[a].re += s * t.re;
-    acc.c[a].im += s * t.im;
-  }
-}
 
 template <class T, bool DOT>
 __global__ __launch_bounds__(256) void k_hop(Geom g, const typename Prec<T>::L* __restrict__ U,

@@ -153,6 +153,22 @@ int polyakov_grid(sfh_ctx* c, const Geom& g, const double2* U, double* out);
 int cg_solve(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* p, const double2* phi, double2* x,
              sfh_cg_stats* stats);
 int apply_ddag_dev(sfh_ctx* c, const sfh_gauge* gauge, double mass, const double2* in, double2* out);
+// Maps a stopped solve's device status to the reference's error text.
+int map_status(sfh_ctx* c, const CGState& s, int max_iter);
+// FP64 dot of two device vectors of ncomplex complex numbers (synchronises).
+double dev_dot(sfh_ctx* c, const double2* a, const double2* b, size_t ncomplex);
+// Brings the FP32 / FP16 link copy of g up to date with its FP64 links.
+int ensure_lp(sfh_ctx* c, sfh_gauge* g, int precision);
+
+// ------------------------------------------------------------------ batched solves (batch.hip)
+struct BatchSys {
+  const sfh_gauge* gauge;
+  double mass;
+  int nrhs;
+  const double2* const* rhs;  // device spinors, nrhs of them
+  double2* const* x;
+};
+int cg_batch(sfh_ctx* c, int nsys, const BatchSys* sys, const sfh_cg_params* p, sfh_cg_stats* stats);
 
 // host generators (gen.cpp)
 void gen_links(const int* dims, double eps, unsigned long long seed, double* out);

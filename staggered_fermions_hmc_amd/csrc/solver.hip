@@ -15,18 +15,6 @@
 
 namespace sfh {
 
-namespace {
-
-constexpr int kChunk = 16;
-
-struct Bufs64 {
-  double2 *r, *p, *t, *v;
-};
-
-template <class T> T* buf(sfh_ctx* c, const char* key, size_t count) {
-  return static_cast<T*>(scratch(c, key, count * sizeof(T)));
-}
-
 int map_status(sfh_ctx* c, const CGState& s, int max_iter) {
   char msg[256];
   switch (s.status) {
@@ -54,6 +42,37 @@ double dev_dot(sfh_ctx* c, const double2* a, const double2* b, size_t ncomplex) 
   hipMemcpyAsync(c->h_scalar, &c->d_state[1].alpha, sizeof(double), hipMemcpyDeviceToHost, c->stream);
   if (hip_check(c, hipStreamSynchronize(c->stream), "dot")) return NAN;
   return *c->h_scalar;
+}
+
+// Low-precision links are derived lazily from the FP64 links and invalidated
+// whenever those change.
+int ensure_lp(sfh_ctx* c, sfh_gauge* gauge, int precision) {
+  const Geom& g = gauge->g;
+  if (precision == SFH_PREC_F32 && !gauge->lp32_valid) {
+    if (!gauge->lp.u32 && hipMalloc(&gauge->lp.u32, 36 * (size_t)g.V * sizeof(float2)) != hipSuccess)
+      return set_err(c, SFH_E_NOMEM, "FP32 link copy");
+    links_to_f32(c->stream, g, gauge->U, gauge->lp.u32);
+    gauge->lp32_valid = true;
+  }
+  if (precision == SFH_PREC_F16 && !gauge->lp16_valid) {
+    if (!gauge->lp.u16 && hipMalloc(&gauge->lp.u16, 36 * (size_t)g.V * sizeof(__half2)) != hipSuccess)
+      return set_err(c, SFH_E_NOMEM, "FP16 link copy");
+    links_to_f16(c->stream, g, gauge->U, gauge->lp.u16);
+    gauge->lp16_valid = true;
+  }
+  return hip_check(c, hipGetLastError(), "link copy");
+}
+
+namespace {
+
+constexpr int kChunk = 16;
+
+struct Bufs64 {
+  double2 *r, *p, *t, *v;
+};
+
+template <class T> T* buf(sfh_ctx* c, const char* key, size_t count) {
+  return static_cast<T*>(scratch(c, key, count * sizeof(T)));
 }
 
 int upload_state(sfh_ctx* c, const CGState& s) {
@@ -165,23 +184,7 @@ static int cg_mixed(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* prm, cons
   double2* r64 = buf<double2>(c, "cg_r", n);
   float2* xl = buf<float2>(c, "cgl_x", n);
   if (!r64 || !xl) return SFH_E_NOMEM;
-  // low-precision links, refreshed lazily after the FP64 links change
-  if (!HALF && !gauge->lp32_valid) {
-    if (!gauge->lp.u32) {
-      if (hipMalloc(&gauge->lp.u32, 36 * (size_t)g.V * sizeof(float2)) != hipSuccess)
-        return set_err(c, SFH_E_NOMEM, "FP32 link copy");
-    }
-    links_to_f32(c->stream, g, gauge->U, gauge->lp.u32);
-    gauge->lp32_valid = true;
-  }
-  if (HALF && !gauge->lp16_valid) {
-    if (!gauge->lp.u16) {
-      if (hipMalloc(&gauge->lp.u16, 36 * (size_t)g.V * sizeof(__half2)) != hipSuccess)
-        return set_err(c, SFH_E_NOMEM, "FP16 link copy");
-    }
-    links_to_f16(c->stream, g, gauge->U, gauge->lp.u16);
-    gauge->lp16_valid = true;
-  }
+  if (int e = ensure_lp(c, gauge, HALF ? SFH_PREC_F16 : SFH_PREC_F32)) return e;
   // sloppy vectors
   float2 *r32 = nullptr, *p32 = nullptr, *t32 = nullptr, *v32 = nullptr;
   __half2 *r16 = nullptr, *p16 = nullptr, *t16 = nullptr, *v16 = nullptr;

@@ -257,6 +257,83 @@ class Field:
             pass
 
 
+def cg_batch(ctx, systems, tol=1e-10, max_iter=100000, precision="f32", reliable_delta=0.1):
+    """Solve many systems together.  systems: list of (gauge, mass, [rhs
+    Field, ...], [x Field, ...]).  Returns one stats dict per right-hand side,
+    systems in order."""
+    n = len(systems)
+    arr = (_abi.BatchSystem * n)()
+    keep = []
+    total = 0
+    for k, (g, mass, rhs, xs) in enumerate(systems):
+        assert len(rhs) == len(xs) and 1 <= len(rhs) <= 8
+        r = (ctypes.c_void_p * len(rhs))(*[f.h.value for f in rhs])
+        x = (ctypes.c_void_p * len(xs))(*[f.h.value for f in xs])
+        keep += [r, x]
+        arr[k].gauge = g.h
+        arr[k].mass = mass
+        arr[k].nrhs = len(rhs)
+        arr[k].rhs = ctypes.cast(r, ctypes.POINTER(ctypes.c_void_p))
+        arr[k].x = ctypes.cast(x, ctypes.POINTER(ctypes.c_void_p))
+        total += len(rhs)
+    prm = _abi.CGParams(0.0, tol, max_iter, _abi.PREC[precision], reliable_delta, 0)
+    stats = (_abi.CGStats * total)()
+    ctx.check(ctx._lib.sfh_cg_batch(ctx.h, n, arr, ctypes.byref(prm), stats))
+    return [s.as_dict() for s in stats]
+
+
+class Service:
+    """Continuous-batching solve service (sfh_service_*): chains register
+    device-resident links; requests are (rhs Field, x Field) pairs."""
+
+    def __init__(self, device=0, mem_budget=0, tol=1e-10, max_iter=100000):
+        self._lib = _abi.lib()
+        h = ctypes.c_void_p()
+        st = self._lib.sfh_service_create(int(device), int(mem_budget), float(tol), int(max_iter), ctypes.byref(h))
+        if st:
+            raise SfhError(st, "sfh_service_create")
+        self.h = h
+
+    def _check(self, st):
+        if st:
+            msg = self._lib.sfh_service_last_error(self.h)
+            _raise(st, msg.decode() if msg else "")
+
+    def add_chain(self, gauge, mass, max_inflight=4):
+        cid = ctypes.c_int()
+        self._check(self._lib.sfh_service_add_chain(self.h, gauge.h, float(mass), int(max_inflight),
+                                                    ctypes.byref(cid)))
+        return cid.value
+
+    def submit(self, chain, rhs, x):
+        t = ctypes.c_longlong()
+        self._check(self._lib.sfh_service_submit(self.h, int(chain), rhs.h, x.h, ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket, timeout=60.0):
+        st = _abi.CGStats()
+        self._check(self._lib.sfh_service_wait(self.h, int(ticket), float(timeout), ctypes.byref(st)))
+        return st.as_dict()
+
+    def poll(self, timeout=1.0, max_n=256):
+        tk = (ctypes.c_longlong * max_n)()
+        sts = (_abi.CGStats * max_n)()
+        n = ctypes.c_int()
+        self._check(self._lib.sfh_service_poll(self.h, float(timeout), int(max_n), tk, sts, ctypes.byref(n)))
+        return [(tk[i], sts[i].as_dict()) for i in range(n.value)]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._lib.sfh_service_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def set_hop_variant(precision, variant):
     """0 one lane per site, 1 four lanes per site, -1 by size (default)."""
     st = _abi.lib().sfh_set_hop_variant(_abi.PREC[precision], int(variant))

@@ -1,0 +1,110 @@
+"""Batched CG (sfh_cg_batch): every right-hand side reaches its FP64 true
+residual, matches the single-system solver and the oracle, and gives the
+same result alone or in any mix (batch invariance)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as oc
+from staggered_fermions_hmc_amd import device as dev
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = dev.Context(0)
+    yield c
+    c.close()
+
+
+def make_systems(ctx, specs, seed0=100):
+    """specs: list of (dims, mass, nrhs)."""
+    out = []
+    for k, (dims, m, nrhs) in enumerate(specs):
+        U = dev.random_links(dims, (0.2, 0.3, 0.4)[k % 3], seed=seed0 + k)
+        g = dev.Gauge(ctx, dims, U)
+        bs = [dev.random_spinor(dims, seed=seed0 + 1000 + 10 * k + j) for j in range(nrhs)]
+        rf = [dev.Field(g, b) for b in bs]
+        xf = [dev.Field(g) for _ in bs]
+        out.append(dict(dims=dims, mass=m, U=U, g=g, b=bs, rf=rf, xf=xf))
+    return out
+
+
+def run(ctx, systems, precision="f32", tol=1e-10):
+    return dev.cg_batch(ctx, [(s["g"], s["mass"], s["rf"], s["xf"]) for s in systems], tol=tol,
+                        precision=precision)
+
+
+SPECS = [((4, 4, 4, 4), 0.1, 1), ((8, 8, 8, 8), 0.03, 3), ((6, 4, 4, 2), 0.05, 2), ((8, 6, 4, 4), 0.08, 1)]
+
+
+def test_batch_mixed_true_residuals(ctx):
+    systems = make_systems(ctx, SPECS)
+    stats = run(ctx, systems, "f32")
+    k = 0
+    for s in systems:
+        for j, b in enumerate(s["b"]):
+            x = s["xf"][j].download()
+            r = b - oc.apply_ddag(s["dims"], s["mass"], s["U"], x)
+            true = np.sqrt(r @ r / (b @ b))
+            assert true < 1e-10, (s["dims"], j, true, stats[k])
+            assert abs(stats[k]["true_rel_resid"] - true) < 1e-11
+            assert stats[k]["status"] == 0
+            k += 1
+
+
+def test_batch_f64_matches_oracle(ctx):
+    systems = make_systems(ctx, SPECS[:3], seed0=200)
+    stats = run(ctx, systems, "f64")
+    k = 0
+    for s in systems:
+        for j, b in enumerate(s["b"]):
+            x = s["xf"][j].download()
+            xr, it, _, _ = oc.cg(s["dims"], s["mass"], s["U"], b, tol=1e-10)
+            assert abs(stats[k]["iterations"] - it) <= 1
+            assert np.abs(x - xr).max() < 1e-9 * np.abs(xr).max()
+            k += 1
+
+
+def test_batch_matches_single_solves(ctx):
+    """Same hop form (four lanes per site) and the same reliable-update
+    sequence: the batched and the single-system solver agree."""
+    systems = make_systems(ctx, SPECS, seed0=300)
+    run(ctx, systems, "f32")
+    try:
+        dev.set_hop_variant("f32", 1)
+        dev.set_hop_variant("f64", 1)
+        for s in systems:
+            for j, b in enumerate(s["b"]):
+                xs, st = s["g"].cg(b, s["mass"], tol=1e-10, precision="f32")
+                xb = s["xf"][j].download()
+                assert np.abs(xs - xb).max() <= 1e-12 * np.abs(xs).max()
+    finally:
+        dev.set_hop_variant("f32", -1)
+        dev.set_hop_variant("f64", -1)
+
+
+def test_batch_invariance_alone_vs_mix(ctx):
+    """Each right-hand side is bit-for-bit the same solved alone or in a mix."""
+    mix = make_systems(ctx, SPECS, seed0=400)
+    run(ctx, mix, "f32")
+    for k in range(len(SPECS)):
+        alone = make_systems(ctx, [SPECS[k]], seed0=400 + k)
+        # same links and sources as system k of the mix
+        alone[0]["g"].upload(mix[k]["U"])
+        for j, b in enumerate(mix[k]["b"]):
+            alone[0]["rf"][j].upload(b)
+        run(ctx, alone, "f32")
+        for j in range(len(mix[k]["b"])):
+            assert np.array_equal(alone[0]["xf"][j].download(), mix[k]["xf"][j].download()), (k, j)
+
+
+def test_batch_zero_source_and_errors(ctx):
+    systems = make_systems(ctx, [((4, 4, 4, 4), 0.1, 2)], seed0=500)
+    systems[0]["rf"][1].upload(np.zeros(256 * 6))
+    stats = run(ctx, systems, "f32")
+    assert stats[1]["iterations"] == 0
+    assert not systems[0]["xf"][1].download().any()
+    with pytest.raises(RuntimeError, match="failed to converge"):
+        dev.cg_batch(ctx, [(systems[0]["g"], 0.1, systems[0]["rf"][:1], systems[0]["xf"][:1])], max_iter=3,
+                     precision="f32")
