@@ -81,12 +81,202 @@ def traffic_from_profiles(kernel, workload):
         return None
 
 
+BATCH_SIZES = [(8, 8, 8, 8), (12, 12, 12, 12), (16, 16, 16, 16), (24, 24, 24, 24)]
+
+
+def make_batch_systems(dev, ctx, n_per_size=16, seed=42):
+    """64 systems: 16 each of 8^4, 12^4, 16^4, 24^4; m ~ U[0.01, 0.1];
+    eps in {0.2, 0.3, 0.4}; 1-8 Gaussian right-hand sides each."""
+    rng = np.random.default_rng(seed)
+    systems = []
+    k = 0
+    for dims in BATCH_SIZES:
+        for _ in range(n_per_size):
+            eps = float(rng.choice([0.2, 0.3, 0.4]))
+            m = float(rng.uniform(0.01, 0.1))
+            nrhs = int(rng.integers(1, 9))
+            g = dev.Gauge(ctx, dims, dev.random_links(dims, eps, seed=10000 + k))
+            rf = [dev.Field(g, dev.random_spinor(dims, seed=20000 + 16 * k + j)) for j in range(nrhs)]
+            xf = [dev.Field(g) for _ in range(nrhs)]
+            systems.append(dict(dims=dims, eps=eps, mass=m, g=g, rf=rf, xf=xf))
+            k += 1
+    return systems
+
+
+def bench_batch(args, dev):
+    """64 uneven systems with 1-8 RHS each, solved together (FP32 inner +
+    FP64 reliable updates, graph-replayed chunks) vs one after another."""
+    ctx = dev.Context(0)
+    log("[bench] batch: building 64 systems")
+    systems = make_batch_systems(dev, ctx)
+    nrhs = sum(len(s["rf"]) for s in systems)
+    spec = [(s["g"], s["mass"], s["rf"], s["xf"]) for s in systems]
+    dev.cg_batch(ctx, spec[:2], tol=1e-10, precision="f32")  # warm-up
+    ctx.synchronize()
+    ctx.set_profiling(1)
+    t0 = time.perf_counter()
+    stats = dev.cg_batch(ctx, spec, tol=1e-10, precision="f32")
+    ctx.synchronize()
+    makespan = time.perf_counter() - t0
+    chunk_ms, nchunks = ctx.kernel_time("batch_chunk_f32")
+    ctx.set_profiling(0)
+    worst = max(s["true_rel_resid"] for s in stats)
+    log(f"[bench] batch: {nrhs} solves in {makespan:.3f}s, worst true residual {worst:.2e}")
+    # algorithmic bytes: FP32 links read twice per iteration while any RHS of
+    # the system is active, plus 336 B/site of FP32 vectors per RHS iteration
+    k = 0
+    link_bytes = vec_bytes = 0.0
+    for s in systems:
+        V = int(np.prod(s["dims"]))
+        its = [stats[k + j]["iterations"] for j in range(len(s["rf"]))]
+        k += len(s["rf"])
+        link_bytes += 2 * 288.0 * V * max(its)
+        vec_bytes += 336.0 * V * sum(its)
+    # sequential baselines over the same systems: plain FP64 and mixed single solves
+    seq = {}
+    for prec in ("f64", "f32"):
+        t0 = time.perf_counter()
+        for s in systems:
+            for rf, xf in zip(s["rf"], s["xf"]):
+                s["g"].cg_field(rf, xf, s["mass"], tol=1e-10, max_iter=100000, precision=prec)
+        ctx.synchronize()
+        seq[prec] = time.perf_counter() - t0
+        log(f"[bench] batch: sequential {prec} {seq[prec]:.3f}s")
+    value = nrhs / makespan
+    achieved = (link_bytes + vec_bytes) / (chunk_ms * 1e-3) / 1e9 if chunk_ms else 0.0
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(makespan * 1e3, 2), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 inner / f64 reliable updates and reductions", "data": "synthetic (seeded Philox)",
+        "config": {"workload": "64 systems (16 each of 8^4, 12^4, 16^4, 24^4), m~U[0.01,0.1], eps in "
+                               "{0.2,0.3,0.4}, 1-8 RHS each, batched CG to true residual 1e-10, hipGraph chunks",
+                   "rhs_total": nrhs, "parallelism": "single GPU"},
+        "sequential_fp64_solves_per_s": round(nrhs / seq["f64"], 3),
+        "sequential_mixed_solves_per_s": round(nrhs / seq["f32"], 3),
+        "speedup_vs_sequential_fp64": round(seq["f64"] / makespan, 3),
+        "worst_true_rel_residual": worst,
+        "roofline": {"bound": "hbm", "kernel": "batched CG chunk (hop x2 + vector updates)",
+                     "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
+def bench_serve(args, dev):
+    """32 chains on {8,12,16,24}^4 with resident links; Poisson arrivals at
+    rates swept around the sequential FP64 capacity; p50/p99 latency and
+    solves/s of the service vs a sequential FP64 CG server on the same
+    arrivals (FIFO replay of measured FP64 solve times)."""
+    import threading
+    ctx = dev.Context(0)
+    rng = np.random.default_rng(7)
+    chains = []
+    for k in range(32):
+        dims = BATCH_SIZES[k % 4]
+        m = float(rng.uniform(0.01, 0.1))
+        g = dev.Gauge(ctx, dims, dev.random_links(dims, 0.3, seed=30000 + k))
+        rhs = [dev.Field(g, dev.random_spinor(dims, seed=40000 + 8 * k + j)) for j in range(4)]
+        chains.append(dict(dims=dims, mass=m, g=g, rhs=rhs, free=[dev.Field(g) for _ in range(8)]))
+    log("[bench] serve: measuring sequential FP64 solve times")
+    fp64_t = {}
+    for k, c in enumerate(chains):
+        t0 = time.perf_counter()
+        c["g"].cg_field(c["rhs"][0], c["free"][0], c["mass"], tol=1e-10, max_iter=100000, precision="f64")
+        fp64_t[k] = time.perf_counter() - t0
+    mean_fp64 = float(np.mean(list(fp64_t.values())))
+    capacity = 1.0 / mean_fp64
+    svc = dev.Service(0, tol=1e-10)
+    ids = [svc.add_chain(c["g"], c["mass"], max_inflight=8) for c in chains]
+    results = {}
+    for mult in [float(r) for r in args.rates.split(",")]:
+        lam = mult * capacity
+        n = args.requests
+        gaps = rng.exponential(1.0 / lam, size=n)
+        which = rng.integers(0, len(chains), size=n)
+        done = {}
+        inflight = {}
+        lock = threading.Lock()
+
+        def collect():
+            while True:
+                with lock:
+                    if len(done) >= n:
+                        return
+                for t, st in svc.poll(timeout=0.5):
+                    with lock:
+                        k, xf, ta = inflight.pop(t)
+                        chains[k]["free"].append(xf)
+                        done[t] = (ta, st)
+
+        th = threading.Thread(target=collect)
+        th.start()
+        t_start = time.perf_counter()
+        t_next = t_start
+        arrivals = []
+        for r in range(n):
+            t_next += gaps[r]
+            now = time.perf_counter()
+            if t_next > now:
+                time.sleep(t_next - now)
+            k = int(which[r])
+            c = chains[k]
+            while True:
+                with lock:
+                    xf = c["free"].pop() if c["free"] else None
+                if xf is not None:
+                    break
+                time.sleep(0.0005)
+            ta = time.perf_counter() - t_start
+            with lock:
+                t = svc.submit(ids[k], c["rhs"][r % 4], xf)
+                inflight[t] = (k, xf, ta)
+            arrivals.append((ta, k))
+        th.join()
+        makespan = time.perf_counter() - t_start
+        lat = np.array([st["seconds"] for _, st in done.values()]) * 1e3
+        worst = max(st["true_rel_resid"] for _, st in done.values())
+        # sequential FP64 FIFO server on the same arrivals
+        fin, seq_lat = 0.0, []
+        for ta, k in arrivals:
+            fin = max(fin, ta) + fp64_t[k]
+            seq_lat.append((fin - ta) * 1e3)
+        seq_lat = np.array(seq_lat)
+        results[f"{mult:g}x"] = {
+            "rate_per_s": round(lam, 3), "requests": n, "solves_per_s": round(n / makespan, 3),
+            "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+            "worst_true_rel_residual": worst,
+            "sequential_fp64": {"solves_per_s": round(n / fin, 3), "p50_ms": round(float(np.percentile(seq_lat, 50)), 2),
+                                "p99_ms": round(float(np.percentile(seq_lat, 99)), 2)},
+        }
+        log(f"[bench] serve {mult}x: {results[f'{mult:g}x']}")
+    svc.close()
+    top = results[max(results, key=lambda k: float(k[:-1]))]
+    out = {
+        "metric": METRIC, "value": top["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.requests,
+        "warmup": 0, "ms_per_step": round(1e3 / max(top["solves_per_s"], 1e-9), 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 inner / f64 reliable updates and reductions",
+        "data": "synthetic (seeded Philox)",
+        "config": {"workload": "serving: 32 chains on {8,12,16,24}^4 with resident links (eps=0.3), "
+                               "m~U[0.01,0.1], Poisson arrivals, continuous batching, true residual 1e-10",
+                   "parallelism": "single GPU"},
+        "p50_solve_ms": top["p50_ms"], "p99_solve_ms": top["p99_ms"],
+        "sequential_fp64_capacity_per_s": round(capacity, 3),
+        "rates": results,
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="16^4", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="16^4", choices=sorted(WORKLOADS) + ["batch", "serve"])
+    ap.add_argument("--rates", default="0.5,1,2", help="serve: arrival rates as multiples of the measured "
+                                                     "sequential FP64 capacity")
+    ap.add_argument("--requests", type=int, default=200, help="serve: requests per arrival rate")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -95,6 +285,11 @@ def main():
         sys.exit(2)
 
     from staggered_fermions_hmc_amd import device as dev
+
+    if args.workload == "batch":
+        return bench_batch(args, dev)
+    if args.workload == "serve":
+        return bench_serve(args, dev)
 
     wl = WORKLOADS[args.workload]
     dims, mass = wl["dims"], wl["mass"]
