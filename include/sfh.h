@@ -137,6 +137,13 @@ int sfh_gauge_create(sfh_ctx* ctx, const int dims[4], sfh_gauge** out);
 int sfh_gauge_destroy(sfh_ctx* ctx, sfh_gauge* g);
 int sfh_gauge_upload(sfh_ctx* ctx, sfh_gauge* g, const double* links);
 int sfh_gauge_download(sfh_ctx* ctx, const sfh_gauge* g, double* links);
+/* Checkpoint of device-resident links in the reference's byte format: the n
+ * link arrays, each V*4*18 little-endian doubles in the reference AoS order,
+ * written back to back (save_all_lattice_configs / load_all_lattice_configs,
+ * Lattice.cpp:810-847, with CONFIG = n).  Load fails with SFH_E_INVALID if
+ * the file is shorter than n arrays. */
+int sfh_gauge_save(sfh_ctx* ctx, const sfh_gauge* const* gauges, int n, const char* path);
+int sfh_gauge_load(sfh_ctx* ctx, sfh_gauge* const* gauges, int n, const char* path);
 /* Seeded links U = exp(i eps H), H = sum_a r_a lambda_a/2, r_a ~ N(0,1)
  * (Philox4x32-10 + Box-Muller, counter = link index), generated on the host
  * so the same seed gives the same bytes everywhere.  eps = 0 gives a cold

@@ -69,6 +69,8 @@ SIGNATURES = {
     "sfh_gauge_destroy": (_I, [_P, _P]),
     "sfh_gauge_upload": (_I, [_P, _P, _P]),
     "sfh_gauge_download": (_I, [_P, _P, _P]),
+    "sfh_gauge_save": (_I, [_P, ctypes.POINTER(_P), _I, ctypes.c_char_p]),
+    "sfh_gauge_load": (_I, [_P, ctypes.POINTER(_P), _I, ctypes.c_char_p]),
     "sfh_random_links": (_I, [_P, _D, _ULL, _P]),
     "sfh_random_spinor": (_I, [_P, _D, _ULL, _P]),
     "sfh_random_momenta": (_I, [_P, _ULL, _P]),

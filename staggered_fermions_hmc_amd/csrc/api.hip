@@ -375,6 +375,52 @@ int sfh_gauge_download(sfh_ctx* c, const sfh_gauge* g, double* links) {
   return download_links(c, g->g, g->U, links);
 }
 
+int sfh_gauge_save(sfh_ctx* c, const sfh_gauge* const* gauges, int n, const char* path) {
+  if (!c || !gauges || n < 1 || !path) return SFH_E_INVALID;
+  Guard gd(c);
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return set_err(c, SFH_E_INVALID, std::string("Error opening file for writing: ") + path);
+  int rc = SFH_OK;
+  std::vector<double> host;
+  for (int k = 0; k < n && rc == SFH_OK; ++k) {
+    if (!gauges[k]) {
+      rc = SFH_E_INVALID;
+      break;
+    }
+    host.resize((size_t)gauges[k]->g.V * 72);
+    if ((rc = download_links(c, gauges[k]->g, gauges[k]->U, host.data()))) break;
+    if (std::fwrite(host.data(), sizeof(double), host.size(), f) != host.size())
+      rc = set_err(c, SFH_E_INVALID, std::string("Error writing complete binary data to: ") + path);
+  }
+  std::fclose(f);
+  return rc;
+}
+
+int sfh_gauge_load(sfh_ctx* c, sfh_gauge* const* gauges, int n, const char* path) {
+  if (!c || !gauges || n < 1 || !path) return SFH_E_INVALID;
+  Guard gd(c);
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return set_err(c, SFH_E_INVALID, std::string("Error opening file for reading: ") + path);
+  int rc = SFH_OK;
+  std::vector<double> host;
+  for (int k = 0; k < n && rc == SFH_OK; ++k) {
+    if (!gauges[k]) {
+      rc = SFH_E_INVALID;
+      break;
+    }
+    host.resize((size_t)gauges[k]->g.V * 72);
+    if (std::fread(host.data(), sizeof(double), host.size(), f) != host.size()) {
+      rc = set_err(c, SFH_E_INVALID, std::string("Error reading complete binary data from: ") + path);
+      break;
+    }
+    invalidate_lp(gauges[k]);
+    if ((rc = upload_links(c, gauges[k]->g, host.data(), gauges[k]->U))) break;
+    rc = hip_check(c, hipStreamSynchronize(c->stream), "gauge load");
+  }
+  std::fclose(f);
+  return rc;
+}
+
 int sfh_random_links(const int dims[4], double eps, unsigned long long seed, double* links) {
   if (!dims_ok(dims) || !links || !std::isfinite(eps)) return SFH_E_INVALID;
   gen_links(dims, eps, seed, links);

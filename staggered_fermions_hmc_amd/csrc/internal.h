@@ -74,7 +74,9 @@ struct sfh_ctx {
 
 namespace sfh {
 
-constexpr int kPartialSlot = 65536;  // max workgroups per reduction slot
+// Max workgroups per reduction slot: the four-lane hop on the largest
+// allowed lattice (2^24 sites) launches 2^24 / 64 = 2^18 workgroups.
+constexpr int kPartialSlot = 1 << 18;
 
 // ------------------------------------------------------------------ helpers
 int set_err(sfh_ctx* c, int code, const std::string& msg);

@@ -257,6 +257,18 @@ class Field:
             pass
 
 
+def save_links(ctx, gauges, path):
+    """Checkpoint in the reference's byte format (Lattice.cpp:810-828)."""
+    arr = (ctypes.c_void_p * len(gauges))(*[g.h.value for g in gauges])
+    ctx.check(ctx._lib.sfh_gauge_save(ctx.h, arr, len(gauges), str(path).encode()))
+
+
+def load_links(ctx, gauges, path):
+    """Restore a checkpoint written by save_links or the reference (Lattice.cpp:830-847)."""
+    arr = (ctypes.c_void_p * len(gauges))(*[g.h.value for g in gauges])
+    ctx.check(ctx._lib.sfh_gauge_load(ctx.h, arr, len(gauges), str(path).encode()))
+
+
 def cg_batch(ctx, systems, tol=1e-10, max_iter=100000, precision="f32", reliable_delta=0.1):
     """Solve many systems together.  systems: list of (gauge, mass, [rhs
     Field, ...], [x Field, ...]).  Returns one stats dict per right-hand side,

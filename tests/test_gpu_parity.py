@@ -244,6 +244,24 @@ def test_polyakov_grid_matches_oracle(ctx, dims):
     assert np.abs(g.polyakov_grid() - oc.polyakov_grid(dims, U)).max() < 1e-12
 
 
+def test_checkpoint_byte_format(ctx, tmp_path):
+    """sfh_gauge_save writes the reference's format (Lattice.cpp:810-828):
+    the AoS link arrays back to back as raw doubles; load restores them."""
+    dims = (4, 4, 4, 6)
+    U1 = dev.random_links(dims, 0.3, seed=45)
+    U2 = dev.random_links(dims, 0.5, seed=46)
+    g1, g2 = dev.Gauge(ctx, dims, U1), dev.Gauge(ctx, dims, U2)
+    path = tmp_path / "final_lattice_configs.bin"
+    dev.save_links(ctx, [g1, g2], path)
+    raw = np.fromfile(path, dtype="<f8")
+    assert np.array_equal(raw, np.concatenate([U1, U2]))
+    h1, h2 = dev.Gauge(ctx, dims), dev.Gauge(ctx, dims)
+    dev.load_links(ctx, [h1, h2], path)
+    assert np.array_equal(h1.download(), U1) and np.array_equal(h2.download(), U2)
+    with pytest.raises(ValueError):
+        dev.load_links(ctx, [h1, h2, dev.Gauge(ctx, dims)], path)
+
+
 def test_tr_p2(ctx):
     dims = (4, 4, 4, 6)
     P = dev.random_momenta(dims, seed=27)
