@@ -223,15 +223,20 @@ int Engine<T>::add_system(const sfh_gauge* gauge, double mass, int pool) {
     h.sys = (int)sys_.size();
     BSlot<T>& L = h.dev;
     L.V = V;
-    L.x = reinterpret_cast<S*>(take(n * sizeof(S)));
+    // FP64 solves iterate directly in the caller's x (set at admission); the
+    // mixed ones keep an FP32 x and an FP64 residual of their own.  The
+    // arrays taken here are exactly those counted by slot_bytes().
+    if (kMixed) L.x = reinterpret_cast<S*>(take(n * sizeof(S)));
     L.r = reinterpret_cast<S*>(take(n * sizeof(S)));
     L.p = reinterpret_cast<S*>(take(n * sizeof(S)));
     L.t = reinterpret_cast<S*>(take(n * sizeof(S)));
-    if (kMixed) L.v = reinterpret_cast<S*>(take(n * sizeof(S)));
-    else L.v = reinterpret_cast<S*>(take(n * sizeof(S)));
+    L.v = reinterpret_cast<S*>(take(n * sizeof(S)));
     if (kMixed) h.r64 = reinterpret_cast<double2*>(take(n * sizeof(double2)));
     L.st = reinterpret_cast<CGState*>(take(sizeof(CGState)));
     L.part = reinterpret_cast<double*>(p);
+    const int nbh = (V / 2 + 31) / 32, nbb = (V + kBlas - 1) / kBlas;
+    const size_t used = (size_t)(p - (base + sb * k)) + align256(std::max(nbh, nbb) * sizeof(double));
+    if (used > sb) return -set_err(c_, SFH_E_INVALID, "slot layout exceeds its allocation");
     s.pool.push_back((int)slots_.size());
     slots_.push_back(h);
   }
