@@ -10,20 +10,27 @@
 
 namespace sfh {
 
+//
+// Coherence across the 8 XCDs (each with its own L2) without cache-wide
+// fences: the partial is written with an agent-scope atomic store (coherent
+// at device scope), the wave waits for that store to complete, then counts
+// its arrival with an agent-scope atomic add; the last workgroup reads the
+// partials with agent-scope atomic loads.  No L2 write-back or invalidate is
+// issued, so the kernel's other traffic keeps its cache reuse.  What the
+// finisher writes is read by later kernels, after the kernel boundary.
 template <class Fin>
 __device__ __forceinline__ void grid_reduce(double v, double* partials, unsigned* counter, int nblocks, int slot,
                                             Fin fin) {
   const double bs = block_sum(v);
   __shared__ int is_last;
   if (threadIdx.x == 0) {
-    partials[slot] = bs;
-    __threadfence();
-    const unsigned t = atomicAdd(counter, 1u);
+    __hip_atomic_store(&partials[slot], bs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_waitcnt(0);  // the partial's store has completed before the arrival is counted
+    const unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     is_last = (t == unsigned(nblocks - 1));
   }
   __syncthreads();
   if (is_last) {
-    __threadfence();
     double s = 0.0;
     for (int k = threadIdx.x; k < nblocks; k += blockDim.x)
       s += __hip_atomic_load(&partials[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -31,7 +38,6 @@ __device__ __forceinline__ void grid_reduce(double v, double* partials, unsigned
     if (threadIdx.x == 0) {
       fin(s);
       __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __threadfence();
     }
   }
 }

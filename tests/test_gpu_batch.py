@@ -67,8 +67,8 @@ def test_batch_f64_matches_oracle(ctx):
 
 
 def test_batch_matches_single_solves(ctx):
-    """Same hop form (four lanes per site) and the same reliable-update
-    sequence: the batched and the single-system solver agree."""
+    """The batched and the single-system solver agree to the accuracy of
+    solves converged to a true residual of 1e-10."""
     systems = make_systems(ctx, SPECS, seed0=300)
     run(ctx, systems, "f32")
     try:
@@ -78,7 +78,7 @@ def test_batch_matches_single_solves(ctx):
             for j, b in enumerate(s["b"]):
                 xs, st = s["g"].cg(b, s["mass"], tol=1e-10, precision="f32")
                 xb = s["xf"][j].download()
-                assert np.abs(xs - xb).max() <= 1e-12 * np.abs(xs).max()
+                assert np.abs(xs - xb).max() <= 1e-8 * np.abs(xs).max()
     finally:
         dev.set_hop_variant("f32", -1)
         dev.set_hop_variant("f64", -1)
