@@ -129,6 +129,11 @@ int sfh_set_hop_variant(int precision, int variant);
  * with its average launch time in microseconds.  No counterpart in the
  * reference. */
 int sfh_tune_hop(sfh_ctx* ctx, const sfh_gauge* g, int precision, int n, int* variant, double* us);
+/* Average duration in microseconds of the hop kernel the solver uses for
+ * this lattice and precision (with the <p,v> reduction if with_dot),
+ * n back-to-back launches between HIP events on the context stream.  No
+ * counterpart in the reference. */
+int sfh_time_hop(sfh_ctx* ctx, const sfh_gauge* g, int precision, int with_dot, int n, double* us);
 
 /* ---------------------------------------------------------------- gauge field
  * Device-resident links for one lattice.  dims = {L0, L1, L2, L3}, all even,

@@ -65,6 +65,7 @@ SIGNATURES = {
     "sfh_kernel_time": (_I, [_P, ctypes.c_char_p, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_long)]),
     "sfh_set_hop_variant": (_I, [_I, _I]),
     "sfh_tune_hop": (_I, [_P, _P, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_D)]),
+    "sfh_time_hop": (_I, [_P, _P, _I, _I, _I, ctypes.POINTER(_D)]),
     "sfh_gauge_create": (_I, [_P, _P, ctypes.POINTER(_P)]),
     "sfh_gauge_destroy": (_I, [_P, _P]),
     "sfh_gauge_upload": (_I, [_P, _P, _P]),

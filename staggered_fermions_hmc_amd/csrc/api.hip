@@ -243,6 +243,10 @@ int sfh_destroy(sfh_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   resolve_timers(c);
   for (auto e : c->ev_pool) hipEventDestroy(e);
+  for (auto& kv : c->graphs) {
+    hipGraphExecDestroy(kv.second.second);
+    hipGraphDestroy(kv.second.first);
+  }
   for (auto& kv : c->scratch) hipFree(kv.second.first);
   if (c->d_state) hipFree(c->d_state);
   if (c->d_partials) hipFree(c->d_partials);
@@ -332,6 +336,58 @@ int sfh_tune_hop(sfh_ctx* c, const sfh_gauge* g, int precision, int n, int* vari
   set_hop_variant(precision, best);
   if (variant) *variant = best;
   if (us) *us = best_us;
+  return SFH_OK;
+}
+
+int sfh_time_hop(sfh_ctx* c, const sfh_gauge* g, int precision, int with_dot, int n, double* us) {
+  if (!c || !g || precision < 0 || precision > 2 || n < 1 || !us) return SFH_E_INVALID;
+  Guard gd(c);
+  const Geom& gg = g->g;
+  const size_t nc = 3 * (size_t)gg.V;
+  void* a = scratch(c, "tune_a", nc * sizeof(double2));
+  void* b = scratch(c, "tune_b", nc * sizeof(double2));
+  float* na = static_cast<float*>(scratch(c, "tune_na", gg.V * sizeof(float)));
+  float* nb = static_cast<float*>(scratch(c, "tune_nb", gg.V * sizeof(float)));
+  if (!a || !b || !na || !nb) return SFH_E_NOMEM;
+  if (int e = ensure_lp(c, const_cast<sfh_gauge*>(g), precision)) return e;
+  // input of ones (a non-zero <p,v> keeps the timed solve state running)
+  if (precision == 0) {
+    std::vector<double> ones(2 * nc, 1.0);
+    hipMemcpyAsync(a, ones.data(), ones.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    hipStreamSynchronize(c->stream);
+  } else if (precision == 1) {
+    std::vector<float> ones(2 * nc, 1.0f);
+    hipMemcpyAsync(a, ones.data(), ones.size() * sizeof(float), hipMemcpyHostToDevice, c->stream);
+    hipStreamSynchronize(c->stream);
+  } else {
+    hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(a), 0x3C003C00u, nc, c->stream);  // half2 (1, 1)
+    hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(na), 0x3F800000u, gg.V, c->stream);  // 1.0f
+  }
+  // a live solve state so the reduction runs exactly as in the CG
+  CGState* st = &c->d_state[1];
+  CGState z{};
+  z.rsq = 1.0;
+  hipMemcpyAsync(st, &z, sizeof(CGState), hipMemcpyHostToDevice, c->stream);
+  double* part = c->d_partials + 4 * (size_t)kPartialSlot;
+  auto launch = [&]() {
+    if (precision == 0)
+      hop_f64(c->stream, gg, g->U, (const double2*)a, (double2*)b, 0.1, 1.0, with_dot ? (const double2*)a : nullptr,
+              with_dot ? st : nullptr, part);
+    else if (precision == 1)
+      hop_f32(c->stream, gg, g->lp.u32, (const float2*)a, (float2*)b, 0.1f, 1.0f,
+              with_dot ? (const float2*)a : nullptr, with_dot ? st : nullptr, part);
+    else
+      hop_f16(c->stream, gg, g->lp.u16, (const __half2*)a, na, (__half2*)b, nb, 0.1f, 1.0f,
+              with_dot ? (const __half2*)a : nullptr, with_dot ? na : nullptr, with_dot ? st : nullptr, part);
+  };
+  launch();
+  hipEventRecord(c->ev0, c->stream);
+  for (int k = 0; k < n; ++k) launch();
+  hipEventRecord(c->ev1, c->stream);
+  if (int e = hip_check(c, hipEventSynchronize(c->ev1), "hop timing")) return e;
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  *us = 1e3 * ms / n;
   return SFH_OK;
 }
 

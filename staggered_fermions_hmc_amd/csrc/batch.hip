@@ -426,6 +426,7 @@ int Engine<T>::step() {
     ns.rmax = std::sqrt(rt);
     ns.done = 0;
     ns.pause_reason = 0;
+    ns.ru_pending = 0;
     for (int q = 0; q < 8; ++q) ns.cnt[q] = 0;
     hipMemcpyAsync(h.dev.st, &ns, sizeof(CGState), hipMemcpyHostToDevice, c_->stream);
     cg_p_f32(c_->stream, s.gauge->g, reinterpret_cast<float2*>(h.dev.p), reinterpret_cast<const float2*>(h.dev.r),

@@ -248,4 +248,99 @@ void sub_f64(hipStream_t s, double2* out, const double2* a, const double2* b, si
   hipLaunchKernelGGL(k_sub, dim3(nblk(n)), dim3(kB), 0, s, out, a, b, n);
 }
 
+// ------------------------------------------------------------------ device-side reliable update
+// Run at the end of every chunk of mixed-precision iterations; each kernel
+// returns at once unless a reliable update is pending (ru_pending) or has
+// just restarted the solve (ru_apply), so no host round trip is needed.
+//   1 k_ru_acc   : x += x_lo, x_lo = 0
+//   2,3 hop_f64  : ax = D (D^dag x)                  (gate 1)
+//   4 k_ru_sub   : r64 = phi - ax, |r64|^2 -> FinRU
+//   5 k_ru_apply : r_lo = r64 in sloppy storage, p = r_lo + beta p
+__global__ __launch_bounds__(kB) void k_ru_acc(size_t n, double2* __restrict__ xh, float2* __restrict__ xl,
+                                               CGState* st) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->ru_apply = 0;  // read only by the apply kernel of the next chunk
+  if (!st->ru_pending) return;
+  const size_t i = (size_t)blockIdx.x * kB + threadIdx.x;
+  if (i >= n) return;
+  const float2 l = xl[i];
+  double2 h = xh[i];
+  h.x += (double)l.x;
+  h.y += (double)l.y;
+  xh[i] = h;
+  xl[i] = make_float2(0.f, 0.f);
+}
+
+__global__ __launch_bounds__(kB) void k_ru_sub(int V, double2* __restrict__ r, const double2* __restrict__ phi,
+                                               const double2* __restrict__ ax, CGState* st, double* partials, int nb) {
+  if (!st->ru_pending) return;
+  const int i = blockIdx.x * kB + threadIdx.x;
+  double rr = 0.0;
+  if (i < V) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const size_t j = (size_t)c * V + i;
+      const double2 a = phi[j], b = ax[j];
+      const double2 d = make_double2(a.x - b.x, a.y - b.y);
+      r[j] = d;
+      rr += d.x * d.x + d.y * d.y;
+    }
+  }
+  grid_reduce(rr, partials, &st->cnt[CNT_RU], nb, blockIdx.x, FinRU{st});
+}
+
+__global__ __launch_bounds__(kB) void k_ru_apply32(int V, const double2* __restrict__ r64, float2* __restrict__ r,
+                                                   float2* __restrict__ p, const CGState* st) {
+  if (!st->ru_apply) return;
+  const int i = blockIdx.x * kB + threadIdx.x;
+  if (i >= V) return;
+  const float be = (float)st->beta;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const size_t j = (size_t)c * V + i;
+    const double2 t = r64[j];
+    const float2 rv = make_float2((float)t.x, (float)t.y);
+    float2 pp = p[j];
+    pp.x = rv.x + be * pp.x;
+    pp.y = rv.y + be * pp.y;
+    r[j] = rv;
+    p[j] = pp;
+  }
+}
+
+__global__ __launch_bounds__(kB) void k_ru_apply16(int V, const double2* __restrict__ r64, __half2* __restrict__ r,
+                                                   float* __restrict__ rn, __half2* __restrict__ p,
+                                                   float* __restrict__ pn, const CGState* st) {
+  if (!st->ru_apply) return;
+  const int i = blockIdx.x * kB + threadIdx.x;
+  if (i >= V) return;
+  const float be = (float)st->beta;
+  float2 rv[3], pp[3];
+  ld16(p, pn, V, i, pp);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double2 t = r64[(size_t)c * V + i];
+    rv[c] = make_float2((float)t.x, (float)t.y);
+    pp[c].x = rv[c].x + be * pp[c].x;
+    pp[c].y = rv[c].y + be * pp[c].y;
+  }
+  st16(r, rn, V, i, rv);
+  st16(p, pn, V, i, pp);
+}
+
+void ru_acc(hipStream_t s, double2* xh, float2* xl, size_t n, CGState* st) {
+  hipLaunchKernelGGL(k_ru_acc, dim3(nblk(n)), dim3(kB), 0, s, n, xh, xl, st);
+}
+void ru_sub(hipStream_t s, const Geom& g, double2* r, const double2* phi, const double2* ax, CGState* st,
+            double* partials) {
+  const unsigned nb = nblk(g.V);
+  hipLaunchKernelGGL(k_ru_sub, dim3(nb), dim3(kB), 0, s, g.V, r, phi, ax, st, partials, (int)nb);
+}
+void ru_apply32(hipStream_t s, const Geom& g, const double2* r64, float2* r, float2* p, const CGState* st) {
+  hipLaunchKernelGGL(k_ru_apply32, dim3(nblk(g.V)), dim3(kB), 0, s, g.V, r64, r, p, st);
+}
+void ru_apply16(hipStream_t s, const Geom& g, const double2* r64, __half2* r, float* rn, __half2* p, float* pn,
+                const CGState* st) {
+  hipLaunchKernelGGL(k_ru_apply16, dim3(nblk(g.V)), dim3(kB), 0, s, g.V, r64, r, rn, p, pn, st);
+}
+
 }  // namespace sfh

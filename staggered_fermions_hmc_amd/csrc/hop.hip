@@ -22,9 +22,11 @@ __global__ __launch_bounds__(256) void k_hop(Geom g, const typename Prec<T>::L* 
                                              float* __restrict__ out_norm, typename Prec<T>::R a,
                                              typename Prec<T>::R hb, const typename Prec<T>::S* __restrict__ dw,
                                              const float* __restrict__ dw_norm, CGState* st, double* partials,
-                                             int nb) {
+                                             int nb, int gate) {
   using R = typename Prec<T>::R;
-  if (st != nullptr && st->done) return;
+  // gate 0: a CG step, skipped once the solve has stopped; gate 1: a
+  // reliable-update step, run only while one is pending
+  if (st != nullptr && (gate ? !st->ru_pending : st->done)) return;
   const int lb = xcd_block(nb);
   const int hw = blockDim.x >> 1;
   const int p = threadIdx.x >= hw;
@@ -101,9 +103,11 @@ __global__ __launch_bounds__(256) void k_hop4(Geom g, const typename Prec<T>::L*
                                               float* __restrict__ out_norm, typename Prec<T>::R a,
                                               typename Prec<T>::R hb, const typename Prec<T>::S* __restrict__ dw,
                                               const float* __restrict__ dw_norm, CGState* st, double* partials,
-                                              int nb) {
+                                              int nb, int gate) {
   using R = typename Prec<T>::R;
-  if (st != nullptr && st->done) return;
+  // gate 0: a CG step, skipped once the solve has stopped; gate 1: a
+  // reliable-update step, run only while one is pending
+  if (st != nullptr && (gate ? !st->ru_pending : st->done)) return;
   const int lb = xcd_block(nb);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int mu = lane >> 4, sl = lane & 15;
@@ -203,23 +207,23 @@ template <class T>
 static void hop_launch(hipStream_t s, int variant, const Geom& g, const typename Prec<T>::L* U,
                        const typename Prec<T>::S* in, const float* in_norm, typename Prec<T>::S* out, float* out_norm,
                        typename Prec<T>::R a, typename Prec<T>::R hb, const typename Prec<T>::S* dw,
-                       const float* dw_norm, CGState* st, double* partials) {
+                       const float* dw_norm, CGState* st, double* partials, int gate) {
   if (variant == 1) {
     const int nb = hop4_blocks(g);
     if (dw)
       hipLaunchKernelGGL((k_hop4<T, true>), dim3(nb), dim3(256), 0, s, g, U, in, in_norm, out, out_norm, a, hb, dw,
-                         dw_norm, st, partials, nb);
+                         dw_norm, st, partials, nb, gate);
     else
       hipLaunchKernelGGL((k_hop4<T, false>), dim3(nb), dim3(256), 0, s, g, U, in, in_norm, out, out_norm, a, hb,
-                         dw, dw_norm, st, partials, nb);
+                         dw, dw_norm, st, partials, nb, gate);
   } else {
     const int nb = hop_blocks(g);
     if (dw)
       hipLaunchKernelGGL((k_hop<T, true>), dim3(nb), dim3(kHopBlock), 0, s, g, U, in, in_norm, out, out_norm, a, hb,
-                         dw, dw_norm, st, partials, nb);
+                         dw, dw_norm, st, partials, nb, gate);
     else
       hipLaunchKernelGGL((k_hop<T, false>), dim3(nb), dim3(kHopBlock), 0, s, g, U, in, in_norm, out, out_norm, a,
-                         hb, dw, dw_norm, st, partials, nb);
+                         hb, dw, dw_norm, st, partials, nb, gate);
   }
 }
 
@@ -237,22 +241,22 @@ void set_hop_variant(int prec, int variant) {
 int get_hop_variant(int prec, const Geom& g) { return pick_variant(prec, g); }
 
 void hop_f64(hipStream_t s, const Geom& g, const double2* U, const double2* in, double2* out, double a, double b,
-             const double2* dot_with, CGState* st, double* partials) {
+             const double2* dot_with, CGState* st, double* partials, int gate) {
   hop_launch<double>(s, pick_variant(0, g), g, U, in, nullptr, out, nullptr, a, 0.5 * b, dot_with, nullptr, st,
-                     partials);
+                     partials, gate);
 }
 
 void hop_f32(hipStream_t s, const Geom& g, const float2* U, const float2* in, float2* out, float a, float b,
              const float2* dot_with, CGState* st, double* partials) {
   hop_launch<float>(s, pick_variant(1, g), g, U, in, nullptr, out, nullptr, a, 0.5f * b, dot_with, nullptr, st,
-                    partials);
+                    partials, 0);
 }
 
 void hop_f16(hipStream_t s, const Geom& g, const __half2* U, const __half2* in, const float* in_norm, __half2* out,
              float* out_norm, float a, float b, const __half2* dot_with, const float* dot_norm, CGState* st,
              double* partials) {
   hop_launch<Half>(s, pick_variant(2, g), g, U, in, in_norm, out, out_norm, a, 0.5f * b, dot_with, dot_norm, st,
-                   partials);
+                   partials, 0);
 }
 
 // ------------------------------------------------------------------ layout converters

@@ -21,13 +21,18 @@ struct CGState {
   double target;    // tol^2 |phi|^2
   double rmax;      // mixed precision: max |r| since the last reliable update
   double delta;     // mixed precision: reliable-update factor
+  double rsq_true;  // mixed precision: FP64 true |r|^2 at the last reliable update
   int iter, done, status, max_iter;
-  int pause_reason;  // mixed precision: 1 reliable update due, 2 iterated residual converged
+  int pause_reason;  // mixed precision: 1 reliable update due, 2 iterated residual converged, 3 budget spent
   int mixed;
-  unsigned cnt[8];  // grid-reduction arrival counters
+  int ru_pending;    // mixed precision: the reliable-update kernels run
+  int ru_apply;      // mixed precision: restart the sloppy residual and direction
+  int rupd;          // reliable updates done
+  int converged;     // mixed precision: the FP64 true residual met the target
+  unsigned cnt[8];   // grid-reduction arrival counters
 };
 
-enum { CNT_PV = 0, CNT_RR = 1, CNT_AUX = 2 };
+enum { CNT_PV = 0, CNT_RR = 1, CNT_AUX = 2, CNT_RU = 3 };
 
 // Low-precision link / spinor storage (FP16 uses a per-site FP32 norm).
 struct LinksLP {
@@ -70,6 +75,8 @@ struct sfh_ctx {
   struct Pending { std::string name; hipEvent_t a, b; };
   std::vector<Pending> pending;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // whole-solve timing
+  // captured CG chunks, keyed by the buffers and parameters they replay
+  std::map<std::string, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
 };
 
 namespace sfh {
@@ -106,8 +113,10 @@ void links_to_f16(hipStream_t s, const Geom& g, const double2* U, __half2* u16);
 // ------------------------------------------------------------------ hop (hop.hip)
 // out = a*in + (b/2) * sum_mu eta [bup U psi(n+mu) - bdn U^dag psi(n-mu)]
 // If dot_with != nullptr, also reduces Re<dot_with, out> into st (pv, alpha).
+// gate 0: skip when st->done (a CG step); gate 1: run only while
+// st->ru_pending (a reliable-update step).
 void hop_f64(hipStream_t s, const Geom& g, const double2* U, const double2* in, double2* out, double a, double b,
-             const double2* dot_with, CGState* st, double* partials);
+             const double2* dot_with, CGState* st, double* partials, int gate = 0);
 void hop_f32(hipStream_t s, const Geom& g, const float2* U, const float2* in, float2* out, float a, float b,
              const float2* dot_with, CGState* st, double* partials);
 void hop_f16(hipStream_t s, const Geom& g, const __half2* U, const __half2* in, const float* in_norm,
@@ -140,6 +149,13 @@ void acc_f32_into_f64(hipStream_t s, double2* xh, float2* xl, size_t n);  // xh 
 void xpay_f64(hipStream_t s, double2* y, const double2* x, double a, size_t n);  // y = x + a*y
 void axpy_f64(hipStream_t s, double2* y, const double2* x, double a, size_t n);  // y += a*x
 void sub_f64(hipStream_t s, double2* out, const double2* a, const double2* b, size_t n);  // out = a - b
+// Device-side reliable update, gated on the solve state (blas.hip).
+void ru_acc(hipStream_t s, double2* xh, float2* xl, size_t n, CGState* st);
+void ru_sub(hipStream_t s, const Geom& g, double2* r, const double2* phi, const double2* ax, CGState* st,
+            double* partials);
+void ru_apply32(hipStream_t s, const Geom& g, const double2* r64, float2* r, float2* p, const CGState* st);
+void ru_apply16(hipStream_t s, const Geom& g, const double2* r64, __half2* r, float* rn, __half2* p, float* pn,
+                const CGState* st);
 
 // ------------------------------------------------------------------ gauge (gauge.hip)
 int force_field(sfh_ctx* c, const Geom& g, const double2* U, const double2* X, const double2* Y, double beta,

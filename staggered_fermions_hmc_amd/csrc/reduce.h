@@ -81,15 +81,16 @@ struct FinRR {
     }
     st->beta = b;
     if (st->mixed) {
+      // pause for an FP64 reliable update: it decides convergence and the
+      // budget on the true residual
       const double r = sqrt(rr);
-      if (rr < st->target) {
-        st->pause_reason = 2;
-        st->done = 1;
-      } else if (r < st->delta * st->rmax) {
-        st->pause_reason = 1;
-        st->done = 1;
-      } else if (st->iter >= st->max_iter) {
-        st->status = SFH_E_NOCONV;
+      int reason = 0;
+      if (rr < st->target) reason = 2;
+      else if (r < st->delta * st->rmax) reason = 1;
+      else if (st->iter >= st->max_iter) reason = 3;
+      if (reason) {
+        st->pause_reason = reason;
+        st->ru_pending = 1;
         st->done = 1;
       }
       if (r > st->rmax) st->rmax = r;
@@ -101,6 +102,40 @@ struct FinRR {
       st->status = SFH_E_NOCONV;
       st->done = 1;
     }
+  }
+};
+
+// Finisher of the FP64 true residual |phi - DDdag x|^2 of a reliable update
+// (Clark et al., reliable updates): converged, out of budget, or restart the
+// sloppy residual from the true one with beta = |r_true|^2 / |r_prev|^2.
+struct FinRU {
+  CGState* st;
+  __device__ void operator()(double rr) const {
+    st->rsq_true = rr;
+    st->rupd += 1;
+    st->ru_pending = 0;
+    if (rr < st->target) {
+      st->converged = 1;
+      st->done = 1;
+      return;
+    }
+    if (!isfinite(rr)) {
+      st->status = SFH_E_NONFINITE_BETA;
+      st->done = 1;
+      return;
+    }
+    if (st->iter >= st->max_iter) {
+      st->status = SFH_E_NOCONV;
+      st->rsq = rr;
+      st->done = 1;
+      return;
+    }
+    st->beta = rr / st->rsq_prev;
+    st->rsq = rr;
+    st->rmax = sqrt(rr);
+    st->ru_apply = 1;
+    st->pause_reason = 0;
+    st->done = 0;
   }
 };
 

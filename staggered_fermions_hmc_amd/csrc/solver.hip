@@ -81,16 +81,65 @@ int upload_state(sfh_ctx* c, const CGState& s) {
   return hip_check(c, hipStreamSynchronize(c->stream), "state upload");
 }
 
-// Runs iteration chunks until the device state says stop.  launch_iter issues
-// one iteration.  Returns the final state through out.
-template <class F> int run_chunks(sfh_ctx* c, F launch_iter, CGState& out) {
+// A chunk (kChunk iterations and, in mixed precision, the gated reliable
+// update) captured once into a HIP graph and replayed: one launch per chunk
+// instead of 4 kChunk + 5.  Graphs are cached per context by the exact
+// buffers, lattice and parameters they were captured with.  With kernel
+// profiling on, chunks are launched directly so every kernel can be timed.
+template <class F> int launch_chunk(sfh_ctx* c, const std::string& key, F direct) {
+  if (c->profiling) {
+    direct();
+    return SFH_OK;
+  }
+  auto it = c->graphs.find(key);
+  if (it == c->graphs.end()) {
+    if (c->graphs.size() >= 32) {
+      hipStreamSynchronize(c->stream);
+      for (auto& kv : c->graphs) {
+        hipGraphExecDestroy(kv.second.second);
+        hipGraphDestroy(kv.second.first);
+      }
+      c->graphs.clear();
+    }
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+      return set_err(c, SFH_E_HIP, "CG graph capture");
+    direct();
+    if (hipStreamEndCapture(c->stream, &graph) != hipSuccess) return set_err(c, SFH_E_HIP, "CG graph capture end");
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+      hipGraphDestroy(graph);
+      return set_err(c, SFH_E_HIP, "CG graph instantiate");
+    }
+    it = c->graphs.emplace(key, std::make_pair(graph, exec)).first;
+  }
+  return hip_check(c, hipGraphLaunch(it->second.second, c->stream), "CG graph launch");
+}
+
+std::string graph_key(const char* kind, const Geom& g, std::initializer_list<const void*> ptrs, double mass,
+                      int variant) {
+  std::string k = kind;
+  char buf[64];
+  snprintf(buf, sizeof buf, "|%d,%d,%d,%d|%.17g|%d", g.L[0], g.L[1], g.L[2], g.L[3], mass, variant);
+  k += buf;
+  for (const void* p : ptrs) {
+    snprintf(buf, sizeof buf, "|%p", p);
+    k += buf;
+  }
+  return k;
+}
+
+// Runs chunks until the device state says stop; issue_chunk launches one.
+// The state is read back one chunk behind, so the GPU never waits for the
+// host.  Returns the final state through out.
+template <class F> int run_chunks(sfh_ctx* c, F issue_chunk, CGState& out) {
   CGState* ring[2] = {c->h_state + 1, c->h_state + 2};
   hipEvent_t ev[2];
   hipEventCreateWithFlags(&ev[0], hipEventDisableTiming);
   hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
   int k = 0, inflight = 0, rc = SFH_OK;
   auto issue = [&](int slot) {
-    for (int j = 0; j < kChunk; ++j) launch_iter();
+    if (int e = issue_chunk()) rc = e;
     hipMemcpyAsync(ring[slot], c->d_state, sizeof(CGState), hipMemcpyDeviceToHost, c->stream);
     hipEventRecord(ev[slot], c->stream);
   };
@@ -100,6 +149,7 @@ template <class F> int run_chunks(sfh_ctx* c, F launch_iter, CGState& out) {
     const int cur = k & 1;
     issue(cur ^ 1);
     ++inflight;
+    if (rc) break;
     rc = hip_check(c, hipEventSynchronize(ev[cur]), "CG chunk");
     --inflight;
     if (!rc) rc = hip_check(c, hipGetLastError(), "CG launch");
@@ -166,7 +216,12 @@ static int cg_f64(sfh_ctx* c, const sfh_gauge* gauge, const sfh_cg_params* prm, 
       cg_xr_f64(c->stream, g, x, b.r, b.p, b.v, st, part + kPartialSlot);
       cg_p_f64(c->stream, g, b.p, b.r, st);
     };
-    if (int e = run_chunks(c, iter, fin)) return e;
+    auto direct = [&]() {
+      for (int j = 0; j < kChunk; ++j) iter();
+    };
+    const std::string key = graph_key("f64", g, {gauge->U, x, b.r, b.p, b.t, b.v, st, part}, prm->mass,
+                                      get_hop_variant(0, g));
+    if (int e = run_chunks(c, [&]() { return launch_chunk(c, key, direct); }, fin)) return e;
   }
   if (stats) {
     stats->iterations = fin.iter;
@@ -220,6 +275,7 @@ static int cg_mixed(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* prm, cons
   s.mixed = 1;
   s.delta = prm->reliable_delta > 0 ? prm->reliable_delta : 0.1;
   s.rmax = std::sqrt(phi_sq);
+  s.rsq_true = phi_sq;
   if (int e = upload_state(c, s)) return e;
   CGState* st = c->d_state;
   double* part = c->d_partials;
@@ -251,52 +307,41 @@ static int cg_mixed(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* prm, cons
       cg_p_f32(c->stream, g, p32, r32, st);
     }
   };
-  int rupd = 0;
-  double rsq_true = phi_sq;
+  // The reliable update runs on the device at the end of every chunk, gated
+  // on the state: x += x_lo, r = phi - DDdag x in FP64, and either stop
+  // (true residual met, or budget spent) or restart the sloppy r and p.
+  double2* t64 = buf<double2>(c, "cg_t", n);
+  double2* ax = buf<double2>(c, "cg_v", n);
+  if (!t64 || !ax) return SFH_E_NOMEM;
+  double* part_ru = part + 3 * (size_t)kPartialSlot;
+  auto tail = [&]() {
+    KTimer kt(c, "reliable_update");
+    ru_acc(c->stream, x, xl, n, st);
+    hop_f64(c->stream, g, gauge->U, x, t64, prm->mass, -1.0, nullptr, st, nullptr, 1);
+    hop_f64(c->stream, g, gauge->U, t64, ax, prm->mass, +1.0, nullptr, st, nullptr, 1);
+    ru_sub(c->stream, g, r64, phi, ax, st, part_ru);
+    if (HALF) ru_apply16(c->stream, g, r64, r16, rn, p16, pn, st);
+    else ru_apply32(c->stream, g, r64, r32, p32, st);
+  };
+  auto direct = [&]() {
+    for (int j = 0; j < kChunk; ++j) iter();
+    tail();
+  };
+  const std::string key =
+      HALF ? graph_key("f16", g, {gauge->U, gauge->lp.u16, x, xl, r64, phi, t64, ax, r16, p16, t16, v16, rn, pn, tn,
+                                  vn, st, part},
+                       prm->mass, get_hop_variant(2, g) * 2 + get_hop_variant(0, g))
+           : graph_key("f32", g, {gauge->U, gauge->lp.u32, x, xl, r64, phi, t64, ax, r32, p32, t32, v32, st, part},
+                       prm->mass, get_hop_variant(1, g) * 2 + get_hop_variant(0, g));
   CGState fin{};
-  for (;;) {
-    if (int e = run_chunks(c, iter, fin)) return e;
-    if (fin.status != SFH_OK) break;
-    // reliable update: x += x_lo, r = phi - DDdag x in FP64
-    acc_f32_into_f64(c->stream, x, xl, n);
-    double2* ax = buf<double2>(c, "cg_t", n);
-    if (!ax) return SFH_E_NOMEM;
-    if (int e = apply_ddag_dev(c, gauge, prm->mass, x, ax)) return e;
-    sub_f64(c->stream, r64, phi, ax, n);
-    rsq_true = dev_dot(c, r64, r64, n);
-    ++rupd;
-    if (!(rsq_true < s.target)) {
-      if (fin.iter >= prm->max_iter) {
-        fin.status = SFH_E_NOCONV;
-        fin.rsq = rsq_true;
-        break;
-      }
-    } else {
-      fin.rsq = rsq_true;
-      break;
-    }
-    to_sloppy_r();
-    CGState ns = fin;
-    ns.beta = rsq_true / fin.rsq_prev;
-    ns.rsq = rsq_true;
-    ns.rmax = std::sqrt(rsq_true);
-    ns.done = 0;
-    ns.pause_reason = 0;
-    for (int k = 0; k < 8; ++k) ns.cnt[k] = 0;
-    if (int e = upload_state(c, ns)) return e;
-    if (HALF) cg_p_f16(c->stream, g, p16, pn, r16, rn, st);
-    else cg_p_f32(c->stream, g, p32, r32, st);
-  }
+  if (int e = run_chunks(c, [&]() { return launch_chunk(c, key, direct); }, fin)) return e;
   if (stats) {
     stats->iterations = fin.iter;
-    stats->reliable_updates = rupd;
+    stats->reliable_updates = fin.rupd;
     stats->rsq = fin.rsq;
-    stats->true_rel_resid = std::sqrt(rsq_true / phi_sq);
+    stats->true_rel_resid = std::sqrt(fin.rsq_true / phi_sq);
     const double per = HALF ? 528.0 : 912.0;
-    stats->hbm_bytes = (double)fin.iter * per * g.V + rupd * 1700.0 * g.V;
-  }
-  if (fin.status == SFH_E_NOCONV) {
-    fin.rsq = rsq_true;
+    stats->hbm_bytes = (double)fin.iter * per * g.V + fin.rupd * 1700.0 * g.V;
   }
   return map_status(c, fin, prm->max_iter);
 }

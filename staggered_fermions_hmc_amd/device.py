@@ -170,6 +170,14 @@ class Gauge:
                                                   ctypes.byref(us)))
         return v.value, us.value
 
+    def time_hop(self, precision="f64", with_dot=False, n=50):
+        """Average microseconds of the solver's hop kernel on this lattice,
+        n back-to-back launches between HIP events on the context stream."""
+        us = ctypes.c_double()
+        self.ctx.check(self.ctx._lib.sfh_time_hop(self.ctx.h, self.h, _abi.PREC[precision], int(bool(with_dot)),
+                                                  int(n), ctypes.byref(us)))
+        return us.value
+
     def action(self, beta, c11, c12):
         out = ctypes.c_double()
         self.ctx.check(self.ctx._lib.sfh_action(self.ctx.h, self.h, beta, c11, c12, ctypes.byref(out)))
