@@ -225,8 +225,13 @@ int sfh_create(int device, sfh_ctx** out) {
   ok = ok && hipHostMalloc(&c->h_state, 3 * sizeof(CGState), hipHostMallocDefault) == hipSuccess;
   ok = ok && hipHostMalloc(&c->h_scalar, 8 * sizeof(double), hipHostMallocDefault) == hipSuccess;
   ok = ok && hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess;
+  ok = ok && hipMalloc(&c->d_ctr, 2 * kCtrBytes) == hipSuccess;
   if (ok) {
-    ok = hipMemsetAsync(c->d_state, 0, 2 * sizeof(CGState), c->stream) == hipSuccess &&
+    CGState z[2] = {};
+    z[0].ctr = c->d_ctr;
+    z[1].ctr = c->d_ctr + kCtrKinds * kCtrPerKind;
+    ok = hipMemsetAsync(c->d_ctr, 0, 2 * kCtrBytes, c->stream) == hipSuccess &&
+         hipMemcpyAsync(c->d_state, z, sizeof z, hipMemcpyHostToDevice, c->stream) == hipSuccess &&
          hipStreamSynchronize(c->stream) == hipSuccess;
   }
   if (!ok) {
@@ -250,6 +255,7 @@ int sfh_destroy(sfh_ctx* c) {
   for (auto& kv : c->scratch) hipFree(kv.second.first);
   if (c->d_state) hipFree(c->d_state);
   if (c->d_partials) hipFree(c->d_partials);
+  if (c->d_ctr) hipFree(c->d_ctr);
   if (c->h_state) hipHostFree(c->h_state);
   if (c->h_scalar) hipHostFree(c->h_scalar);
   if (c->ev0) hipEventDestroy(c->ev0);
@@ -367,6 +373,7 @@ int sfh_time_hop(sfh_ctx* c, const sfh_gauge* g, int precision, int with_dot, in
   CGState* st = &c->d_state[1];
   CGState z{};
   z.rsq = 1.0;
+  z.ctr = c->d_ctr + kCtrKinds * kCtrPerKind;
   hipMemcpyAsync(st, &z, sizeof(CGState), hipMemcpyHostToDevice, c->stream);
   double* part = c->d_partials + 4 * (size_t)kPartialSlot;
   auto launch = [&]() {
@@ -582,7 +589,7 @@ int sfh_inner(sfh_ctx* c, size_t n, const double* a, const double* b, double* ou
   if (!da || !db) return SFH_E_NOMEM;
   hipMemcpyAsync(da, a, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
   hipMemcpyAsync(db, b, n * sizeof(double), hipMemcpyHostToDevice, c->stream);
-  dot_f64(c->stream, da, db, n, c->d_partials + 2 * kPartialSlot, &c->d_state[1].cnt[CNT_AUX],
+  dot_f64(c->stream, da, db, n, c->d_partials + 2 * kPartialSlot, c->aux_ctr(),
           &c->d_state[1].alpha);
   hipMemcpyAsync(c->h_scalar, &c->d_state[1].alpha, sizeof(double), hipMemcpyDeviceToHost, c->stream);
   if (int e = hip_check(c, hipStreamSynchronize(c->stream), "inner")) return e;
@@ -756,7 +763,7 @@ int sfh_hmc_trajectory(sfh_ctx* c, sfh_gauge* g, const sfh_md_params* p, const d
   {
     double2* tmp = dX;
     dot_f64(c->stream, reinterpret_cast<const double*>(dphi), reinterpret_cast<const double*>(tmp), 2 * n,
-            c->d_partials + 2 * kPartialSlot, &c->d_state[1].cnt[CNT_AUX], &c->d_state[1].alpha);
+            c->d_partials + 2 * kPartialSlot, c->aux_ctr(), &c->d_state[1].alpha);
     hipMemcpyAsync(c->h_scalar, &c->d_state[1].alpha, sizeof(double), hipMemcpyDeviceToHost, c->stream);
     if (int e = hip_check(c, hipStreamSynchronize(c->stream), "S_f")) return e;
     sf_i = *c->h_scalar;
@@ -774,7 +781,7 @@ int sfh_hmc_trajectory(sfh_ctx* c, sfh_gauge* g, const sfh_md_params* p, const d
   const double tp_f = tr_p2(c, gg, dP);
   const double sg_f = gauge_action(c, gg, g->U, p->beta, p->c11, p->c12);
   dot_f64(c->stream, reinterpret_cast<const double*>(dphi), reinterpret_cast<const double*>(dX), 2 * n,
-          c->d_partials + 2 * kPartialSlot, &c->d_state[1].cnt[CNT_AUX], &c->d_state[1].alpha);
+          c->d_partials + 2 * kPartialSlot, c->aux_ctr(), &c->d_state[1].alpha);
   hipMemcpyAsync(c->h_scalar, &c->d_state[1].alpha, sizeof(double), hipMemcpyDeviceToHost, c->stream);
   if (int e = hip_check(c, hipStreamSynchronize(c->stream), "S_f final")) return e;
   const double sf_f = *c->h_scalar;

@@ -82,6 +82,7 @@ template <class T> class Engine {
     const double2* b = nullptr;
     double2* xh = nullptr;
     double2* r64 = nullptr;
+    unsigned* ctr = nullptr;  // this slot's reduction counters
     long long ticket = 0;
     std::chrono::steady_clock::time_point t_admit;
     CGState last{};

@@ -102,7 +102,7 @@ __global__ __launch_bounds__(256) void kb_hop(const BSys<T>* __restrict__ sys, c
         dotv = (double)w.re * (double)re + (double)w.im * (double)im;
       }
     }
-    if constexpr (DOT) grid_reduce(dotv, L.part, &L.st->cnt[CNT_PV], S.nb_hop, lb, FinPV{L.st});
+    if constexpr (DOT) grid_reduce(dotv, L.part, ctr_of(L.st->ctr, CNT_PV), S.nb_hop, lb, FinPV{L.st});
   }
 }
 
@@ -134,7 +134,7 @@ __global__ __launch_bounds__(kBlas) void kb_xr(const BSlot<T>* __restrict__ slot
       rr += (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
     }
   }
-  grid_reduce(rr, L.part, &st->cnt[CNT_RR], nb, m.y, FinRR{st});
+  grid_reduce(rr, L.part, ctr_of(st->ctr, CNT_RR), nb, m.y, FinRR{st});
 }
 
 template <class T>
@@ -168,7 +168,7 @@ size_t Engine<T>::slot_bytes(int V) {
   const size_t n = 3 * (size_t)V;
   const int nbh = (V / 2 + 31) / 32, nbb = (V + kBlas - 1) / kBlas;
   return align256(n * sizeof(S)) * (kMixed ? 5 : 4) + (kMixed ? align256(n * sizeof(double2)) : 0) +
-         align256(std::max(nbh, nbb) * sizeof(double)) + align256(sizeof(CGState));
+         align256(std::max(nbh, nbb) * sizeof(double)) + align256(sizeof(CGState)) + align256(kCtrBytes);
 }
 
 template <class T>
@@ -210,6 +210,9 @@ int Engine<T>::add_system(const sfh_gauge* gauge, double mass, int pool) {
   char* base = nullptr;
   if (hipMalloc(&base, sb * pool) != hipSuccess) return -set_err(c_, SFH_E_NOMEM, "slot pool allocation");
   owned_.push_back(base);
+  // reduction counters must start at zero
+  if (hipMemsetAsync(base, 0, sb * pool, c_->stream) != hipSuccess || hipStreamSynchronize(c_->stream) != hipSuccess)
+    return -set_err(c_, SFH_E_HIP, "slot pool clear");
   bytes_ += sb * pool;
   const size_t n = 3 * (size_t)V;
   for (int k = 0; k < pool; ++k) {
@@ -233,6 +236,7 @@ int Engine<T>::add_system(const sfh_gauge* gauge, double mass, int pool) {
     L.v = reinterpret_cast<S*>(take(n * sizeof(S)));
     if (kMixed) h.r64 = reinterpret_cast<double2*>(take(n * sizeof(double2)));
     L.st = reinterpret_cast<CGState*>(take(sizeof(CGState)));
+    h.ctr = reinterpret_cast<unsigned*>(take(kCtrBytes));
     L.part = reinterpret_cast<double*>(p);
     const int nbh = (V / 2 + 31) / 32, nbb = (V + kBlas - 1) / kBlas;
     const size_t used = (size_t)(p - (base + sb * k)) + align256(std::max(nbh, nbb) * sizeof(double));
@@ -285,6 +289,7 @@ int Engine<T>::admit(int sys, const double2* b, double2* x, long long ticket) {
   z.mixed = kMixed ? 1 : 0;
   z.delta = delta_;
   z.rmax = std::sqrt(phi_sq);
+  z.ctr = h.ctr;
   if (phi_sq < 1e-28) {  // x = 0 (Fermions.cpp:562-564)
     z.done = 1;
     h.finished = true;
@@ -427,7 +432,6 @@ int Engine<T>::step() {
     ns.done = 0;
     ns.pause_reason = 0;
     ns.ru_pending = 0;
-    for (int q = 0; q < 8; ++q) ns.cnt[q] = 0;
     hipMemcpyAsync(h.dev.st, &ns, sizeof(CGState), hipMemcpyHostToDevice, c_->stream);
     cg_p_f32(c_->stream, s.gauge->g, reinterpret_cast<float2*>(h.dev.p), reinterpret_cast<const float2*>(h.dev.r),
              h.dev.st);

@@ -56,7 +56,7 @@ __global__ __launch_bounds__(kB) void k_cg_xr(int V, S* __restrict__ x, S* __res
       rr += (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
     }
   }
-  grid_reduce(rr, partials, &st->cnt[CNT_RR], nb, blockIdx.x, FinRR{st});
+  grid_reduce(rr, partials, ctr_of(st->ctr, CNT_RR), nb, blockIdx.x, FinRR{st});
 }
 
 // p = r + beta p
@@ -142,7 +142,7 @@ __global__ __launch_bounds__(kB) void k_cg_xr16(int V, float2* __restrict__ x, _
     }
     st16(r, rn, V, i, rv);
   }
-  grid_reduce(rr, partials, &st->cnt[CNT_RR], nb, blockIdx.x, FinRR{st});
+  grid_reduce(rr, partials, ctr_of(st->ctr, CNT_RR), nb, blockIdx.x, FinRR{st});
 }
 
 __global__ __launch_bounds__(kB) void k_cg_p16(int V, __half2* __restrict__ p, float* __restrict__ pn,
@@ -285,7 +285,7 @@ __global__ __launch_bounds__(kB) void k_ru_sub(int V, double2* __restrict__ r, c
       rr += d.x * d.x + d.y * d.y;
     }
   }
-  grid_reduce(rr, partials, &st->cnt[CNT_RU], nb, blockIdx.x, FinRU{st});
+  grid_reduce(rr, partials, ctr_of(st->ctr, CNT_RU), nb, blockIdx.x, FinRU{st});
 }
 
 __global__ __launch_bounds__(kB) void k_ru_apply32(int V, const double2* __restrict__ r64, float2* __restrict__ r,

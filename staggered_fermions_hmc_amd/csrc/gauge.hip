@@ -265,7 +265,7 @@ static double link_sum(sfh_ctx* c, const Geom& g, const double2* U, double beta,
   if (nb > kPartialSlot) nb = kPartialSlot;
   KTimer kt(c, "force");
   hipLaunchKernelGGL(k_link_sum, dim3(nb), dim3(128), 0, c->stream, g, U, beta, c11, c12, kind,
-                     c->d_partials + 2 * kPartialSlot, &c->d_state[1].cnt[CNT_AUX], &c->d_state[1].pv, nb);
+                     c->d_partials + 2 * kPartialSlot, c->aux_ctr(), &c->d_state[1].pv, nb);
   hipMemcpyAsync(c->h_scalar, &c->d_state[1].pv, sizeof(double), hipMemcpyDeviceToHost, c->stream);
   if (hip_check(c, hipStreamSynchronize(c->stream), "link reduction")) return NAN;
   return *c->h_scalar;

@@ -29,10 +29,20 @@ struct CGState {
   int ru_apply;      // mixed precision: restart the sloppy residual and direction
   int rupd;          // reliable updates done
   int converged;     // mixed precision: the FP64 true residual met the target
-  unsigned cnt[8];   // grid-reduction arrival counters
+  unsigned* ctr;     // grid-reduction arrival counters (kCtrKinds x kCtrPerKind, device)
 };
 
 enum { CNT_PV = 0, CNT_RR = 1, CNT_AUX = 2, CNT_RU = 3 };
+
+// Arrival counting of the grid reductions is split over kRedGroups counters
+// (each on its own 128-byte line) and one top counter, so no single address
+// takes more than kRedGroups atomics per reduction.
+constexpr int kRedGroups = 64;
+constexpr int kCtrStride = 32;
+constexpr int kCtrPerKind = (kRedGroups + 1) * kCtrStride;
+constexpr int kCtrKinds = 4;
+constexpr size_t kCtrBytes = sizeof(unsigned) * kCtrKinds * kCtrPerKind;
+SFH_HD unsigned* ctr_of(unsigned* arena, int kind) { return arena + kind * kCtrPerKind; }
 
 // Low-precision link / spinor storage (FP16 uses a per-site FP32 norm).
 struct LinksLP {
@@ -64,7 +74,9 @@ struct sfh_ctx {
   // pinned host staging for scalars
   sfh::CGState* h_state = nullptr;
   sfh::CGState* d_state = nullptr;
-  double* d_partials = nullptr;  // reduction partials (8 slots x 65536)
+  double* d_partials = nullptr;  // reduction partials (8 slots of kPartialSlot)
+  unsigned* d_ctr = nullptr;     // reduction counters of d_state[0] and d_state[1] (2 x kCtrBytes)
+  unsigned* aux_ctr() const { return sfh::ctr_of(d_ctr + sfh::kCtrKinds * sfh::kCtrPerKind, sfh::CNT_AUX); }
   double* h_scalar = nullptr;
   bool profiling = false;
   int prof_every = 1;   // time one launch in prof_every per kernel name

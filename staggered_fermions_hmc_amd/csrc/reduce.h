@@ -1,5 +1,5 @@
 // Deterministic grid reductions: every workgroup writes its partial sum, the
-// last workgroup to arrive (arrival counted with one device-scope atomic)
+// last workgroup to arrive (arrivals counted with device-scope atomics)
 // sums all partials in index order and runs the finisher.  The result does
 // not depend on arrival order, only on the grid shape, so it is reproducible
 // run to run and independent of what else shares the GPU.
@@ -18,16 +18,30 @@ namespace sfh {
 // partials with agent-scope atomic loads.  No L2 write-back or invalidate is
 // issued, so the kernel's other traffic keeps its cache reuse.  What the
 // finisher writes is read by later kernels, after the kernel boundary.
+//
+// Arrivals are counted in two levels (kRedGroups group counters on separate
+// cache lines, then one top counter): same-address atomics serialise in the
+// memory system, so one counter for thousands of workgroups would add
+// microseconds to every reduction.
 template <class Fin>
-__device__ __forceinline__ void grid_reduce(double v, double* partials, unsigned* counter, int nblocks, int slot,
+__device__ __forceinline__ void grid_reduce(double v, double* partials, unsigned* ctr, int nblocks, int slot,
                                             Fin fin) {
   const double bs = block_sum(v);
+  const int ng = nblocks < kRedGroups ? nblocks : kRedGroups;
   __shared__ int is_last;
   if (threadIdx.x == 0) {
     __hip_atomic_store(&partials[slot], bs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __builtin_amdgcn_s_waitcnt(0);  // the partial's store has completed before the arrival is counted
-    const unsigned t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    is_last = (t == unsigned(nblocks - 1));
+    const int grp = slot % ng;
+    const unsigned gsize = unsigned(nblocks / ng + (grp < nblocks % ng ? 1 : 0));
+    int last = 0;
+    const unsigned t = __hip_atomic_fetch_add(&ctr[grp * kCtrStride], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (t == gsize - 1) {
+      const unsigned u =
+          __hip_atomic_fetch_add(&ctr[kRedGroups * kCtrStride], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      last = (u == unsigned(ng - 1));
+    }
+    is_last = last;
   }
   __syncthreads();
   if (is_last) {
@@ -35,9 +49,11 @@ __device__ __forceinline__ void grid_reduce(double v, double* partials, unsigned
     for (int k = threadIdx.x; k < nblocks; k += blockDim.x)
       s += __hip_atomic_load(&partials[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     s = block_sum(s);
+    for (int k = threadIdx.x; k < ng; k += blockDim.x)
+      __hip_atomic_store(&ctr[k * kCtrStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (threadIdx.x == 0) {
       fin(s);
-      __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ctr[kRedGroups * kCtrStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }

@@ -38,7 +38,7 @@ int map_status(sfh_ctx* c, const CGState& s, int max_iter) {
 
 double dev_dot(sfh_ctx* c, const double2* a, const double2* b, size_t ncomplex) {
   dot_f64(c->stream, reinterpret_cast<const double*>(a), reinterpret_cast<const double*>(b), 2 * ncomplex,
-          c->d_partials + 2 * kPartialSlot, &c->d_state[1].cnt[CNT_AUX], &c->d_state[1].alpha);
+          c->d_partials + 2 * kPartialSlot, c->aux_ctr(), &c->d_state[1].alpha);
   hipMemcpyAsync(c->h_scalar, &c->d_state[1].alpha, sizeof(double), hipMemcpyDeviceToHost, c->stream);
   if (hip_check(c, hipStreamSynchronize(c->stream), "dot")) return NAN;
   return *c->h_scalar;
@@ -77,6 +77,7 @@ template <class T> T* buf(sfh_ctx* c, const char* key, size_t count) {
 
 int upload_state(sfh_ctx* c, const CGState& s) {
   *c->h_state = s;
+  c->h_state->ctr = c->d_ctr;  // d_state[0]'s counters (zero between reductions)
   hipMemcpyAsync(c->d_state, c->h_state, sizeof(CGState), hipMemcpyHostToDevice, c->stream);
   return hip_check(c, hipStreamSynchronize(c->stream), "state upload");
 }
