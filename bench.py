@@ -8,9 +8,11 @@ on the same inputs is timed beside it.  Inputs are resident in HBM before the
 timed region starts.
 
 Prints one JSON line: value = mixed-precision solves/s, the FP64 solves/s,
-p50/p99 solve latency, the Dslash (hop) HBM roofline of the dominant kernel
-measured with HIP events on the library stream over the timed region, and
-the CPU baseline (the FP64 oracle, timed on a bounded sample on the host).
+p50/p99 solve latency, the HBM roofline of the dominant kernel of the mixed
+solve (the second hop of each iteration with its dot products) and of the
+Dslash (hop) alone in both precisions, measured with HIP events on the
+library stream, and the CPU baseline (the FP64 oracle, timed on a bounded
+sample on the host).
 
 ``--workload 32^4`` runs the 32^4, m = 0.01 solve with FP16 inner iterations
 instead.
@@ -44,6 +46,11 @@ WORKLOADS = {
 HOP_BYTES_PER_SITE = {"f64": 72 * 8 + 12 * 8, "f32": 72 * 4 + 12 * 4, "f16": 72 * 2 + 12 * 2 + 2 * 4}
 # The hop that closes each CG iteration also reads p for <p, v>.
 DOT_EXTRA_PER_SITE = {"f64": 48, "f32": 24, "f16": 16}
+# The two steps of one mixed-precision iteration, compulsory bytes per site:
+# A reads links, r, v, p, x and writes r, p, x, t; B reads links, t, p, r and
+# writes v (FP16 vectors carry one FP32 norm per site; x is FP32).
+MIX_A_BYTES_PER_SITE = {"f32": 288 + 3 * 24 + 2 * 24 + 2 * 24 + 24, "f16": 144 + 3 * 16 + 2 * 16 + 2 * 24 + 16}
+MIX_B_BYTES_PER_SITE = {"f32": 288 + 24 + 24 + 2 * 24, "f16": 144 + 16 + 16 + 2 * 16}
 
 
 def log(*a):
@@ -278,6 +285,11 @@ def main():
                                                      "sequential FP64 capacity")
     ap.add_argument("--requests", type=int, default=200, help="serve: requests per arrival rate")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only the kernel timing loops of the workload (the hop in FP64 and in the inner precision, "
+                         "the two steps of the mixed iteration), e.g. under a hardware-counter profiler; the first "
+                         "use of the low-precision link copy also runs its conversion kernel once (reads 576 B, "
+                         "writes 288 B (FP32) or 144 B (FP16) per site)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a "
@@ -300,6 +312,16 @@ def main():
 
     ctx = dev.Context(0)
     g = dev.Gauge(ctx, dims, U)
+    if args.kernels_only:
+        out = {"workload": args.workload, "lattice": "x".join(map(str, dims))}
+        for prec in (wl["mixed"], "f64"):
+            for dot in (False, True):
+                us = g.time_hop(prec, with_dot=dot, n=20)
+                out[f"hop{'_dot' if dot else ''}_{prec}_us"] = round(us, 2)
+        ua, ub = g.time_cg_kernels(wl["mixed"], n=20)
+        out[f"mix_a_{wl['mixed']}_us"], out[f"mix_b_{wl['mixed']}_us"] = round(ua, 2), round(ub, 2)
+        print(json.dumps(out))
+        return
     phi = dev.Field(g, b)
     x = dev.Field(g, None)
     tol = 1e-10
@@ -329,15 +351,17 @@ def main():
         # chunk graphs, so each kernel can be timed)
         ctx.set_profiling(1)
         solve(prec)
-        kt = {k: ctx.kernel_time(f"{k}_{prec}") for k in ("hop", "hop_dot")}
-        kt["cg_blas"] = ctx.kernel_time("cg_blas")
-        kt["reliable_update"] = ctx.kernel_time("reliable_update")
+        names = ([f"hop_{prec}", f"hop_dot_{prec}", "cg_blas"] if prec == "f64" else
+                 [f"mix_a_{prec}", "mix_upd", f"hop_{prec}", f"mix_b_{prec}", "reliable_update"])
+        kt = {k: ctx.kernel_time(k) for k in names}
         ctx.set_profiling(0)
         # the hop kernel alone: back-to-back launches between two HIP events
         hop_us = g.time_hop(prec, with_dot=False, n=50)
         hop_dot_us = g.time_hop(prec, with_dot=True, n=50)
+        # the two steps of the mixed iteration as the solver runs them
+        mix_us = g.time_cg_kernels(prec, n=50) if prec != "f64" else None
         res[prec] = dict(total=total, lat=lat, stats=stats, kt=kt, true_res=true_res, hop_us=hop_us,
-                         hop_dot_us=hop_dot_us)
+                         hop_dot_us=hop_dot_us, mix_us=mix_us)
         log(f"[bench] {prec}: {args.steps} solves in {total:.3f}s, iters {stats['iterations']}, "
             f"reliable updates {stats['reliable_updates']}, true residual {true_res:.2e}, hop {hop_us:.2f} us")
 
@@ -358,24 +382,39 @@ def main():
 
     hm = hop_gbs(wl["mixed"], mixed)
     h64 = hop_gbs("f64", f64)
-    # dominant kernel of the headline solve: the hop in the inner precision
-    kern = f"hop_{wl['mixed']}"
+    mp = wl["mixed"]
+
+    def gbs(bytes_per_site, us):
+        return bytes_per_site * V / (us * 1e-6) / 1e9
+
+    # Dominant kernel of the timed (mixed) solve: step B of the iteration,
+    # the second hop with its four dot products; step A is the first hop with
+    # the vector updates.  Both timed as the solver launches them.
+    us_a, us_b = mixed["mix_us"]
+    kern = f"mix_b_{mp}"
+    achieved = gbs(MIX_B_BYTES_PER_SITE[mp], us_b)
     traffic = traffic_from_profiles(kern, args.workload)
     roofline = {
         "bound": "hbm",
         "kernel": kern,
-        "achieved": round(hm["hop_gbs"], 1),
+        "achieved": round(achieved, 1),
         "peak": PEAK_HBM_GBS,
         "unit": "GB/s",
-        "frac": round(hm["hop_gbs"] / PEAK_HBM_GBS, 4),
+        "frac": round(achieved / PEAK_HBM_GBS, 4),
         "traffic": traffic,
-        "algorithmic_bytes_per_launch": HOP_BYTES_PER_SITE[wl["mixed"]] * V,
-        "avg_launch_us": round(hm["hop_avg_us"], 2),
-        "fp64_hop": {"achieved": round(h64["hop_gbs"], 1), "frac": round(h64["hop_gbs"] / PEAK_HBM_GBS, 4),
-                     "avg_launch_us": round(h64["hop_avg_us"], 2),
-                     "algorithmic_bytes_per_launch": HOP_BYTES_PER_SITE["f64"] * V},
-        "timing": "HIP events on the library stream around 50 back-to-back launches of the solver's hop kernel "
-                  "at this lattice",
+        "algorithmic_bytes_per_launch": MIX_B_BYTES_PER_SITE[mp] * V,
+        "avg_launch_us": round(us_b, 2),
+        "step_a": {"achieved": round(gbs(MIX_A_BYTES_PER_SITE[mp], us_a), 1),
+                   "frac": round(gbs(MIX_A_BYTES_PER_SITE[mp], us_a) / PEAK_HBM_GBS, 4), "avg_us": round(us_a, 2),
+                   "algorithmic_bytes": MIX_A_BYTES_PER_SITE[mp] * V},
+        "dslash": {"precision": mp, "achieved": round(hm["hop_gbs"], 1),
+                   "frac": round(hm["hop_gbs"] / PEAK_HBM_GBS, 4), "avg_launch_us": round(hm["hop_avg_us"], 2),
+                   "algorithmic_bytes_per_launch": HOP_BYTES_PER_SITE[mp] * V},
+        "dslash_fp64": {"achieved": round(h64["hop_gbs"], 1), "frac": round(h64["hop_gbs"] / PEAK_HBM_GBS, 4),
+                        "avg_launch_us": round(h64["hop_avg_us"], 2),
+                        "algorithmic_bytes_per_launch": HOP_BYTES_PER_SITE["f64"] * V},
+        "timing": "HIP events on the library stream around 50 back-to-back launches of each kernel as the solver "
+                  "runs it at this lattice",
     }
 
     cpu = None

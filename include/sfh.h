@@ -134,6 +134,23 @@ int sfh_tune_hop(sfh_ctx* ctx, const sfh_gauge* g, int precision, int n, int* va
  * n back-to-back launches between HIP events on the context stream.  No
  * counterpart in the reference. */
 int sfh_time_hop(sfh_ctx* ctx, const sfh_gauge* g, int precision, int with_dot, int n, double* us);
+/* Form of the mixed-precision CG iteration, per precision (1 f32, 2 f16):
+ * 0 = fused (the first hop recomputes the vector updates at its neighbours,
+ * two kernels per iteration), 1 = split (in-place update kernel, then the
+ * plain hop: three kernels), -1 = choose by lattice size (the default).
+ * Process-wide.  No counterpart in the reference: a tuning knob. */
+int sfh_set_cg_form(int precision, int form);
+/* Times both forms on g's lattice (sfh_time_cg_kernels, n repetitions),
+ * fixes the faster one for that precision and returns it with its time per
+ * iteration in microseconds.  No counterpart in the reference. */
+int sfh_tune_cg_form(sfh_ctx* ctx, const sfh_gauge* g, int precision, int n, int* form, double* us);
+/* Average durations in microseconds of the two steps of one mixed-precision
+ * CG iteration (precision SFH_PREC_F32 or SFH_PREC_F16) as the solver runs
+ * them for this lattice: us_a = the update and first hop (one kernel, or the
+ * in-place update and the plain hop on large lattices), us_b = the second hop
+ * with its four dot products.  n back-to-back repetitions of each between
+ * HIP events on the context stream.  No counterpart in the reference. */
+int sfh_time_cg_kernels(sfh_ctx* ctx, const sfh_gauge* g, int precision, int n, double* us_a, double* us_b);
 
 /* ---------------------------------------------------------------- gauge field
  * Device-resident links for one lattice.  dims = {L0, L1, L2, L3}, all even,

@@ -66,6 +66,9 @@ SIGNATURES = {
     "sfh_set_hop_variant": (_I, [_I, _I]),
     "sfh_tune_hop": (_I, [_P, _P, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_D)]),
     "sfh_time_hop": (_I, [_P, _P, _I, _I, _I, ctypes.POINTER(_D)]),
+    "sfh_time_cg_kernels": (_I, [_P, _P, _I, _I, ctypes.POINTER(_D), ctypes.POINTER(_D)]),
+    "sfh_set_cg_form": (_I, [_I, _I]),
+    "sfh_tune_cg_form": (_I, [_P, _P, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_D)]),
     "sfh_gauge_create": (_I, [_P, _P, ctypes.POINTER(_P)]),
     "sfh_gauge_destroy": (_I, [_P, _P]),
     "sfh_gauge_upload": (_I, [_P, _P, _P]),

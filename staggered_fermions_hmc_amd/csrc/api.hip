@@ -398,6 +398,42 @@ int sfh_time_hop(sfh_ctx* c, const sfh_gauge* g, int precision, int with_dot, in
   return SFH_OK;
 }
 
+int sfh_time_cg_kernels(sfh_ctx* c, const sfh_gauge* g, int precision, int n, double* us_a, double* us_b) {
+  if (!c || !g || (precision != SFH_PREC_F32 && precision != SFH_PREC_F16) || n < 1 || !us_a || !us_b)
+    return SFH_E_INVALID;
+  Guard gd(c);
+  return time_mixed(c, const_cast<sfh_gauge*>(g), precision, n, us_a, us_b);
+}
+
+int sfh_set_cg_form(int precision, int form) {
+  if ((precision != SFH_PREC_F32 && precision != SFH_PREC_F16) || form < -1 || form > 1) return SFH_E_INVALID;
+  set_mix_form(precision, form);
+  return SFH_OK;
+}
+
+int sfh_tune_cg_form(sfh_ctx* c, const sfh_gauge* g, int precision, int n, int* form, double* us) {
+  if (!c || !g || (precision != SFH_PREC_F32 && precision != SFH_PREC_F16) || n < 1) return SFH_E_INVALID;
+  Guard gd(c);
+  double best_us = 1e30;
+  int best = 0;
+  for (int f = 0; f < 2; ++f) {
+    set_mix_form(precision, f);
+    double ua = 0.0, ub = 0.0;
+    if (int e = time_mixed(c, const_cast<sfh_gauge*>(g), precision, n, &ua, &ub)) {
+      set_mix_form(precision, -1);
+      return e;
+    }
+    if (ua + ub < best_us) {
+      best_us = ua + ub;
+      best = f;
+    }
+  }
+  set_mix_form(precision, best);
+  if (form) *form = best;
+  if (us) *us = best_us;
+  return SFH_OK;
+}
+
 // ---------------------------------------------------------------- gauge
 int sfh_gauge_create(sfh_ctx* c, const int dims[4], sfh_gauge** out) {
   if (!c || !out) return SFH_E_INVALID;

@@ -168,7 +168,7 @@ size_t Engine<T>::slot_bytes(int V) {
   const size_t n = 3 * (size_t)V;
   const int nbh = (V / 2 + 31) / 32, nbb = (V + kBlas - 1) / kBlas;
   return align256(n * sizeof(S)) * (kMixed ? 5 : 4) + (kMixed ? align256(n * sizeof(double2)) : 0) +
-         align256(std::max(nbh, nbb) * sizeof(double)) + align256(sizeof(CGState)) + align256(kCtrBytes);
+         align256((std::max(nbh, nbb) + kRedGroups) * sizeof(double)) + align256(sizeof(CGState)) + align256(kCtrBytes);
 }
 
 template <class T>
@@ -239,7 +239,7 @@ int Engine<T>::add_system(const sfh_gauge* gauge, double mass, int pool) {
     h.ctr = reinterpret_cast<unsigned*>(take(kCtrBytes));
     L.part = reinterpret_cast<double*>(p);
     const int nbh = (V / 2 + 31) / 32, nbb = (V + kBlas - 1) / kBlas;
-    const size_t used = (size_t)(p - (base + sb * k)) + align256(std::max(nbh, nbb) * sizeof(double));
+    const size_t used = (size_t)(p - (base + sb * k)) + align256((std::max(nbh, nbb) + kRedGroups) * sizeof(double));
     if (used > sb) return -set_err(c_, SFH_E_INVALID, "slot layout exceeds its allocation");
     s.pool.push_back((int)slots_.size());
     slots_.push_back(h);

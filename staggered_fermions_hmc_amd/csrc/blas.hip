@@ -25,7 +25,7 @@ __global__ __launch_bounds__(kB) void k_dot(const double* __restrict__ a, const 
 void dot_f64(hipStream_t s, const double* a, const double* b, size_t n, double* partials, unsigned* cnt,
              double* result) {
   size_t per = 4;
-  while (nblk((n + per - 1) / per) > (unsigned)kPartialSlot) per *= 2;
+  while (nblk((n + per - 1) / per) > (unsigned)(kPartialSlot - kRedGroups)) per *= 2;
   const unsigned nb = nblk((n + per - 1) / per);
   hipLaunchKernelGGL(k_dot, dim3(nb), dim3(kB), 0, s, a, b, n, per, partials, cnt, result, (int)nb);
 }
@@ -87,24 +87,11 @@ void cg_xr_f64(hipStream_t s, const Geom& g, double2* x, double2* r, const doubl
 void cg_p_f64(hipStream_t s, const Geom& g, double2* p, const double2* r, CGState* st) {
   hipLaunchKernelGGL(k_cg_p<double2>, dim3(nblk(g.V)), dim3(kB), 0, s, g.V, p, r, st);
 }
-void cg_xr_f32(hipStream_t s, const Geom& g, float2* x, float2* r, const float2* p, const float2* v, CGState* st,
-               double* partials) {
-  const unsigned nb = nblk(g.V);
-  hipLaunchKernelGGL(k_cg_xr<float2>, dim3(nb), dim3(kB), 0, s, g.V, x, r, p, v, st, partials, (int)nb);
-}
 void cg_p_f32(hipStream_t s, const Geom& g, float2* p, const float2* r, CGState* st) {
   hipLaunchKernelGGL(k_cg_p<float2>, dim3(nblk(g.V)), dim3(kB), 0, s, g.V, p, r, st);
 }
 
 // ------------------------------------------------------------------ FP16 (per-site norm) CG updates
-__device__ __forceinline__ void ld16(const __half2* f, const float* nrm, int V, int i, float2 v[3]) {
-  const float s = nrm[i];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const float2 t = __half22float2(f[(size_t)c * V + i]);
-    v[c] = make_float2(s * t.x, s * t.y);
-  }
-}
 __device__ __forceinline__ void st16(__half2* f, float* nrm, int V, int i, const float2 v[3]) {
   float m = 0.f;
 #pragma unroll
@@ -113,64 +100,6 @@ __device__ __forceinline__ void st16(__half2* f, float* nrm, int V, int i, const
   nrm[i] = m;
 #pragma unroll
   for (int c = 0; c < 3; ++c) f[(size_t)c * V + i] = __floats2half2_rn(v[c].x * inv, v[c].y * inv);
-}
-
-__global__ __launch_bounds__(kB) void k_cg_xr16(int V, float2* __restrict__ x, __half2* __restrict__ r,
-                                                float* __restrict__ rn, const __half2* __restrict__ p,
-                                                const float* __restrict__ pn, const __half2* __restrict__ v,
-                                                const float* __restrict__ vn, CGState* st, double* partials,
-                                                int nb) {
-  if (st->done) return;
-  const int i = blockIdx.x * kB + threadIdx.x;
-  const float al = (float)st->alpha;
-  double rr = 0.0;
-  if (i < V) {
-    float2 pp[3], vv[3], rv[3];
-    ld16(p, pn, V, i, pp);
-    ld16(v, vn, V, i, vv);
-    ld16(r, rn, V, i, rv);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const size_t j = (size_t)c * V + i;
-      float2 xx = x[j];
-      xx.x += al * pp[c].x;
-      xx.y += al * pp[c].y;
-      x[j] = xx;
-      rv[c].x -= al * vv[c].x;
-      rv[c].y -= al * vv[c].y;
-      rr += (double)rv[c].x * rv[c].x + (double)rv[c].y * rv[c].y;
-    }
-    st16(r, rn, V, i, rv);
-  }
-  grid_reduce(rr, partials, ctr_of(st->ctr, CNT_RR), nb, blockIdx.x, FinRR{st});
-}
-
-__global__ __launch_bounds__(kB) void k_cg_p16(int V, __half2* __restrict__ p, float* __restrict__ pn,
-                                               const __half2* __restrict__ r, const float* __restrict__ rn,
-                                               CGState* st) {
-  if (st->done) return;
-  const int i = blockIdx.x * kB + threadIdx.x;
-  if (i >= V) return;
-  const float be = (float)st->beta;
-  float2 pp[3], rv[3];
-  ld16(p, pn, V, i, pp);
-  ld16(r, rn, V, i, rv);
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    pp[c].x = rv[c].x + be * pp[c].x;
-    pp[c].y = rv[c].y + be * pp[c].y;
-  }
-  st16(p, pn, V, i, pp);
-}
-
-void cg_xr_f16(hipStream_t s, const Geom& g, float2* x, __half2* r, float* rn, const __half2* p, const float* pn,
-               const __half2* v, const float* vn, CGState* st, double* partials) {
-  const unsigned nb = nblk(g.V);
-  hipLaunchKernelGGL(k_cg_xr16, dim3(nb), dim3(kB), 0, s, g.V, x, r, rn, p, pn, v, vn, st, partials, (int)nb);
-}
-void cg_p_f16(hipStream_t s, const Geom& g, __half2* p, float* pn, const __half2* r, const float* rn,
-              CGState* st) {
-  hipLaunchKernelGGL(k_cg_p16, dim3(nblk(g.V)), dim3(kB), 0, s, g.V, p, pn, r, rn, st);
 }
 
 // ------------------------------------------------------------------ precision conversion / FP64 BLAS
@@ -249,27 +178,9 @@ void sub_f64(hipStream_t s, double2* out, const double2* a, const double2* b, si
 }
 
 // ------------------------------------------------------------------ device-side reliable update
-// Run at the end of every chunk of mixed-precision iterations; each kernel
-// returns at once unless a reliable update is pending (ru_pending) or has
-// just restarted the solve (ru_apply), so no host round trip is needed.
-//   1 k_ru_acc   : x += x_lo, x_lo = 0
-//   2,3 hop_f64  : ax = D (D^dag x)                  (gate 1)
-//   4 k_ru_sub   : r64 = phi - ax, |r64|^2 -> FinRU
-//   5 k_ru_apply : r_lo = r64 in sloppy storage, p = r_lo + beta p
-__global__ __launch_bounds__(kB) void k_ru_acc(size_t n, double2* __restrict__ xh, float2* __restrict__ xl,
-                                               CGState* st) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->ru_apply = 0;  // read only by the apply kernel of the next chunk
-  if (!st->ru_pending) return;
-  const size_t i = (size_t)blockIdx.x * kB + threadIdx.x;
-  if (i >= n) return;
-  const float2 l = xl[i];
-  double2 h = xh[i];
-  h.x += (double)l.x;
-  h.y += (double)l.y;
-  xh[i] = h;
-  xl[i] = make_float2(0.f, 0.f);
-}
-
+// The FP64 true residual of a reliable update, r64 = phi - D D^dag x, and its
+// norm (finisher FinRU); returns at once unless an update is pending, so the
+// mixed solver can run it at the end of every chunk with no host round trip.
 __global__ __launch_bounds__(kB) void k_ru_sub(int V, double2* __restrict__ r, const double2* __restrict__ phi,
                                                const double2* __restrict__ ax, CGState* st, double* partials, int nb) {
   if (!st->ru_pending) return;
@@ -288,59 +199,10 @@ __global__ __launch_bounds__(kB) void k_ru_sub(int V, double2* __restrict__ r, c
   grid_reduce(rr, partials, ctr_of(st->ctr, CNT_RU), nb, blockIdx.x, FinRU{st});
 }
 
-__global__ __launch_bounds__(kB) void k_ru_apply32(int V, const double2* __restrict__ r64, float2* __restrict__ r,
-                                                   float2* __restrict__ p, const CGState* st) {
-  if (!st->ru_apply) return;
-  const int i = blockIdx.x * kB + threadIdx.x;
-  if (i >= V) return;
-  const float be = (float)st->beta;
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const size_t j = (size_t)c * V + i;
-    const double2 t = r64[j];
-    const float2 rv = make_float2((float)t.x, (float)t.y);
-    float2 pp = p[j];
-    pp.x = rv.x + be * pp.x;
-    pp.y = rv.y + be * pp.y;
-    r[j] = rv;
-    p[j] = pp;
-  }
-}
-
-__global__ __launch_bounds__(kB) void k_ru_apply16(int V, const double2* __restrict__ r64, __half2* __restrict__ r,
-                                                   float* __restrict__ rn, __half2* __restrict__ p,
-                                                   float* __restrict__ pn, const CGState* st) {
-  if (!st->ru_apply) return;
-  const int i = blockIdx.x * kB + threadIdx.x;
-  if (i >= V) return;
-  const float be = (float)st->beta;
-  float2 rv[3], pp[3];
-  ld16(p, pn, V, i, pp);
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const double2 t = r64[(size_t)c * V + i];
-    rv[c] = make_float2((float)t.x, (float)t.y);
-    pp[c].x = rv[c].x + be * pp[c].x;
-    pp[c].y = rv[c].y + be * pp[c].y;
-  }
-  st16(r, rn, V, i, rv);
-  st16(p, pn, V, i, pp);
-}
-
-void ru_acc(hipStream_t s, double2* xh, float2* xl, size_t n, CGState* st) {
-  hipLaunchKernelGGL(k_ru_acc, dim3(nblk(n)), dim3(kB), 0, s, n, xh, xl, st);
-}
 void ru_sub(hipStream_t s, const Geom& g, double2* r, const double2* phi, const double2* ax, CGState* st,
             double* partials) {
   const unsigned nb = nblk(g.V);
   hipLaunchKernelGGL(k_ru_sub, dim3(nb), dim3(kB), 0, s, g.V, r, phi, ax, st, partials, (int)nb);
-}
-void ru_apply32(hipStream_t s, const Geom& g, const double2* r64, float2* r, float2* p, const CGState* st) {
-  hipLaunchKernelGGL(k_ru_apply32, dim3(nblk(g.V)), dim3(kB), 0, s, g.V, r64, r, p, st);
-}
-void ru_apply16(hipStream_t s, const Geom& g, const double2* r64, __half2* r, float* rn, __half2* p, float* pn,
-                const CGState* st) {
-  hipLaunchKernelGGL(k_ru_apply16, dim3(nblk(g.V)), dim3(kB), 0, s, g.V, r64, r, rn, p, pn, st);
 }
 
 }  // namespace sfh

@@ -1,0 +1,316 @@
+// Two-kernel iteration of the mixed-precision CG on (D D^dag) x = phi
+// (Fermions.cpp:553-617 in FP32 or FP16 storage, with FP64 reliable updates).
+//
+// The textbook iteration needs four passes (two hops, x/r update, p update)
+// because alpha and beta are global reductions.  Here each iteration is two
+// kernels:
+//   A_k: r_k = r_{k-1} - alpha_{k-1} v_{k-1},  p_k = r_k + beta_{k-1} p_{k-1},
+//        x += alpha_{k-1} p_{k-1} at the own site; r_k and p_k recomputed at
+//        the 8 neighbours from their stored r, v, p; then t = D^dag p_k.
+//   B_k: v_k = D t and, in one grid reduction, <p_k, v_k>, <r_k, v_k>,
+//        <v_k, v_k>, <r_k, r_k>; the finisher forms alpha_k = |r_k|^2 /
+//        <p,v> and |r_{k+1}|^2 = |r_k|^2 - 2 alpha <r,v> + alpha^2 <v,v>.
+// A neighbour recomputes exactly what the owning site stores (same inputs,
+// same fused multiply-adds, same FP16 rounding), so every site sees one
+// consistent p_k.  Kernel launches per iteration drop from four to two and
+// the r, p and v vectors are read once per iteration instead of twice.
+#include "mixed.h"
+#include "reduce.h"
+
+namespace sfh {
+
+namespace {
+
+constexpr int kBlk = 256;
+
+// FP16 storage of one site: halves scaled by the site's max-norm.  v is
+// replaced by the value that a later load of the stored site returns.
+struct Q16 {
+  __half2 h[3];
+  float m;
+};
+__device__ __forceinline__ void quant16(vec3<float>& v, Q16& q) {
+  float m = 0.f;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) m = fmaxf(m, fmaxf(fabsf(v.c[c].re), fabsf(v.c[c].im)));
+  const float inv = m > 0.f ? 1.0f / m : 0.f;
+  q.m = m;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    q.h[c] = __floats2half2_rn(v.c[c].re * inv, v.c[c].im * inv);
+    const float2 t = __half22float2(q.h[c]);
+    v.c[c] = {m * t.x, m * t.y};
+  }
+}
+
+template <class T> constexpr bool is_half() { return std::is_same<T, Half>::value; }
+
+// r <- r - al v ; p <- r + be p, each rounded to storage as the owner stores it.
+template <class T>
+__device__ __forceinline__ void advance(vec3<float>& r, const vec3<float>& v, vec3<float>& p, float al, float be,
+                                        Q16& qr, Q16& qp) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    r.c[c].re = __fmaf_rn(-al, v.c[c].re, r.c[c].re);
+    r.c[c].im = __fmaf_rn(-al, v.c[c].im, r.c[c].im);
+  }
+  if constexpr (is_half<T>()) quant16(r, qr);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    p.c[c].re = __fmaf_rn(be, p.c[c].re, r.c[c].re);
+    p.c[c].im = __fmaf_rn(be, p.c[c].im, r.c[c].im);
+  }
+  if constexpr (is_half<T>()) quant16(p, qp);
+}
+
+template <class T>
+__device__ __forceinline__ void store_q(typename Prec<T>::S* f, float* nrm, int V, int i, const vec3<float>& v,
+                                        const Q16& q) {
+  if constexpr (is_half<T>()) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f[(size_t)c * V + i] = q.h[c];
+    nrm[i] = q.m;
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f[(size_t)c * V + i] = make_float2(v.c[c].re, v.c[c].im);
+  }
+}
+
+// Site of this lane: each workgroup takes one checkerboard range on both
+// parities (as the single-system hop kernel), workgroups remapped per XCD.
+struct Site {
+  int lb, cb, p, i, q;
+};
+__device__ __forceinline__ Site site_of(const Geom& g, int nb) {
+  Site s;
+  s.lb = xcd_block(nb);
+  const int hw = blockDim.x >> 1;
+  s.p = threadIdx.x >= hw;
+  s.cb = s.lb * hw + (threadIdx.x - s.p * hw);
+  s.i = s.p * g.Vh + s.cb;
+  s.q = (1 - s.p) * g.Vh;
+  return s;
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlk) void k_mix_a(Geom g, const typename Prec<T>::L* __restrict__ U, MixBufs<T> b,
+                                                float mass, CGState* st, int nb) {
+  if (st->done) return;
+  const Site s = site_of(g, nb);
+  if (s.cb >= g.Vh) return;
+  const int cur = st->iter & 1, prv = cur ^ 1;
+  const typename Prec<T>::S* __restrict__ rp = b.r[prv];
+  const typename Prec<T>::S* __restrict__ pp = b.p[prv];
+  const float* __restrict__ rpn = b.rn[prv];
+  const float* __restrict__ ppn = b.pn[prv];
+  const float al = (float)st->alpha, be = (float)st->beta;
+  const int V = g.V;
+  int n[4];
+  cb_coords(g, s.cb, s.p, n);
+  Q16 qr, qp;
+  // own site: advance, store r_k and p_k, accumulate x
+  vec3<float> ro, vo, po;
+  load_spinor<T>(rp, rpn, V, s.i, ro);
+  load_spinor<T>(b.v, b.vn, V, s.i, vo);
+  load_spinor<T>(pp, ppn, V, s.i, po);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const size_t j = (size_t)c * V + s.i;
+    float2 xx = b.x[j];
+    xx.x = __fmaf_rn(al, po.c[c].re, xx.x);
+    xx.y = __fmaf_rn(al, po.c[c].im, xx.y);
+    b.x[j] = xx;
+  }
+  advance<T>(ro, vo, po, al, be, qr, qp);
+  store_q<T>(b.r[cur], b.rn[cur], V, s.i, ro, qr);
+  store_q<T>(b.p[cur], b.pn[cur], V, s.i, po, qp);
+  // t = m p_k - (1/2) sum_mu eta [bup U p_k(n+mu) - bdn U^dag p_k(n-mu)]
+  vec3<float> acc;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) acc.c[c] = {0.f, 0.f};
+#pragma unroll
+  for (int mu = 0; mu < 4; ++mu) {
+    int fw, bw;
+    float sf, sb;
+    neighbours<float>(g, s.cb, n, mu, fw, bw, sf, sb);
+    mat3<float> Uf, Ub;
+    vec3<float> rf, vf, pf, rb, vb, pb;
+    load_link<T>(U, V, mu, s.i, Uf);
+    load_spinor<T>(rp, rpn, V, s.q + fw, rf);
+    load_spinor<T>(b.v, b.vn, V, s.q + fw, vf);
+    load_spinor<T>(pp, ppn, V, s.q + fw, pf);
+    load_link<T>(U, V, mu, s.q + bw, Ub);
+    load_spinor<T>(rp, rpn, V, s.q + bw, rb);
+    load_spinor<T>(b.v, b.vn, V, s.q + bw, vb);
+    load_spinor<T>(pp, ppn, V, s.q + bw, pb);
+    Q16 t0, t1;
+    advance<T>(rf, vf, pf, al, be, t0, t1);
+    advance<T>(rb, vb, pb, al, be, t0, t1);
+    acc_mv(acc, Uf, pf, sf);
+    acc_mdv(acc, Ub, pb, -sb);
+  }
+  vec3<float> res;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    res.c[c].re = mass * po.c[c].re - 0.5f * acc.c[c].re;
+    res.c[c].im = mass * po.c[c].im - 0.5f * acc.c[c].im;
+  }
+  store_spinor<T>(b.t, b.tn, V, s.i, res);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlk) void k_mix_b(Geom g, const typename Prec<T>::L* __restrict__ U, MixBufs<T> b,
+                                                float mass, CGState* st, double* partials, int nb) {
+  if (st->done) return;
+  const Site s = site_of(g, nb);
+  double d[4] = {0.0, 0.0, 0.0, 0.0};
+  if (s.cb < g.Vh) {
+    const int cur = st->iter & 1;
+    const int V = g.V;
+    int n[4];
+    cb_coords(g, s.cb, s.p, n);
+    vec3<float> acc;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc.c[c] = {0.f, 0.f};
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) {
+      int fw, bw;
+      float sf, sb;
+      neighbours<float>(g, s.cb, n, mu, fw, bw, sf, sb);
+      mat3<float> Uf, Ub;
+      vec3<float> tf, tb;
+      load_link<T>(U, V, mu, s.i, Uf);
+      load_spinor<T>(b.t, b.tn, V, s.q + fw, tf);
+      load_link<T>(U, V, mu, s.q + bw, Ub);
+      load_spinor<T>(b.t, b.tn, V, s.q + bw, tb);
+      acc_mv(acc, Uf, tf, sf);
+      acc_mdv(acc, Ub, tb, -sb);
+    }
+    vec3<float> self, v, pk, rk;
+    load_spinor<T>(b.t, b.tn, V, s.i, self);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      v.c[c].re = mass * self.c[c].re + 0.5f * acc.c[c].re;
+      v.c[c].im = mass * self.c[c].im + 0.5f * acc.c[c].im;
+    }
+    Q16 qv;
+    if constexpr (is_half<T>()) quant16(v, qv);
+    store_q<T>(b.v, b.vn, V, s.i, v, qv);
+    load_spinor<T>(b.p[cur], b.pn[cur], V, s.i, pk);
+    load_spinor<T>(b.r[cur], b.rn[cur], V, s.i, rk);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double vr = v.c[c].re, vi = v.c[c].im, pr = pk.c[c].re, pi = pk.c[c].im, rr = rk.c[c].re,
+                   ri = rk.c[c].im;
+      d[0] += pr * vr + pi * vi;
+      d[1] += rr * vr + ri * vi;
+      d[2] += vr * vr + vi * vi;
+      d[3] += rr * rr + ri * ri;
+    }
+  }
+  grid_reduce_n<4>(d, partials, ctr_of(st->ctr, CNT_PV), nb, s.lb, FinFused{st});
+}
+
+// Split form, own-site part of kernel A: r_k, p_k and x updated in place
+// (r and p single-buffered: both parity slots point to one array).  The hop
+// t = D^dag p_k then runs as a separate launch.
+template <class T>
+__global__ __launch_bounds__(kBlk) void k_mix_upd(int V, MixBufs<T> b, const CGState* st) {
+  if (st->done) return;
+  const int i = blockIdx.x * kBlk + threadIdx.x;
+  if (i >= V) return;
+  const int cur = st->iter & 1, prv = cur ^ 1;
+  const float al = (float)st->alpha, be = (float)st->beta;
+  vec3<float> ro, vo, po;
+  load_spinor<T>(b.r[prv], b.rn[prv], V, i, ro);
+  load_spinor<T>(b.v, b.vn, V, i, vo);
+  load_spinor<T>(b.p[prv], b.pn[prv], V, i, po);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const size_t j = (size_t)c * V + i;
+    float2 xx = b.x[j];
+    xx.x = __fmaf_rn(al, po.c[c].re, xx.x);
+    xx.y = __fmaf_rn(al, po.c[c].im, xx.y);
+    b.x[j] = xx;
+  }
+  Q16 qr, qp;
+  advance<T>(ro, vo, po, al, be, qr, qp);
+  store_q<T>(b.r[cur], b.rn[cur], V, i, ro, qr);
+  store_q<T>(b.p[cur], b.pn[cur], V, i, po, qp);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlk) void k_mix_acc(int V, double2* __restrict__ xh, MixBufs<T> b, CGState* st) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->ru_apply = 0;  // read only by the apply kernel of this chunk
+  if (!st->ru_pending) return;
+  const int i = blockIdx.x * kBlk + threadIdx.x;
+  if (i >= V) return;
+  const int k = (st->iter & 1) ^ 1;  // buffers of p_k, the direction of the last iteration
+  const double al = st->alpha;
+  vec3<float> pk;
+  load_spinor<T>(b.p[k], b.pn[k], V, i, pk);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const size_t j = (size_t)c * V + i;
+    const float2 xl = b.x[j];
+    double2 h = xh[j];
+    h.x += (double)xl.x + al * (double)pk.c[c].re;
+    h.y += (double)xl.y + al * (double)pk.c[c].im;
+    xh[j] = h;
+    b.x[j] = make_float2(0.f, 0.f);
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlk) void k_mix_apply(int V, const double2* __restrict__ r64, MixBufs<T> b,
+                                                    const CGState* st) {
+  if (!st->ru_apply) return;
+  const int i = blockIdx.x * kBlk + threadIdx.x;
+  if (i >= V) return;
+  const int k = (st->iter & 1) ^ 1;  // read as r_{k-1} by the next kernel A
+  vec3<float> r;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double2 t = r64[(size_t)c * V + i];
+    r.c[c] = {(float)t.x, (float)t.y};
+  }
+  store_spinor<T>(b.r[k], b.rn[k], V, i, r);
+}
+
+inline int mix_blocks(const Geom& g) { return (g.Vh + kBlk / 2 - 1) / (kBlk / 2); }
+
+}  // namespace
+
+template <class T>
+void mix_a(hipStream_t s, const Geom& g, const typename Prec<T>::L* U, const MixBufs<T>& b, float mass, CGState* st) {
+  const int nb = mix_blocks(g);
+  hipLaunchKernelGGL(k_mix_a<T>, dim3(nb), dim3(kBlk), 0, s, g, U, b, mass, st, nb);
+}
+template <class T>
+void mix_b(hipStream_t s, const Geom& g, const typename Prec<T>::L* U, const MixBufs<T>& b, float mass, CGState* st,
+           double* partials) {
+  const int nb = mix_blocks(g);
+  hipLaunchKernelGGL(k_mix_b<T>, dim3(nb), dim3(kBlk), 0, s, g, U, b, mass, st, partials, nb);
+}
+template <class T> void mix_upd(hipStream_t s, const Geom& g, const MixBufs<T>& b, CGState* st) {
+  hipLaunchKernelGGL(k_mix_upd<T>, dim3((g.V + kBlk - 1) / kBlk), dim3(kBlk), 0, s, g.V, b, st);
+}
+template <class T> void mix_acc(hipStream_t s, const Geom& g, double2* x64, const MixBufs<T>& b, CGState* st) {
+  hipLaunchKernelGGL(k_mix_acc<T>, dim3((g.V + kBlk - 1) / kBlk), dim3(kBlk), 0, s, g.V, x64, b, st);
+}
+template <class T> void mix_apply(hipStream_t s, const Geom& g, const double2* r64, const MixBufs<T>& b, CGState* st) {
+  hipLaunchKernelGGL(k_mix_apply<T>, dim3((g.V + kBlk - 1) / kBlk), dim3(kBlk), 0, s, g.V, r64, b, st);
+}
+
+template void mix_a<float>(hipStream_t, const Geom&, const float2*, const MixBufs<float>&, float, CGState*);
+template void mix_a<Half>(hipStream_t, const Geom&, const __half2*, const MixBufs<Half>&, float, CGState*);
+template void mix_b<float>(hipStream_t, const Geom&, const float2*, const MixBufs<float>&, float, CGState*, double*);
+template void mix_b<Half>(hipStream_t, const Geom&, const __half2*, const MixBufs<Half>&, float, CGState*, double*);
+template void mix_upd<float>(hipStream_t, const Geom&, const MixBufs<float>&, CGState*);
+template void mix_upd<Half>(hipStream_t, const Geom&, const MixBufs<Half>&, CGState*);
+template void mix_acc<float>(hipStream_t, const Geom&, double2*, const MixBufs<float>&, CGState*);
+template void mix_acc<Half>(hipStream_t, const Geom&, double2*, const MixBufs<Half>&, CGState*);
+template void mix_apply<float>(hipStream_t, const Geom&, const double2*, const MixBufs<float>&, CGState*);
+template void mix_apply<Half>(hipStream_t, const Geom&, const double2*, const MixBufs<Half>&, CGState*);
+
+}  // namespace sfh

@@ -262,7 +262,7 @@ __global__ __launch_bounds__(128) void k_link_sum(Geom g, const double2* __restr
 
 static double link_sum(sfh_ctx* c, const Geom& g, const double2* U, double beta, double c11, double c12, int kind) {
   int nb = (4 * g.V + 127) / 128;
-  if (nb > kPartialSlot) nb = kPartialSlot;
+  if (nb > kPartialSlot - kRedGroups) nb = kPartialSlot - kRedGroups;
   KTimer kt(c, "force");
   hipLaunchKernelGGL(k_link_sum, dim3(nb), dim3(128), 0, c->stream, g, U, beta, c11, c12, kind,
                      c->d_partials + 2 * kPartialSlot, c->aux_ctr(), &c->d_state[1].pv, nb);

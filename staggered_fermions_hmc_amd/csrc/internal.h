@@ -93,9 +93,11 @@ struct sfh_ctx {
 
 namespace sfh {
 
-// Max workgroups per reduction slot: the four-lane hop on the largest
-// allowed lattice (2^24 sites) launches 2^24 / 64 = 2^18 workgroups.
-constexpr int kPartialSlot = 1 << 18;
+// Doubles per reduction slot: one partial per workgroup and value plus one
+// per arrival group and value.  The four-lane hop on the largest allowed
+// lattice (2^24 sites) reduces one value over 2^24 / 64 = 2^18 workgroups;
+// the two-kernel mixed iteration reduces four over 2^24 / 256 = 2^16.
+constexpr int kPartialSlot = (1 << 18) + 4 * kRedGroups;
 
 // ------------------------------------------------------------------ helpers
 int set_err(sfh_ctx* c, int code, const std::string& msg);
@@ -147,13 +149,7 @@ void dot_f64(hipStream_t s, const double* a, const double* b, size_t n, double* 
 void cg_xr_f64(hipStream_t s, const Geom& g, double2* x, double2* r, const double2* p, const double2* v,
                CGState* st, double* partials);
 void cg_p_f64(hipStream_t s, const Geom& g, double2* p, const double2* r, CGState* st);
-void cg_xr_f32(hipStream_t s, const Geom& g, float2* x, float2* r, const float2* p, const float2* v,
-               CGState* st, double* partials);
 void cg_p_f32(hipStream_t s, const Geom& g, float2* p, const float2* r, CGState* st);
-void cg_xr_f16(hipStream_t s, const Geom& g, float2* x, __half2* r, float* rn, const __half2* p, const float* pn,
-               const __half2* v, const float* vn, CGState* st, double* partials);
-void cg_p_f16(hipStream_t s, const Geom& g, __half2* p, float* pn, const __half2* r, const float* rn,
-              CGState* st);
 // mixed-precision support
 void f64_to_f32(hipStream_t s, const double2* in, float2* out, size_t n);
 void f64_to_f16(hipStream_t s, const Geom& g, const double2* in, __half2* out, float* norm);
@@ -162,12 +158,8 @@ void xpay_f64(hipStream_t s, double2* y, const double2* x, double a, size_t n); 
 void axpy_f64(hipStream_t s, double2* y, const double2* x, double a, size_t n);  // y += a*x
 void sub_f64(hipStream_t s, double2* out, const double2* a, const double2* b, size_t n);  // out = a - b
 // Device-side reliable update, gated on the solve state (blas.hip).
-void ru_acc(hipStream_t s, double2* xh, float2* xl, size_t n, CGState* st);
 void ru_sub(hipStream_t s, const Geom& g, double2* r, const double2* phi, const double2* ax, CGState* st,
             double* partials);
-void ru_apply32(hipStream_t s, const Geom& g, const double2* r64, float2* r, float2* p, const CGState* st);
-void ru_apply16(hipStream_t s, const Geom& g, const double2* r64, __half2* r, float* rn, __half2* p, float* pn,
-                const CGState* st);
 
 // ------------------------------------------------------------------ gauge (gauge.hip)
 int force_field(sfh_ctx* c, const Geom& g, const double2* U, const double2* X, const double2* Y, double beta,
@@ -183,6 +175,11 @@ int polyakov_grid(sfh_ctx* c, const Geom& g, const double2* U, double* out);
 int cg_solve(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* p, const double2* phi, double2* x,
              sfh_cg_stats* stats);
 int apply_ddag_dev(sfh_ctx* c, const sfh_gauge* gauge, double mass, const double2* in, double2* out);
+// Average microseconds of the two steps of one mixed-precision iteration.
+int time_mixed(sfh_ctx* c, sfh_gauge* gauge, int precision, int n, double* us_a, double* us_b);
+// Mixed iteration form per precision: 0 fused, 1 split, -1 by lattice size.
+void set_mix_form(int precision, int form);
+int get_mix_form(int precision, const Geom& g);
 // Maps a stopped solve's device status to the reference's error text.
 int map_status(sfh_ctx* c, const CGState& s, int max_iter);
 // FP64 dot of two device vectors of ncomplex complex numbers (synchronises).

@@ -1,0 +1,40 @@
+// Two-kernel mixed-precision CG iteration (cgmixed.hip).
+#pragma once
+
+#include "hopcore.h"
+#include "internal.h"
+
+namespace sfh {
+
+// Sloppy vectors of one mixed-precision solve.  In the fused form r and p are
+// double-buffered: iteration k writes r_k and p_k into buffer k & 1 while
+// other workgroups still read r_{k-1} and p_{k-1} from buffer (k - 1) & 1.
+// In the split form both slots point to one array.  Norms are the per-site
+// max-norms of FP16 storage (unused in FP32).
+template <class T> struct MixBufs {
+  typename Prec<T>::S *r[2], *p[2], *v, *t;
+  float *rn[2], *pn[2], *vn, *tn;
+  float2* x;  // FP32 solution accumulated since the last reliable update
+};
+
+// Kernel A of iteration k: r_k = r_{k-1} - alpha v_{k-1}, p_k = r_k +
+// beta p_{k-1} and x += alpha p_{k-1} at each site, with r_k and p_k
+// recomputed at the neighbours, then t = D^dag p_k.
+template <class T>
+void mix_a(hipStream_t s, const Geom& g, const typename Prec<T>::L* U, const MixBufs<T>& b, float mass, CGState* st);
+// Split form of kernel A (for lattices whose neighbour vectors do not stay
+// in cache): the own-site update alone, in place; the caller then runs the
+// plain hop t = D^dag p_k.
+template <class T> void mix_upd(hipStream_t s, const Geom& g, const MixBufs<T>& b, CGState* st);
+// Fused form chosen below this many sites; the split form at and above.
+constexpr int kMixFusedMaxV = 1 << 18;
+// Kernel B of iteration k: v = D t and the four dots of FinFused.
+template <class T>
+void mix_b(hipStream_t s, const Geom& g, const typename Prec<T>::L* U, const MixBufs<T>& b, float mass, CGState* st,
+           double* partials);
+// Reliable update, first and last steps (gated on the solve state):
+// x64 += x + alpha p_k and x = 0; then r_{k} = r64 in sloppy storage.
+template <class T> void mix_acc(hipStream_t s, const Geom& g, double2* x64, const MixBufs<T>& b, CGState* st);
+template <class T> void mix_apply(hipStream_t s, const Geom& g, const double2* r64, const MixBufs<T>& b, CGState* st);
+
+}  // namespace sfh

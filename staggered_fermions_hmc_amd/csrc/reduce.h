@@ -12,50 +12,122 @@ namespace sfh {
 
 //
 // Coherence across the 8 XCDs (each with its own L2) without cache-wide
-// fences: the partial is written with an agent-scope atomic store (coherent
-// at device scope), the wave waits for that store to complete, then counts
-// its arrival with an agent-scope atomic add; the last workgroup reads the
-// partials with agent-scope atomic loads.  No L2 write-back or invalidate is
-// issued, so the kernel's other traffic keeps its cache reuse.  What the
-// finisher writes is read by later kernels, after the kernel boundary.
+// fences: partial sums are written with agent-scope atomic stores (coherent
+// at device scope), the wave waits for each store to complete, then counts
+// its arrival with an agent-scope atomic add; partials are read back with
+// agent-scope atomic loads.  No L2 write-back or invalidate is issued, so the
+// kernel's other traffic keeps its cache reuse.  What the finisher writes is
+// read by later kernels, after the kernel boundary.
 //
-// Arrivals are counted in two levels (kRedGroups group counters on separate
-// cache lines, then one top counter): same-address atomics serialise in the
-// memory system, so one counter for thousands of workgroups would add
-// microseconds to every reduction.
+// Arrivals are counted in two levels: workgroup `slot` belongs to group
+// slot % ng (ng = min(nblocks, kRedGroups)), whose counter sits on a cache
+// line of its own; the last arrival of each group counts on one top counter.
+// Same-address atomics serialise in the memory system, so one counter for
+// thousands of workgroups would add microseconds to every reduction.  Up to
+// kDirectSum workgroups, the last one sums all partials itself (in slot
+// order); above that, the last arrival of each group first sums its group's
+// partials (in slot order) into a group partial, and the last group sums the
+// ng group partials (in group order), so no workgroup reads more than a few
+// partials per lane.
+//
+// N values are reduced together; value j of workgroup b is kept at
+// partials[j * (nblocks + kRedGroups) + b], so partials must hold
+// N * (nblocks + kRedGroups) doubles.  fin receives the N sums.
+constexpr int kDirectSum = 1024;
+
+template <int N>
+__device__ __forceinline__ void block_sum_n(double (&v)[N]) {
+  __shared__ double ws[16][N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) v[j] = wave_sum(v[j]);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int j = 0; j < N; ++j) ws[w][j] = v[j];
+  }
+  __syncthreads();
+  const int nw = blockDim.x >> 6;
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    double t = 0.0;
+    for (int k = 0; k < nw; ++k) t += ws[k][j];
+    v[j] = t;
+  }
+  __syncthreads();
+}
+
+template <int N, class Fin>
+__device__ __forceinline__ void grid_reduce_n(double (&v)[N], double* partials, unsigned* ctr, int nblocks, int slot,
+                                              Fin fin) {
+  block_sum_n<N>(v);
+  const int ng = nblocks < kRedGroups ? nblocks : kRedGroups;
+  const int grp = slot % ng;
+  const size_t stride = (size_t)nblocks + kRedGroups;
+  const bool two = nblocks > kDirectSum;
+  __shared__ int group_last, grid_last;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+      __hip_atomic_store(&partials[j * stride + slot], v[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_waitcnt(0);  // the partials' stores have completed before the arrival is counted
+    const unsigned gsize = unsigned(nblocks / ng + (grp < nblocks % ng ? 1 : 0));
+    const unsigned t = __hip_atomic_fetch_add(&ctr[grp * kCtrStride], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    group_last = (t == gsize - 1);
+  }
+  __syncthreads();
+  if (!group_last) return;
+  if (two) {
+    double s[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) s[j] = 0.0;
+    for (int k = grp + ng * (int)threadIdx.x; k < nblocks; k += ng * (int)blockDim.x) {
+#pragma unroll
+      for (int j = 0; j < N; ++j)
+        s[j] += __hip_atomic_load(&partials[j * stride + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    block_sum_n<N>(s);
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int j = 0; j < N; ++j)
+        __hip_atomic_store(&partials[j * stride + nblocks + grp], s[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+  if (threadIdx.x == 0) {
+    __hip_atomic_store(&ctr[grp * kCtrStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_waitcnt(0);
+    const unsigned u =
+        __hip_atomic_fetch_add(&ctr[kRedGroups * kCtrStride], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    grid_last = (u == unsigned(ng - 1));
+  }
+  __syncthreads();
+  if (!grid_last) return;
+  double s[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) s[j] = 0.0;
+  const int base = two ? nblocks : 0, cnt = two ? ng : nblocks;
+  for (int k = threadIdx.x; k < cnt; k += blockDim.x) {
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+      s[j] += __hip_atomic_load(&partials[j * stride + base + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  block_sum_n<N>(s);
+  if (threadIdx.x == 0) {
+    fin(s);
+    __hip_atomic_store(&ctr[kRedGroups * kCtrStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+template <class Fin> struct FinOne {
+  Fin f;
+  __device__ void operator()(const double* s) const { f(s[0]); }
+};
+
+// One value: partials must hold nblocks + kRedGroups doubles.
 template <class Fin>
 __device__ __forceinline__ void grid_reduce(double v, double* partials, unsigned* ctr, int nblocks, int slot,
                                             Fin fin) {
-  const double bs = block_sum(v);
-  const int ng = nblocks < kRedGroups ? nblocks : kRedGroups;
-  __shared__ int is_last;
-  if (threadIdx.x == 0) {
-    __hip_atomic_store(&partials[slot], bs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __builtin_amdgcn_s_waitcnt(0);  // the partial's store has completed before the arrival is counted
-    const int grp = slot % ng;
-    const unsigned gsize = unsigned(nblocks / ng + (grp < nblocks % ng ? 1 : 0));
-    int last = 0;
-    const unsigned t = __hip_atomic_fetch_add(&ctr[grp * kCtrStride], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (t == gsize - 1) {
-      const unsigned u =
-          __hip_atomic_fetch_add(&ctr[kRedGroups * kCtrStride], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      last = (u == unsigned(ng - 1));
-    }
-    is_last = last;
-  }
-  __syncthreads();
-  if (is_last) {
-    double s = 0.0;
-    for (int k = threadIdx.x; k < nblocks; k += blockDim.x)
-      s += __hip_atomic_load(&partials[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    s = block_sum(s);
-    for (int k = threadIdx.x; k < ng; k += blockDim.x)
-      __hip_atomic_store(&ctr[k * kCtrStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (threadIdx.x == 0) {
-      fin(s);
-      __hip_atomic_store(&ctr[kRedGroups * kCtrStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
+  double a[1] = {v};
+  grid_reduce_n<1>(a, partials, ctr, nblocks, slot, FinOne<Fin>{fin});
 }
 
 // Finisher of <p, DDdag p>: alpha = rsq / pv with the reference's breakdown
@@ -83,41 +155,76 @@ struct FinPV {
 // the iteration budget (Fermions.cpp:568, :594-600, :609-614).  In mixed
 // precision it also pauses the solve when a reliable update is due or the
 // iterated residual meets the target, for the host to run the FP64 update.
+// Iteration bookkeeping once |r_{k+1}|^2 = rr is known: beta, the
+// convergence test, and in mixed precision the pause for a reliable update.
+__device__ __forceinline__ void finish_rr(CGState* st, double rr) {
+  const double b = rr / st->rsq;
+  st->iter += 1;
+  st->rsq_prev = st->rsq;
+  st->rsq = rr;
+  if (!isfinite(b)) {
+    st->status = SFH_E_NONFINITE_BETA;
+    st->done = 1;
+    return;
+  }
+  st->beta = b;
+  if (st->mixed) {
+    // pause for an FP64 reliable update: it decides convergence and the
+    // budget on the true residual
+    const double r = sqrt(rr);
+    int reason = 0;
+    if (rr < st->target) reason = 2;
+    else if (r < st->delta * st->rmax) reason = 1;
+    else if (st->iter >= st->max_iter) reason = 3;
+    if (reason) {
+      st->pause_reason = reason;
+      st->ru_pending = 1;
+      st->done = 1;
+    }
+    if (r > st->rmax) st->rmax = r;
+    return;
+  }
+  if (rr < st->target) {
+    st->done = 1;
+  } else if (st->iter >= st->max_iter) {
+    st->status = SFH_E_NOCONV;
+    st->done = 1;
+  }
+}
+
+// Finisher of |r_{k+1}|^2 (Fermions.cpp:589-614).
 struct FinRR {
   CGState* st;
-  __device__ void operator()(double rr) const {
-    const double b = rr / st->rsq;
-    st->iter += 1;
-    st->rsq_prev = st->rsq;
+  __device__ void operator()(double rr) const { finish_rr(st, rr); }
+};
+
+// Finisher of the two-kernel mixed-precision iteration: s = {<p,v>, <r,v>,
+// <v,v>, <r,r>} of the stored p_k, r_k and v = DDdag p_k.  alpha = |r_k|^2 /
+// <p,v> (breakdown checks of Fermions.cpp:573-586) and |r_{k+1}|^2 =
+// |r_k - alpha v|^2 expanded, from this iteration's dots only.
+struct FinFused {
+  CGState* st;
+  __device__ void operator()(const double* s) const {
+    const double pv = s[0], rv = s[1], vv = s[2], rr = s[3];
+    st->pv = pv;
     st->rsq = rr;
-    if (!isfinite(b)) {
-      st->status = SFH_E_NONFINITE_BETA;
+    if (!isfinite(pv) || fabs(pv) < 1e-30) {
+      st->status = SFH_E_BREAKDOWN;
       st->done = 1;
       return;
     }
-    st->beta = b;
-    if (st->mixed) {
-      // pause for an FP64 reliable update: it decides convergence and the
-      // budget on the true residual
-      const double r = sqrt(rr);
-      int reason = 0;
-      if (rr < st->target) reason = 2;
-      else if (r < st->delta * st->rmax) reason = 1;
-      else if (st->iter >= st->max_iter) reason = 3;
-      if (reason) {
-        st->pause_reason = reason;
-        st->ru_pending = 1;
-        st->done = 1;
-      }
-      if (r > st->rmax) st->rmax = r;
+    const double a = rr / pv;
+    if (!isfinite(a)) {
+      st->status = SFH_E_NONFINITE_ALPHA;
+      st->done = 1;
       return;
     }
-    if (rr < st->target) {
-      st->done = 1;
-    } else if (st->iter >= st->max_iter) {
-      st->status = SFH_E_NOCONV;
-      st->done = 1;
-    }
+    st->alpha = a;
+    double rn = rr - 2.0 * a * rv + a * a * vv;
+    // cancellation can only take it below zero when r_{k+1} is tiny: the
+    // reliable update that this triggers measures the true residual
+    if (!(rn > 0.0)) rn = 0.0;
+    finish_rr(st, rn);
   }
 };
 
@@ -147,6 +254,7 @@ struct FinRU {
       return;
     }
     st->beta = rr / st->rsq_prev;
+    st->alpha = 0.0;  // the two-kernel iteration then restarts from r_true
     st->rsq = rr;
     st->rmax = sqrt(rr);
     st->ru_apply = 1;

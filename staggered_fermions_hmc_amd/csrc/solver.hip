@@ -12,6 +12,7 @@
 #include <string>
 
 #include "internal.h"
+#include "mixed.h"
 
 namespace sfh {
 
@@ -65,7 +66,11 @@ int ensure_lp(sfh_ctx* c, sfh_gauge* gauge, int precision) {
 
 namespace {
 
+// Iterations per chunk: FP64 solves have no pauses; in mixed precision a
+// reliable update stops the iterations until the chunk's end, so shorter
+// chunks waste fewer early-exit iterations per update.
 constexpr int kChunk = 16;
+constexpr int kChunkMixed = 8;
 
 struct Bufs64 {
   double2 *r, *p, *t, *v;
@@ -82,9 +87,9 @@ int upload_state(sfh_ctx* c, const CGState& s) {
   return hip_check(c, hipStreamSynchronize(c->stream), "state upload");
 }
 
-// A chunk (kChunk iterations and, in mixed precision, the gated reliable
-// update) captured once into a HIP graph and replayed: one launch per chunk
-// instead of 4 kChunk + 5.  Graphs are cached per context by the exact
+// A chunk (a fixed number of iterations and, in mixed precision, the gated
+// reliable update) captured once into a HIP graph and replayed: one launch
+// per chunk instead of one per kernel.  Graphs are cached per context by the exact
 // buffers, lattice and parameters they were captured with.  With kernel
 // profiling on, chunks are launched directly so every kernel can be timed.
 template <class F> int launch_chunk(sfh_ctx* c, const std::string& key, F direct) {
@@ -130,46 +135,120 @@ std::string graph_key(const char* kind, const Geom& g, std::initializer_list<con
   return k;
 }
 
-// Runs chunks until the device state says stop; issue_chunk launches one.
-// The state is read back one chunk behind, so the GPU never waits for the
-// host.  Returns the final state through out.
-template <class F> int run_chunks(sfh_ctx* c, F issue_chunk, CGState& out) {
+// Runs chunks until the device state says stop; issue_chunk launches one of
+// `per_chunk` iterations.  Up to two chunks are in flight and the state is
+// read back one chunk behind, so the GPU does not wait for the host.  When
+// the residual history predicts that the chunk in flight ends the solve, the
+// next chunk is held back until that chunk's state is read: a chunk issued
+// after convergence would only run early-exit kernels.  Returns the final
+// state through out.
+template <class F> int run_chunks(sfh_ctx* c, F issue_chunk, int per_chunk, const CGState& init, CGState& out) {
   CGState* ring[2] = {c->h_state + 1, c->h_state + 2};
   hipEvent_t ev[2];
   hipEventCreateWithFlags(&ev[0], hipEventDisableTiming);
   hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
-  int k = 0, inflight = 0, rc = SFH_OK;
-  auto issue = [&](int slot) {
+  long next = 0, head = 0;  // next chunk to issue, oldest chunk in flight
+  int rc = SFH_OK;
+  auto issue = [&]() {
+    const int slot = (int)(next & 1);
     if (int e = issue_chunk()) rc = e;
     hipMemcpyAsync(ring[slot], c->d_state, sizeof(CGState), hipMemcpyDeviceToHost, c->stream);
     hipEventRecord(ev[slot], c->stream);
+    ++next;
   };
-  issue(0);
-  inflight = 1;
+  const double lr0 = std::log(init.rsq > 0 ? init.rsq : 1.0);
+  bool hold = false;
+  long polls = 0;
   for (;;) {
-    const int cur = k & 1;
-    issue(cur ^ 1);
-    ++inflight;
+    while (!rc && (next == head || (!hold && next - head < 2))) issue();
     if (rc) break;
-    rc = hip_check(c, hipEventSynchronize(ev[cur]), "CG chunk");
-    --inflight;
+    rc = hip_check(c, hipEventSynchronize(ev[head & 1]), "CG chunk");
     if (!rc) rc = hip_check(c, hipGetLastError(), "CG launch");
     if (rc) break;
-    if (ring[cur]->done) break;
-    ++k;
+    const CGState s = *ring[head & 1];
+    ++head;
+    if (s.done) break;
     // every iteration advances the device counter or stops the solve, so the
     // budget is exceeded only if kernels are not running at all
-    if ((long)k * kChunk > (long)ring[cur]->max_iter + 4L * kChunk) {
+    if (++polls * per_chunk > (long)s.max_iter + 4L * per_chunk + 64) {
       rc = set_err(c, SFH_E_HIP, "CG state did not advance");
       break;
+    }
+    // iterations still needed at the mean convergence rate so far
+    hold = false;
+    if (s.iter > 0 && s.rsq > 0 && s.target > 0) {
+      const double rate = (lr0 - std::log(s.rsq)) / s.iter;
+      if (rate > 0) hold = std::log(s.rsq / s.target) / rate < per_chunk - 0.5;
     }
   }
   hipStreamSynchronize(c->stream);
   hipMemcpy(&out, c->d_state, sizeof(CGState), hipMemcpyDeviceToHost);
   hipEventDestroy(ev[0]);
   hipEventDestroy(ev[1]);
-  (void)inflight;
   return rc ? rc : hip_check(c, hipGetLastError(), "CG launches");
+}
+
+// Form of the mixed iteration per precision (1 f32, 2 f16): 0 fused kernel
+// A, 1 split (in-place update + plain hop), -1 by lattice size.
+int g_mix_form[3] = {-1, -1, -1};
+
+bool mix_fused(int prec, const Geom& g) {
+  const int f = g_mix_form[prec];
+  return f < 0 ? g.V < kMixFusedMaxV : f == 0;
+}
+
+// Sloppy vectors of a mixed-precision solve on lattice g.  Fused form:
+// kernel A recomputes r_k and p_k at the neighbours, so r and p are double
+// buffers; split form: in-place update, then the plain hop.
+template <class T> int mix_bufs(sfh_ctx* c, const Geom& g, bool fused, MixBufs<T>& b) {
+  using S = typename Prec<T>::S;
+  constexpr bool half = std::is_same<T, Half>::value;
+  const size_t n = 3 * (size_t)g.V;
+  const char* names[6] = {"mx_r0", "mx_p0", "mx_v", "mx_t", "mx_r1", "mx_p1"};
+  S* vecs[6] = {};
+  float* norms[6] = {};
+  for (int k = 0; k < (fused ? 6 : 4); ++k) {
+    vecs[k] = buf<S>(c, names[k], n);
+    if (!vecs[k]) return SFH_E_NOMEM;
+    if (half) {
+      norms[k] = buf<float>(c, (std::string(names[k]) + "n").c_str(), g.V);
+      if (!norms[k]) return SFH_E_NOMEM;
+    }
+  }
+  b.r[0] = vecs[0]; b.p[0] = vecs[1]; b.v = vecs[2]; b.t = vecs[3];
+  b.r[1] = fused ? vecs[4] : vecs[0];
+  b.p[1] = fused ? vecs[5] : vecs[1];
+  b.rn[0] = norms[0]; b.pn[0] = norms[1]; b.vn = norms[2]; b.tn = norms[3];
+  b.rn[1] = fused ? norms[4] : norms[0];
+  b.pn[1] = fused ? norms[5] : norms[1];
+  b.x = buf<float2>(c, "cgl_x", n);
+  return b.x ? SFH_OK : SFH_E_NOMEM;
+}
+
+// Step A of one mixed iteration: the fused kernel, or the in-place update
+// and the plain hop t = D^dag p_k.
+template <class T>
+void mix_step_a(sfh_ctx* c, const sfh_gauge* gauge, bool fused, const MixBufs<T>& b, float m, CGState* st) {
+  constexpr bool half = std::is_same<T, Half>::value;
+  const Geom& g = gauge->g;
+  if (fused) {
+    KTimer kt(c, half ? "mix_a_f16" : "mix_a_f32");
+    mix_a<T>(c->stream, g, reinterpret_cast<const typename Prec<T>::L*>(half ? (const void*)gauge->lp.u16
+                                                                             : (const void*)gauge->lp.u32),
+             b, m, st);
+    return;
+  }
+  {
+    KTimer kt(c, "mix_upd");
+    mix_upd<T>(c->stream, g, b, st);
+  }
+  KTimer kt(c, half ? "hop_f16" : "hop_f32");
+  if (half)
+    hop_f16(c->stream, g, gauge->lp.u16, reinterpret_cast<const __half2*>(b.p[0]), b.pn[0],
+            reinterpret_cast<__half2*>(b.t), b.tn, m, -1.f, nullptr, nullptr, st, nullptr);
+  else
+    hop_f32(c->stream, g, gauge->lp.u32, reinterpret_cast<const float2*>(b.p[0]), reinterpret_cast<float2*>(b.t), m,
+            -1.f, nullptr, st, nullptr);
 }
 
 }  // namespace
@@ -222,7 +301,7 @@ static int cg_f64(sfh_ctx* c, const sfh_gauge* gauge, const sfh_cg_params* prm, 
     };
     const std::string key = graph_key("f64", g, {gauge->U, x, b.r, b.p, b.t, b.v, st, part}, prm->mass,
                                       get_hop_variant(0, g));
-    if (int e = run_chunks(c, [&]() { return launch_chunk(c, key, direct); }, fin)) return e;
+    if (int e = run_chunks(c, [&]() { return launch_chunk(c, key, direct); }, kChunk, s, fin)) return e;
   }
   if (stats) {
     stats->iterations = fin.iter;
@@ -232,42 +311,39 @@ static int cg_f64(sfh_ctx* c, const sfh_gauge* gauge, const sfh_cg_params* prm, 
   return map_status(c, fin, prm->max_iter);
 }
 
+// Mixed-precision CG: the two-kernel iteration of cgmixed.hip in FP32 or
+// FP16 storage, with the FP64 reliable update (Clark et al.) run on the
+// device at the end of every chunk, gated on the solve state:
+//   mix_acc   : x += x_lo + alpha p_k, x_lo = 0
+//   hop_f64 x2: ax = D (D^dag x)                  (gate 1)
+//   ru_sub    : r64 = phi - ax, |r64|^2 -> FinRU (converged, out of budget,
+//               or restart with beta = |r_true|^2 / |r_k|^2 and alpha = 0)
+//   mix_apply : r_k = r64 in sloppy storage
 template <bool HALF>
 static int cg_mixed(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* prm, const double2* phi, double2* x,
                     double phi_sq, sfh_cg_stats* stats) {
+  using T = typename std::conditional<HALF, Half, float>::type;
+  using S = typename Prec<T>::S;
   const Geom& g = gauge->g;
   const size_t n = 3 * (size_t)g.V;
-  double2* r64 = buf<double2>(c, "cg_r", n);
-  float2* xl = buf<float2>(c, "cgl_x", n);
-  if (!r64 || !xl) return SFH_E_NOMEM;
   if (int e = ensure_lp(c, gauge, HALF ? SFH_PREC_F16 : SFH_PREC_F32)) return e;
-  // sloppy vectors
-  float2 *r32 = nullptr, *p32 = nullptr, *t32 = nullptr, *v32 = nullptr;
-  __half2 *r16 = nullptr, *p16 = nullptr, *t16 = nullptr, *v16 = nullptr;
-  float *rn = nullptr, *pn = nullptr, *tn = nullptr, *vn = nullptr;
+  double2* r64 = buf<double2>(c, "cg_r", n);
+  double2* t64 = buf<double2>(c, "cg_t", n);
+  double2* ax = buf<double2>(c, "cg_v", n);
+  const bool fused = mix_fused(HALF ? SFH_PREC_F16 : SFH_PREC_F32, g);
+  MixBufs<T> b{};
+  if (int e = mix_bufs<T>(c, g, fused, b)) return e;
+  if (!r64 || !t64 || !ax || !b.x) return SFH_E_NOMEM;
+  // iteration 0 reads r_{-1} = phi from buffer 1, p_{-1} = v_{-1} = 0, alpha = beta = 0
+  hipMemsetAsync(b.x, 0, n * sizeof(float2), c->stream);
+  hipMemsetAsync(b.p[1], 0, n * sizeof(S), c->stream);
+  hipMemsetAsync(b.v, 0, n * sizeof(S), c->stream);
   if (HALF) {
-    r16 = buf<__half2>(c, "cgh_r", n); p16 = buf<__half2>(c, "cgh_p", n);
-    t16 = buf<__half2>(c, "cgh_t", n); v16 = buf<__half2>(c, "cgh_v", n);
-    rn = buf<float>(c, "cgh_rn", g.V); pn = buf<float>(c, "cgh_pn", g.V);
-    tn = buf<float>(c, "cgh_tn", g.V); vn = buf<float>(c, "cgh_vn", g.V);
-    if (!r16 || !p16 || !t16 || !v16 || !rn || !pn || !tn || !vn) return SFH_E_NOMEM;
+    hipMemsetAsync(b.pn[1], 0, g.V * sizeof(float), c->stream);
+    hipMemsetAsync(b.vn, 0, g.V * sizeof(float), c->stream);
+    f64_to_f16(c->stream, g, phi, reinterpret_cast<__half2*>(b.r[1]), b.rn[1]);
   } else {
-    r32 = buf<float2>(c, "cgl_r", n); p32 = buf<float2>(c, "cgl_p", n);
-    t32 = buf<float2>(c, "cgl_t", n); v32 = buf<float2>(c, "cgl_v", n);
-    if (!r32 || !p32 || !t32 || !v32) return SFH_E_NOMEM;
-  }
-  hipMemsetAsync(xl, 0, n * sizeof(float2), c->stream);
-  hipMemcpyAsync(r64, phi, n * sizeof(double2), hipMemcpyDeviceToDevice, c->stream);
-  auto to_sloppy_r = [&]() {
-    if (HALF) f64_to_f16(c->stream, g, r64, r16, rn);
-    else f64_to_f32(c->stream, r64, r32, n);
-  };
-  to_sloppy_r();
-  if (HALF) {
-    hipMemcpyAsync(p16, r16, n * sizeof(__half2), hipMemcpyDeviceToDevice, c->stream);
-    hipMemcpyAsync(pn, rn, g.V * sizeof(float), hipMemcpyDeviceToDevice, c->stream);
-  } else {
-    hipMemcpyAsync(p32, r32, n * sizeof(float2), hipMemcpyDeviceToDevice, c->stream);
+    f64_to_f32(c->stream, phi, reinterpret_cast<float2*>(b.r[1]), n);
   }
   CGState s{};
   s.rsq = phi_sq;
@@ -280,68 +356,42 @@ static int cg_mixed(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* prm, cons
   if (int e = upload_state(c, s)) return e;
   CGState* st = c->d_state;
   double* part = c->d_partials;
-  const float m = (float)prm->mass;
-  auto iter = [&]() {
-    if (HALF) {
-      {
-        KTimer kt(c, "hop_f16");
-        hop_f16(c->stream, g, gauge->lp.u16, p16, pn, t16, tn, m, -1.f, nullptr, nullptr, st, nullptr);
-      }
-      {
-        KTimer kt(c, "hop_dot_f16");
-        hop_f16(c->stream, g, gauge->lp.u16, t16, tn, v16, vn, m, +1.f, p16, pn, st, part);
-      }
-      KTimer kt(c, "cg_blas");
-      cg_xr_f16(c->stream, g, xl, r16, rn, p16, pn, v16, vn, st, part + kPartialSlot);
-      cg_p_f16(c->stream, g, p16, pn, r16, rn, st);
-    } else {
-      {
-        KTimer kt(c, "hop_f32");
-        hop_f32(c->stream, g, gauge->lp.u32, p32, t32, m, -1.f, nullptr, st, nullptr);
-      }
-      {
-        KTimer kt(c, "hop_dot_f32");
-        hop_f32(c->stream, g, gauge->lp.u32, t32, v32, m, +1.f, p32, st, part);
-      }
-      KTimer kt(c, "cg_blas");
-      cg_xr_f32(c->stream, g, xl, r32, p32, v32, st, part + kPartialSlot);
-      cg_p_f32(c->stream, g, p32, r32, st);
-    }
-  };
-  // The reliable update runs on the device at the end of every chunk, gated
-  // on the state: x += x_lo, r = phi - DDdag x in FP64, and either stop
-  // (true residual met, or budget spent) or restart the sloppy r and p.
-  double2* t64 = buf<double2>(c, "cg_t", n);
-  double2* ax = buf<double2>(c, "cg_v", n);
-  if (!t64 || !ax) return SFH_E_NOMEM;
   double* part_ru = part + 3 * (size_t)kPartialSlot;
+  const float m = (float)prm->mass;
+  const typename Prec<T>::L* U = reinterpret_cast<const typename Prec<T>::L*>(HALF ? (const void*)gauge->lp.u16
+                                                                                  : (const void*)gauge->lp.u32);
+  auto iter = [&]() {
+    mix_step_a<T>(c, gauge, fused, b, m, st);
+    KTimer kt(c, HALF ? "mix_b_f16" : "mix_b_f32");
+    mix_b<T>(c->stream, g, U, b, m, st, part);
+  };
   auto tail = [&]() {
     KTimer kt(c, "reliable_update");
-    ru_acc(c->stream, x, xl, n, st);
+    mix_acc<T>(c->stream, g, x, b, st);
     hop_f64(c->stream, g, gauge->U, x, t64, prm->mass, -1.0, nullptr, st, nullptr, 1);
     hop_f64(c->stream, g, gauge->U, t64, ax, prm->mass, +1.0, nullptr, st, nullptr, 1);
     ru_sub(c->stream, g, r64, phi, ax, st, part_ru);
-    if (HALF) ru_apply16(c->stream, g, r64, r16, rn, p16, pn, st);
-    else ru_apply32(c->stream, g, r64, r32, p32, st);
+    mix_apply<T>(c->stream, g, r64, b, st);
   };
   auto direct = [&]() {
-    for (int j = 0; j < kChunk; ++j) iter();
+    for (int j = 0; j < kChunkMixed; ++j) iter();
     tail();
   };
-  const std::string key =
-      HALF ? graph_key("f16", g, {gauge->U, gauge->lp.u16, x, xl, r64, phi, t64, ax, r16, p16, t16, v16, rn, pn, tn,
-                                  vn, st, part},
-                       prm->mass, get_hop_variant(2, g) * 2 + get_hop_variant(0, g))
-           : graph_key("f32", g, {gauge->U, gauge->lp.u32, x, xl, r64, phi, t64, ax, r32, p32, t32, v32, st, part},
-                       prm->mass, get_hop_variant(1, g) * 2 + get_hop_variant(0, g));
+  const std::string key = graph_key(HALF ? (fused ? "mix16f" : "mix16s") : (fused ? "mix32f" : "mix32s"), g,
+                                    {gauge->U, (const void*)U, x, r64, phi, t64, ax, b.r[0], b.r[1], b.p[0], b.p[1],
+                                     b.v, b.t, b.rn[0], b.rn[1], b.pn[0], b.pn[1], b.vn, b.tn, b.x, st, part},
+                                    prm->mass, get_hop_variant(0, g) * 4 + get_hop_variant(HALF ? 2 : 1, g));
   CGState fin{};
-  if (int e = run_chunks(c, [&]() { return launch_chunk(c, key, direct); }, fin)) return e;
+  if (int e = run_chunks(c, [&]() { return launch_chunk(c, key, direct); }, kChunkMixed, s, fin)) return e;
   if (stats) {
     stats->iterations = fin.iter;
     stats->reliable_updates = fin.rupd;
     stats->rsq = fin.rsq;
     stats->true_rel_resid = std::sqrt(fin.rsq_true / phi_sq);
-    const double per = HALF ? 528.0 : 912.0;
+    // per iteration: kernel A (or the update and the hop) reads links, r, v,
+    // p, x and writes r, p, x, t; kernel B reads links, t, p, r and writes v
+    // (compulsory bytes per site)
+    const double per = HALF ? 496.0 : 864.0;
     stats->hbm_bytes = (double)fin.iter * per * g.V + fin.rupd * 1700.0 * g.V;
   }
   return map_status(c, fin, prm->max_iter);
@@ -387,6 +437,67 @@ int cg_solve(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* prm, const doubl
     }
   }
   return rc;
+}
+
+// Times the two steps of the mixed iteration on inputs of ones with a live
+// (never finishing) solve state, n repetitions of each between HIP events.
+template <class T>
+static int time_mixed_t(sfh_ctx* c, sfh_gauge* gauge, int n, double* us_a, double* us_b) {
+  constexpr bool half = std::is_same<T, Half>::value;
+  const Geom& g = gauge->g;
+  const size_t n3 = 3 * (size_t)g.V;
+  if (int e = ensure_lp(c, gauge, half ? SFH_PREC_F16 : SFH_PREC_F32)) return e;
+  const bool fused = mix_fused(half ? SFH_PREC_F16 : SFH_PREC_F32, g);
+  MixBufs<T> b{};
+  if (int e = mix_bufs<T>(c, g, fused, b)) return e;
+  for (int k = 0; k < 2; ++k) {
+    for (void* v : {(void*)b.r[k], (void*)b.p[k]}) {
+      if (half) hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(v), 0x3C003C00u, n3, c->stream);  // (1, 1)
+      else hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(v), 0x3F800000u, 2 * n3, c->stream);   // 1.0f
+    }
+    if (half) {
+      hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(b.rn[k]), 0x3F800000u, g.V, c->stream);
+      hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(b.pn[k]), 0x3F800000u, g.V, c->stream);
+    }
+  }
+  hipMemsetAsync(b.v, 0, n3 * sizeof(typename Prec<T>::S), c->stream);
+  if (half) hipMemsetAsync(b.vn, 0, g.V * sizeof(float), c->stream);
+  hipMemsetAsync(b.x, 0, n3 * sizeof(float2), c->stream);
+  // alpha = beta = 0 keeps kernel A idempotent; target 0 and no budget keep B running
+  CGState z{};
+  z.rsq = 1.0;
+  z.max_iter = 0x7fffffff;
+  z.ctr = c->d_ctr + kCtrKinds * kCtrPerKind;
+  CGState* st = &c->d_state[1];
+  hipMemcpyAsync(st, &z, sizeof(CGState), hipMemcpyHostToDevice, c->stream);
+  double* part = c->d_partials + 4 * (size_t)kPartialSlot;
+  const float m = 0.1f;
+  const auto* U = reinterpret_cast<const typename Prec<T>::L*>(half ? (const void*)gauge->lp.u16
+                                                                    : (const void*)gauge->lp.u32);
+  auto timed = [&](auto launch, double* us) {
+    launch();
+    hipEventRecord(c->ev0, c->stream);
+    for (int k = 0; k < n; ++k) launch();
+    hipEventRecord(c->ev1, c->stream);
+    if (int e = hip_check(c, hipEventSynchronize(c->ev1), "CG kernel timing")) return e;
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    *us = 1e3 * ms / n;
+    return (int)SFH_OK;
+  };
+  if (int e = timed([&]() { mix_step_a<T>(c, gauge, fused, b, m, st); }, us_a)) return e;
+  // B changes alpha and beta; A is not run again
+  return timed([&]() { mix_b<T>(c->stream, g, U, b, m, st, part); }, us_b);
+}
+
+void set_mix_form(int precision, int form) {
+  if (precision == SFH_PREC_F32 || precision == SFH_PREC_F16) g_mix_form[precision] = form;
+}
+int get_mix_form(int precision, const Geom& g) { return mix_fused(precision, g) ? 0 : 1; }
+
+int time_mixed(sfh_ctx* c, sfh_gauge* gauge, int precision, int n, double* us_a, double* us_b) {
+  return precision == SFH_PREC_F16 ? time_mixed_t<Half>(c, gauge, n, us_a, us_b)
+                                   : time_mixed_t<float>(c, gauge, n, us_a, us_b);
 }
 
 }  // namespace sfh

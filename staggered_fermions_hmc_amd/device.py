@@ -178,6 +178,23 @@ class Gauge:
                                                   int(n), ctypes.byref(us)))
         return us.value
 
+    def tune_cg_form(self, precision="f32", n=20):
+        """Time both mixed-iteration forms on this lattice, keep the faster;
+        returns (form, microseconds per iteration)."""
+        f = ctypes.c_int()
+        us = ctypes.c_double()
+        self.ctx.check(self.ctx._lib.sfh_tune_cg_form(self.ctx.h, self.h, _abi.PREC[precision], int(n),
+                                                      ctypes.byref(f), ctypes.byref(us)))
+        return f.value, us.value
+
+    def time_cg_kernels(self, precision="f32", n=50):
+        """Average microseconds of the two steps of one mixed-precision CG
+        iteration on this lattice (update + first hop, second hop + dots)."""
+        ua, ub = ctypes.c_double(), ctypes.c_double()
+        self.ctx.check(self.ctx._lib.sfh_time_cg_kernels(self.ctx.h, self.h, _abi.PREC[precision], int(n),
+                                                         ctypes.byref(ua), ctypes.byref(ub)))
+        return ua.value, ub.value
+
     def action(self, beta, c11, c12):
         out = ctypes.c_double()
         self.ctx.check(self.ctx._lib.sfh_action(self.ctx.h, self.h, beta, c11, c12, ctypes.byref(out)))
@@ -359,6 +376,14 @@ def set_hop_variant(precision, variant):
     st = _abi.lib().sfh_set_hop_variant(_abi.PREC[precision], int(variant))
     if st:
         raise SfhError(st, "sfh_set_hop_variant")
+
+
+def set_cg_form(precision, form):
+    """Mixed-precision iteration form: 0 fused (two kernels), 1 split (update
+    kernel + plain hop + second hop), -1 by lattice size (default)."""
+    st = _abi.lib().sfh_set_cg_form(_abi.PREC[precision], int(form))
+    if st:
+        raise SfhError(st, "sfh_set_cg_form")
 
 
 def random_links(dims, eps, seed):

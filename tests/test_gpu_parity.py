@@ -151,9 +151,19 @@ def test_cg_f64_matches_oracle(ctx, dims, m):
     assert rel(g.apply_d(m, x, dagger=True), chi) < 1e-8
 
 
+@pytest.fixture(params=[0, 1], ids=["fused", "split"])
+def cg_form(request):
+    """Both forms of the mixed-precision iteration, restored to the default."""
+    for prec in ("f32", "f16"):
+        dev.set_cg_form(prec, request.param)
+    yield request.param
+    for prec in ("f32", "f16"):
+        dev.set_cg_form(prec, -1)
+
+
 @pytest.mark.parametrize("prec", ["f32", "f16"])
-@pytest.mark.parametrize("dims,m", [((4, 4, 4, 4), 0.1), ((8, 8, 8, 8), 0.05)])
-def test_cg_mixed_reaches_true_residual(ctx, dims, m, prec):
+@pytest.mark.parametrize("dims,m", [((4, 4, 4, 4), 0.1), ((8, 8, 8, 8), 0.05), ((6, 4, 8, 10), 0.07)])
+def test_cg_mixed_reaches_true_residual(ctx, dims, m, prec, cg_form):
     U = dev.random_links(dims, 0.3, seed=14)
     phi = oc.apply_d(dims, m, U, dev.random_spinor(dims, seed=15))
     g = dev.Gauge(ctx, dims, U)
@@ -165,6 +175,27 @@ def test_cg_mixed_reaches_true_residual(ctx, dims, m, prec):
     xr, *_ = oc.cg(dims, m, U, phi, tol=1e-12)
     assert rel(x, xr) < 1e-8
     assert st["reliable_updates"] >= 1
+
+
+@pytest.mark.parametrize("prec", ["f32", "f16"])
+def test_cg_forms_agree(ctx, prec):
+    """The fused and split mixed iterations run the same recurrence: equal
+    iteration and update counts, solutions within the solve tolerance."""
+    dims, m = (8, 8, 8, 12), 0.03
+    U = dev.random_links(dims, 0.3, seed=61)
+    g = dev.Gauge(ctx, dims, U)
+    phi = dev.random_spinor(dims, seed=62)
+    out = {}
+    try:
+        for form in (0, 1):
+            dev.set_cg_form(prec, form)
+            out[form] = g.cg(phi, m, tol=1e-10, precision=prec)
+    finally:
+        dev.set_cg_form(prec, -1)
+    (x0, s0), (x1, s1) = out[0], out[1]
+    assert abs(s0["iterations"] - s1["iterations"]) <= 2, (s0, s1)
+    assert s0["true_rel_resid"] < 1e-10 and s1["true_rel_resid"] < 1e-10
+    assert rel(x0, x1) < 1e-8
 
 
 def test_cg_error_paths(ctx):
@@ -306,6 +337,22 @@ def test_tune_hop_picks_a_form(ctx):
     finally:
         for prec in ("f64", "f32", "f16"):
             dev.set_hop_variant(prec, -1)
+
+
+def test_tune_cg_form_and_kernel_times(ctx):
+    dims = (8, 8, 8, 8)
+    g = dev.Gauge(ctx, dims, dev.random_links(dims, 0.3, seed=44))
+    try:
+        for prec in ("f32", "f16"):
+            ua, ub = g.time_cg_kernels(prec, n=5)
+            assert ua > 0 and ub > 0
+            f, us = g.tune_cg_form(prec, n=5)
+            assert f in (0, 1) and us > 0
+    finally:
+        for prec in ("f32", "f16"):
+            dev.set_cg_form(prec, -1)
+    with pytest.raises(dev.SfhError):
+        g.time_cg_kernels("f64", n=5)
 
 
 def test_dropin_selftest_binary():
