@@ -86,7 +86,7 @@ __device__ __forceinline__ Site site_of(const Geom& g, int nb) {
   s.lb = xcd_block(nb);
   const int hw = blockDim.x >> 1;
   s.p = threadIdx.x >= hw;
-  s.cb = s.lb * hw + (threadIdx.x - s.p * hw);
+  s.cb = range_start(g, s.lb, hw) + (threadIdx.x - s.p * hw);
   s.i = s.p * g.Vh + s.cb;
   s.q = (1 - s.p) * g.Vh;
   return s;

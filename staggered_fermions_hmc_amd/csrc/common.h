@@ -37,6 +37,24 @@ inline Geom make_geom(const int* d) {
   return g;
 }
 
+// First checkerboard index of the range of `half` sites that logical
+// workgroup lb covers (on each parity).  When a column of fixed (n0, n1)
+// (S1 checkerboard sites) splits into whole ranges, workgroups walk the
+// columns fastest: lb = chunk * L0 L1 + n0 L1 + n1.  Consecutive workgroups
+// of one XCD are then y-neighbours and x-neighbours are L1 workgroups apart,
+// so the backward links and spinors they share are still in that XCD's L2;
+// in plain checkerboard order x-neighbours are L1 L2 L3 / (2 half)
+// workgroups apart and their links come from HBM a second time.  Otherwise
+// ranges follow checkerboard order.  Either way a bijection onto [0, Vh).
+__device__ __forceinline__ int range_start(const Geom& g, int lb, int half) {
+  if (g.S1 % half == 0) {
+    const int ncol = g.L[0] * g.L[1];
+    const int chunk = lb / ncol;
+    return (lb - chunk * ncol) * g.S1 + chunk * half;
+  }
+  return lb * half;
+}
+
 // Coordinates of checkerboard index cb at parity p.
 SFH_HD void cb_coords(const Geom& g, int cb, int p, int n[4]) {
   int t = cb;

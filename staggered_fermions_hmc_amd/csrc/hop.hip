@@ -7,7 +7,9 @@
 // Each workgroup takes the same checkerboard range on both parities (first
 // half of its lanes even sites, second half odd), so a link loaded as the
 // forward link of one site is reused from cache as the backward link of its
-// neighbour.  Workgroups are remapped so each XCD sweeps a contiguous range.
+// neighbour.  Workgroups are remapped so each XCD sweeps a contiguous range,
+// in an order that keeps x- and y-neighbour ranges close in time
+// (range_start).
 // The hop is HBM-bound (~0.9 flop/byte in FP64): no MFMA.
 #include "hopcore.h"
 #include "internal.h"
@@ -30,7 +32,7 @@ __global__ __launch_bounds__(256) void k_hop(Geom g, const typename Prec<T>::L* 
   const int lb = xcd_block(nb);
   const int hw = blockDim.x >> 1;
   const int p = threadIdx.x >= hw;
-  const int cb = lb * hw + (threadIdx.x - p * hw);
+  const int cb = range_start(g, lb, hw) + (threadIdx.x - p * hw);
   double dotv = 0.0;
   if (cb < g.Vh) {
     const int V = g.V;
