@@ -15,7 +15,10 @@ library stream, and the CPU baseline (the FP64 oracle, timed on a bounded
 sample on the host).
 
 ``--workload 32^4`` runs the 32^4, m = 0.01 solve with FP16 inner iterations
-instead.
+instead; ``batch`` the 64 uneven batched systems, ``serve`` the solve service
+under Poisson arrivals, ``precision`` the errors of the mixed-precision
+results against the FP64 oracle at 16^4, ``traj32`` a 32^4 Symanzik HMC
+trajectory in FP64 and FP16-mixed.
 """
 import argparse
 import json
@@ -78,7 +81,9 @@ def cpu_baseline(dims, mass, U, phi, iters_per_solve, budget_s=15.0):
 
 
 def traffic_from_profiles(kernel, workload):
-    path = os.path.join(ROOT, "profiles", "hop_traffic.json")
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a kernel of known size)."""
+    path = os.path.join(ROOT, "profiles", "kernel_traffic.json")
     if not os.path.exists(path):
         return None
     try:
@@ -275,12 +280,116 @@ def bench_serve(args, dev):
     print(json.dumps(out))
 
 
+def _base_line(value, unit, steps, ms, dtype, workload, extra):
+    out = {"metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": 1,
+           "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+           "data": "synthetic (seeded Philox links, momenta and Gaussian fields)",
+           "config": {"workload": workload, "parallelism": "single GPU"}, "roofline": None, "cpu_baseline": None}
+    out.update(extra)
+    return out
+
+
+def bench_precision(args, dev):
+    """Errors of the mixed-precision results against the FP64 oracle on the
+    16^4, m = 0.05 lattice (links exp(i 0.3 H)): solution and true residual
+    of one solve, the force L-inf built from it, and one HMC trajectory
+    (Wilson beta = 5.6, tau = 1, 10 steps, one pseudofermion): Delta H and
+    the plaquette at its end, for the device FP64, FP32-mixed and
+    FP16-mixed paths."""
+    from oracle import oracle as oc
+    oc.set_threads(min(16, os.cpu_count() or 1))
+    dims, m, tol = (16, 16, 16, 16), 0.05, 1e-10
+    U = dev.random_links(dims, 0.3, seed=1234)
+    chi = dev.random_spinor(dims, seed=99)
+    P0 = dev.random_momenta(dims, seed=77)
+    ctx = dev.Context(0)
+    g = dev.Gauge(ctx, dims, U)
+    phi = g.apply_d(m, chi)
+    log("[bench] precision: oracle solve and force")
+    xo, *_ = oc.cg(dims, m, U, phi, tol=1e-13)
+    Fo = oc.force(dims, U, xo, oc.apply_d(dims, m, U, xo, dagger=True), 5.6, 1.0, 0.0)
+    solve, force = {}, {}
+    for prec in ("f64", "f32", "f16"):
+        x, st = g.cg(phi, m, tol=tol, precision=prec, max_iter=100000)
+        r = phi - oc.apply_ddag(dims, m, U, x)
+        F = g.force(5.6, 1.0, 0.0, x, g.apply_d(m, x, dagger=True))
+        solve[prec] = {"iterations": st["iterations"], "true_rel_residual": float(np.sqrt(r @ r / (phi @ phi))),
+                       "rel_error_vs_oracle": float(np.linalg.norm(x - xo) / np.linalg.norm(xo))}
+        force[prec] = {"linf_error": float(np.max(np.abs(F - Fo))), "linf_force": float(np.max(np.abs(Fo))),
+                       "rel_linf_error": float(np.max(np.abs(F - Fo)) / np.max(np.abs(Fo)))}
+    md = dict(mass=m, beta=5.6, c11=1.0, c12=0.0, eps=0.1, nsteps=10)
+    log("[bench] precision: oracle trajectory")
+    t0 = time.perf_counter()
+    Uo, Ho, _ = oc.hmc_trajectory(dims, m, U, chi, P0, 0.0, beta=5.6, c11=1.0, c12=0.0, eps=0.1, nsteps=10,
+                                  tol=1e-13)
+    t_oracle = time.perf_counter() - t0
+    plaq_o = oc.avg_plaq(dims, Uo)
+    traj = {"oracle": {"delta_H": float(-Ho[6]), "plaquette": plaq_o, "seconds": t_oracle}}
+    for prec in ("f64", "f32", "f16"):
+        g.upload(U)
+        t0 = time.perf_counter()
+        H, _ = g.hmc_trajectory(chi, P0, 0.0, tol=tol, max_iter=100000, precision=prec, **md)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        plaq = g.avg_plaq()
+        traj[prec] = {"delta_H": float(-H[6]), "delta_H_error": float(abs(H[6] - Ho[6])), "plaquette": plaq,
+                      "plaquette_error": float(abs(plaq - plaq_o)), "seconds": round(dt, 4)}
+        log(f"[bench] precision: {prec} trajectory dH {-H[6]:.6f} (oracle {-Ho[6]:.6f}) in {dt:.3f}s")
+    print(json.dumps(_base_line(
+        round(1.0 / traj["f32"]["seconds"], 3), "trajectories/s", 1, round(traj["f32"]["seconds"] * 1e3, 2),
+        "f32 inner / f64 reliable updates, reductions, forces and Metropolis",
+        "16^4, m=0.05, links exp(i 0.3 H): errors against the FP64 CPU oracle of one solve to true residual 1e-10, "
+        "of the force built from it, and of one Wilson HMC trajectory (beta 5.6, tau 1, 10 steps)",
+        {"solve": solve, "force": force, "trajectory": traj})))
+
+
+def bench_traj32(args, dev):
+    """One HMC trajectory at 32^4 (Symanzik tree level c11 = 5/3, c12 = -1/12,
+    beta = 4.0, tau = 1, 20 steps, m = 0.01, one pseudofermion): device FP64
+    CG against FP16-mixed CG (FP64 reliable updates), both to a true
+    relative residual of 1e-10, from the same start; time per trajectory,
+    Delta H, plaquette, and the force L-inf difference at the start."""
+    dims, m, tol = (32, 32, 32, 32), 0.01, 1e-10
+    U = dev.random_links(dims, 0.3, seed=1234)
+    chi = dev.random_spinor(dims, seed=99)
+    P0 = dev.random_momenta(dims, seed=77)
+    ctx = dev.Context(0)
+    g = dev.Gauge(ctx, dims, U)
+    phi = g.apply_d(m, chi)
+    md = dict(mass=m, beta=4.0, c11=5.0 / 3.0, c12=-1.0 / 12.0, eps=0.05, nsteps=20)
+    F = {}
+    for prec in ("f64", "f16"):
+        x, _ = g.cg(phi, m, tol=tol, precision=prec, max_iter=100000)
+        F[prec] = g.force(4.0, 5.0 / 3.0, -1.0 / 12.0, x, g.apply_d(m, x, dagger=True))
+    traj = {}
+    for prec in ("f64", "f16"):
+        g.upload(U)
+        t0 = time.perf_counter()
+        H, _ = g.hmc_trajectory(chi, P0, 0.0, tol=tol, max_iter=100000, precision=prec, **md)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        traj[prec] = {"seconds": round(dt, 4), "delta_H": float(-H[6]), "plaquette": g.avg_plaq()}
+        log(f"[bench] traj32: {prec} trajectory {dt:.3f}s dH {-H[6]:.6f}")
+    dF = float(np.max(np.abs(F["f16"] - F["f64"])))
+    print(json.dumps(_base_line(
+        round(1.0 / traj["f16"]["seconds"], 4), "trajectories/s", 1, round(traj["f16"]["seconds"] * 1e3, 1),
+        "f16 inner / f64 reliable updates, reductions, forces and Metropolis",
+        "32^4 HMC trajectory, Symanzik (5/3, -1/12), beta 4.0, tau 1, 20 steps, m=0.01, CG to true residual "
+        "1e-10: FP16-mixed vs FP64 device CG",
+        {"trajectory": traj, "speedup_mixed_vs_fp64": round(traj["f64"]["seconds"] / traj["f16"]["seconds"], 3),
+         "delta_H_difference": abs(traj["f16"]["delta_H"] - traj["f64"]["delta_H"]),
+         "plaquette_difference": abs(traj["f16"]["plaquette"] - traj["f64"]["plaquette"]),
+         "force_linf_difference": dF, "force_linf": float(np.max(np.abs(F["f64"]))),
+         "force_rel_linf_difference": dF / float(np.max(np.abs(F["f64"])))})))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="16^4", choices=sorted(WORKLOADS) + ["batch", "serve"])
+    ap.add_argument("--workload", default="16^4",
+                    choices=sorted(WORKLOADS) + ["batch", "serve", "precision", "traj32"])
     ap.add_argument("--rates", default="0.5,1,2", help="serve: arrival rates as multiples of the measured "
                                                      "sequential FP64 capacity")
     ap.add_argument("--requests", type=int, default=200, help="serve: requests per arrival rate")
@@ -302,6 +411,10 @@ def main():
         return bench_batch(args, dev)
     if args.workload == "serve":
         return bench_serve(args, dev)
+    if args.workload == "precision":
+        return bench_precision(args, dev)
+    if args.workload == "traj32":
+        return bench_traj32(args, dev)
 
     wl = WORKLOADS[args.workload]
     dims, mass = wl["dims"], wl["mass"]
