@@ -15,7 +15,9 @@ constexpr int kMaxRhs = 8;  // right-hand sides in flight per system
 template <class T> struct BSys {
   Geom g;
   const typename Prec<T>::L* U;
+  const double2* U64;  // FP64 links, for the reliable updates of mixed solves
   typename Prec<T>::R mass;
+  double mass64;
   int nslot;
   int nb_hop;
   int slot[kMaxRhs];
@@ -26,6 +28,10 @@ template <class T> struct BSlot {
   typename Prec<T>::S *x, *r, *p, *t, *v;
   double* part;
   int V;
+  // mixed precision: right-hand side, FP64 solution (the caller's x), and the
+  // FP64 work vectors of the reliable update
+  const double2* b;
+  double2 *x64, *r64, *t64, *ax;
 };
 
 // T = float: FP32 inner iterations with FP64 reliable updates; T = double:
@@ -44,8 +50,10 @@ template <class T> class Engine {
   };
 
   // use_graph: replay each chunk as a captured HIP graph (best when the set
-  // of active solves is stable for many chunks, as in a fixed batch);
-  // otherwise launch the chunk's kernels directly.
+  // of solves is stable for many chunks, as in a fixed batch); otherwise
+  // launch the chunk's kernels directly.  A chunk is `chunk` iterations of
+  // every active solve, then (mixed precision) the reliable update of every
+  // solve that paused for one, then one copy of all solve states to the host.
   Engine(sfh_ctx* c, double tol, int max_iter, double delta, int chunk, bool use_graph = true);
   ~Engine();
   Engine(const Engine&) = delete;
@@ -89,6 +97,7 @@ template <class T> class Engine {
     BSlot<T> dev{};
   };
   int rebuild();
+  void launch_tail();
 
   sfh_ctx* c_;
   double tol_;
@@ -108,6 +117,14 @@ template <class T> class Engine {
   BSys<T>* dsys_ = nullptr;
   BSlot<T>* dslot_ = nullptr;
   int2 *dmh_ = nullptr, *dmb_ = nullptr;
+  // slots in the current launch tables, their gathered states (device and
+  // pinned host copies), and how many of them have finished
+  std::vector<int> mapped_;
+  int* dids_ = nullptr;
+  CGState* dstates_ = nullptr;
+  CGState* hstates_ = nullptr;
+  size_t hstates_cap_ = 0;
+  int finished_mapped_ = 0;
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t gexec_ = nullptr;
 };

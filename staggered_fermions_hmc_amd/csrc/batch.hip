@@ -2,9 +2,10 @@
 // and up to 8 right-hand sides per lattice sharing its links) advanced
 // together, one launch per CG phase for every active solve.
 //
-// * The hop kernel walks a block map of (system, block) pairs; each block
-//   loads its links once and applies them to every active right-hand side
-//   of its system (multi-RHS: the link bytes are shared by the RHS block).
+// * The hop kernel walks a block map of (system, block) pairs, one lane per
+//   site; each block applies its links to every active right-hand side of
+//   its system in turn (multi-RHS: the links come from HBM once per block,
+//   from cache for the other right-hand sides).
 // * The vector kernels walk a map of (slot, block) pairs, a slot being one
 //   right-hand side in flight.
 // * Every slot has its own device CGState and reduction partials, and a
@@ -13,8 +14,10 @@
 // * A chunk of iterations is captured into a HIP graph and replayed until the
 //   set of active slots changes.
 // * Mixed precision (FP32 inner): a slot whose iterated residual has dropped
-//   by the reliable-update factor pauses; between chunks the host runs its
-//   FP64 reliable update and resumes it while the other slots keep going.
+//   by the reliable-update factor pauses; the chunk's tail runs the FP64
+//   reliable update of every paused slot on the device (x accumulated in
+//   FP64, true residual with the FP64 links, restart or stop) while the
+//   others keep their place.  All states then reach the host in one copy.
 // * Slots can be admitted between chunks (continuous batching): the solve
 //   service feeds the engine as requests arrive; each system owns a fixed
 //   pool of slot memory, so admission is by free memory.
@@ -34,32 +37,59 @@ namespace {
 
 constexpr int kBlas = 256;
 
+// Neighbour indices and signs of one site in all four directions.
+template <class R> struct SiteNb {
+  int fw[4], bw[4];
+  R sf[4], sb[4];
+};
+
+// acc = sum_mu eta [bup U_mu(n) in(n+mu) - bdn U_mu(n-mu)^dag in(n-mu)]
+template <class T, class LT>
+__device__ __forceinline__ void site_hop(const LT* __restrict__ U, const typename Prec<T>::S* __restrict__ in, int V,
+                                         int i, int q, const SiteNb<typename Prec<T>::R>& nb,
+                                         vec3<typename Prec<T>::R>& acc) {
+  using R = typename Prec<T>::R;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) acc.c[c] = {R(0), R(0)};
+#pragma unroll
+  for (int mu = 0; mu < 4; ++mu) {
+    mat3<R> Uf, Ub;
+    vec3<R> vf, vb;
+    load_link<T>(U, V, mu, i, Uf);
+    load_spinor<T>(in, nullptr, V, q + nb.fw[mu], vf);
+    load_link<T>(U, V, mu, q + nb.bw[mu], Ub);
+    load_spinor<T>(in, nullptr, V, q + nb.bw[mu], vb);
+    acc_mv(acc, Uf, vf, nb.sf[mu]);
+    acc_mdv(acc, Ub, vb, -nb.sb[mu]);
+  }
+}
+
+// One lane per site over (system, block) pairs; each block covers one
+// checkerboard range on both parities (as the single-system hop) and applies
+// the hop to every active right-hand side of its system in turn: the links
+// come from HBM once and from cache for the other right-hand sides.
 // phase DOT=false: t = D^dag p ; DOT=true: v = D t and <p, v>.
 template <class T, bool DOT>
 __global__ __launch_bounds__(256) void kb_hop(const BSys<T>* __restrict__ sys, const BSlot<T>* __restrict__ slots,
                                               const int2* __restrict__ bmap, typename Prec<T>::R hb) {
   using R = typename Prec<T>::R;
   const int2 m = bmap[blockIdx.x];
-  const BSys<T> S = sys[m.x];
+  const BSys<T>& S = sys[m.x];  // read in place: a private copy with a dynamically indexed slot[] spills
   const int lb = m.y;
   const Geom& g = S.g;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int mu = lane >> 4, sl = lane & 15;
-  const int p = wave >> 1;
-  const int cb = lb * 32 + (wave & 1) * 16 + sl;
+  const int hw = blockDim.x >> 1;
+  const int p = threadIdx.x >= hw;
+  const int cb = range_start(g, lb, hw) + (threadIdx.x - p * hw);
   const bool valid = cb < g.Vh;
   const int V = g.V;
   const int i = p * g.Vh + (valid ? cb : 0);
   const int q = (1 - p) * g.Vh;
-  int n[4];
-  cb_coords(g, valid ? cb : 0, p, n);
-  int fw = 0, bw = 0;
-  R sf = R(1), sb = R(1);
-  mat3<R> Uf, Ub;
+  SiteNb<R> nb;
   if (valid) {
-    neighbours(g, cb, n, mu, fw, bw, sf, sb);
-    load_link<T>(S.U, V, mu, i, Uf);
-    load_link<T>(S.U, V, mu, q + bw, Ub);
+    int n[4];
+    cb_coords(g, cb, p, n);
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) neighbours(g, cb, n, mu, nb.fw[mu], nb.bw[mu], nb.sf[mu], nb.sb[mu]);
   }
   __shared__ int sdone;
   for (int j = 0; j < S.nslot; ++j) {
@@ -71,35 +101,23 @@ __global__ __launch_bounds__(256) void kb_hop(const BSys<T>* __restrict__ sys, c
     if (dn) continue;
     const typename Prec<T>::S* in = DOT ? L.t : L.p;
     typename Prec<T>::S* out = DOT ? L.v : L.t;
-    vec3<R> acc;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) acc.c[c] = {R(0), R(0)};
-    if (valid) {
-      vec3<R> vf, vb;
-      load_spinor<T>(in, nullptr, V, q + fw, vf);
-      load_spinor<T>(in, nullptr, V, q + bw, vb);
-      acc_mv(acc, Uf, vf, sf);
-      acc_mdv(acc, Ub, vb, -sb);
-    }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      R re = acc.c[c].re, im = acc.c[c].im;
-      re += __shfl_xor(re, 16, 64);
-      im += __shfl_xor(im, 16, 64);
-      re += __shfl_xor(re, 32, 64);
-      im += __shfl_xor(im, 32, 64);
-      acc.c[c] = {re, im};
-    }
     double dotv = 0.0;
-    if (valid && mu < 3) {
-      using SS = typename Prec<T>::S;
-      const cpx<R> self = ldc(in + (size_t)mu * V + i);
-      const cpx<R> ac = mu == 0 ? acc.c[0] : (mu == 1 ? acc.c[1] : acc.c[2]);
-      const R re = S.mass * self.re + hb * ac.re, im = S.mass * self.im + hb * ac.im;
-      out[(size_t)mu * V + i] = SS{re, im};
+    if (valid) {
+      vec3<R> acc, self, res;
+      site_hop<T>(S.U, in, V, i, q, nb, acc);
+      load_spinor<T>(in, nullptr, V, i, self);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        res.c[c].re = S.mass * self.c[c].re + hb * acc.c[c].re;
+        res.c[c].im = S.mass * self.c[c].im + hb * acc.c[c].im;
+      }
+      store_spinor<T>(out, nullptr, V, i, res);
       if constexpr (DOT) {
-        const cpx<R> w = ldc(L.p + (size_t)mu * V + i);
-        dotv = (double)w.re * (double)re + (double)w.im * (double)im;
+        vec3<R> w;
+        load_spinor<T>(L.p, nullptr, V, i, w);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          dotv += (double)w.c[c].re * (double)res.c[c].re + (double)w.c[c].im * (double)res.c[c].im;
       }
     }
     if constexpr (DOT) grid_reduce(dotv, L.part, ctr_of(L.st->ctr, CNT_PV), S.nb_hop, lb, FinPV{L.st});
@@ -159,6 +177,125 @@ __global__ __launch_bounds__(kBlas) void kb_p(const BSlot<T>* __restrict__ slots
   }
 }
 
+// Reliable update of every paused mixed solve, on the device (gated on each
+// slot's state, so solves that did not pause cost one state read):
+//   kb_ru_acc   x64 += x, x = 0
+//   kb_ru_hop   t64 = D^dag x64, then ax = D t64 (FP64 links, loaded once per
+//               block for all paused right-hand sides of the system)
+//   kb_ru_sub   r64 = b - ax, |r64|^2 -> FinRU (restart, converged or out of budget)
+//   kb_ru_apply r = r64 in FP32, p = r + beta p
+template <class T>
+__global__ __launch_bounds__(kBlas) void kb_ru_acc(const BSlot<T>* __restrict__ slots, const int2* __restrict__ bmap) {
+  const int2 m = bmap[blockIdx.x];
+  const BSlot<T> L = slots[m.x];
+  if (m.y == 0 && threadIdx.x == 0) L.st->ru_apply = 0;  // read only by kb_ru_apply of this chunk
+  if (!L.st->ru_pending) return;
+  const int i = m.y * kBlas + threadIdx.x;
+  if (i >= L.V) return;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const size_t k = (size_t)c * L.V + i;
+    const auto xl = L.x[k];
+    double2 h = L.x64[k];
+    h.x += (double)xl.x;
+    h.y += (double)xl.y;
+    L.x64[k] = h;
+    L.x[k] = {0, 0};
+  }
+}
+
+template <class T, int PHASE>
+__global__ __launch_bounds__(256) void kb_ru_hop(const BSys<T>* __restrict__ sys, const BSlot<T>* __restrict__ slots,
+                                                 const int2* __restrict__ bmap) {
+  const int2 m = bmap[blockIdx.x];
+  const BSys<T>& S = sys[m.x];  // read in place: a private copy with a dynamically indexed slot[] spills
+  int any = 0;
+  for (int j = 0; j < S.nslot; ++j) any |= slots[S.slot[j]].st->ru_pending;
+  if (!any) return;  // uniform over the block
+  const int lb = m.y;
+  const Geom& g = S.g;
+  const int hw = blockDim.x >> 1;
+  const int p = threadIdx.x >= hw;
+  const int cb = range_start(g, lb, hw) + (threadIdx.x - p * hw);
+  const bool valid = cb < g.Vh;
+  const int V = g.V;
+  const int i = p * g.Vh + (valid ? cb : 0);
+  const int q = (1 - p) * g.Vh;
+  SiteNb<double> nb;
+  if (valid) {
+    int n[4];
+    cb_coords(g, cb, p, n);
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) neighbours(g, cb, n, mu, nb.fw[mu], nb.bw[mu], nb.sf[mu], nb.sb[mu]);
+  }
+  const double hb = PHASE == 0 ? -0.5 : 0.5;
+  for (int j = 0; j < S.nslot; ++j) {
+    const BSlot<T> L = slots[S.slot[j]];
+    if (!valid || !L.st->ru_pending) continue;  // no barrier in this loop
+    const double2* in = PHASE == 0 ? L.x64 : L.t64;
+    double2* out = PHASE == 0 ? L.t64 : L.ax;
+    vec3<double> acc, self;
+    site_hop<double>(S.U64, in, V, i, q, nb, acc);
+    load_spinor<double>(in, nullptr, V, i, self);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      out[(size_t)c * V + i] = make_double2(S.mass64 * self.c[c].re + hb * acc.c[c].re,
+                                            S.mass64 * self.c[c].im + hb * acc.c[c].im);
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlas) void kb_ru_sub(const BSlot<T>* __restrict__ slots, const int2* __restrict__ bmap) {
+  const int2 m = bmap[blockIdx.x];
+  const BSlot<T> L = slots[m.x];
+  CGState* st = L.st;
+  if (!st->ru_pending) return;
+  const int V = L.V;
+  const int nb = (V + kBlas - 1) / kBlas;
+  const int i = m.y * kBlas + threadIdx.x;
+  double rr = 0.0;
+  if (i < V) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const size_t k = (size_t)c * V + i;
+      const double2 a = L.b[k], b = L.ax[k];
+      const double2 d = make_double2(a.x - b.x, a.y - b.y);
+      L.r64[k] = d;
+      rr += d.x * d.x + d.y * d.y;
+    }
+  }
+  grid_reduce(rr, L.part, ctr_of(st->ctr, CNT_RU), nb, m.y, FinRU{st});
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlas) void kb_ru_apply(const BSlot<T>* __restrict__ slots, const int2* __restrict__ bmap) {
+  const int2 m = bmap[blockIdx.x];
+  const BSlot<T> L = slots[m.x];
+  if (!L.st->ru_apply) return;
+  const int i = m.y * kBlas + threadIdx.x;
+  if (i >= L.V) return;
+  const float be = (float)L.st->beta;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const size_t k = (size_t)c * L.V + i;
+    const double2 t = L.r64[k];
+    const float2 rv = make_float2((float)t.x, (float)t.y);
+    float2 pp = L.p[k];
+    pp.x = rv.x + be * pp.x;
+    pp.y = rv.y + be * pp.y;
+    L.r[k] = rv;
+    L.p[k] = pp;
+  }
+}
+
+// All solve states of the launch tables into one array, for one copy to the host.
+template <class T>
+__global__ void kb_gather(const BSlot<T>* __restrict__ slots, const int* __restrict__ ids, int n,
+                          CGState* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] = *slots[ids[k]].st;
+}
+
 inline size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 
 }  // namespace
@@ -166,8 +303,8 @@ inline size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
 template <class T>
 size_t Engine<T>::slot_bytes(int V) {
   const size_t n = 3 * (size_t)V;
-  const int nbh = (V / 2 + 31) / 32, nbb = (V + kBlas - 1) / kBlas;
-  return align256(n * sizeof(S)) * (kMixed ? 5 : 4) + (kMixed ? align256(n * sizeof(double2)) : 0) +
+  const int nbh = (V / 2 + 127) / 128, nbb = (V + kBlas - 1) / kBlas;
+  return align256(n * sizeof(S)) * (kMixed ? 5 : 4) + (kMixed ? 3 * align256(n * sizeof(double2)) : 0) +
          align256((std::max(nbh, nbb) + kRedGroups) * sizeof(double)) + align256(sizeof(CGState)) + align256(kCtrBytes);
 }
 
@@ -185,11 +322,28 @@ void Engine<T>::launch_chunk() {
   }
 }
 
+// Reliable updates of the paused mixed solves, then the state gather and its
+// copy to pinned host memory.
+template <class T>
+void Engine<T>::launch_tail() {
+  if constexpr (kMixed) {
+    hipLaunchKernelGGL((kb_ru_acc<T>), dim3(nbb_), dim3(kBlas), 0, c_->stream, dslot_, dmb_);
+    hipLaunchKernelGGL((kb_ru_hop<T, 0>), dim3(nbh_), dim3(256), 0, c_->stream, dsys_, dslot_, dmh_);
+    hipLaunchKernelGGL((kb_ru_hop<T, 1>), dim3(nbh_), dim3(256), 0, c_->stream, dsys_, dslot_, dmh_);
+    hipLaunchKernelGGL((kb_ru_sub<T>), dim3(nbb_), dim3(kBlas), 0, c_->stream, dslot_, dmb_);
+    hipLaunchKernelGGL((kb_ru_apply<T>), dim3(nbb_), dim3(kBlas), 0, c_->stream, dslot_, dmb_);
+  }
+  const int n = (int)mapped_.size();
+  hipLaunchKernelGGL((kb_gather<T>), dim3((n + 63) / 64), dim3(64), 0, c_->stream, dslot_, dids_, n, dstates_);
+  hipMemcpyAsync(hstates_, dstates_, n * sizeof(CGState), hipMemcpyDeviceToHost, c_->stream);
+}
+
 template <class T>
 Engine<T>::~Engine() {
   if (gexec_) hipGraphExecDestroy(gexec_);
   if (graph_) hipGraphDestroy(graph_);
   for (void* p : owned_) hipFree(p);
+  if (hstates_) hipHostFree(hstates_);
 }
 
 template <class T>
@@ -234,11 +388,16 @@ int Engine<T>::add_system(const sfh_gauge* gauge, double mass, int pool) {
     L.p = reinterpret_cast<S*>(take(n * sizeof(S)));
     L.t = reinterpret_cast<S*>(take(n * sizeof(S)));
     L.v = reinterpret_cast<S*>(take(n * sizeof(S)));
-    if (kMixed) h.r64 = reinterpret_cast<double2*>(take(n * sizeof(double2)));
+    if (kMixed) {
+      L.r64 = reinterpret_cast<double2*>(take(n * sizeof(double2)));
+      L.t64 = reinterpret_cast<double2*>(take(n * sizeof(double2)));
+      L.ax = reinterpret_cast<double2*>(take(n * sizeof(double2)));
+      h.r64 = L.r64;
+    }
     L.st = reinterpret_cast<CGState*>(take(sizeof(CGState)));
     h.ctr = reinterpret_cast<unsigned*>(take(kCtrBytes));
     L.part = reinterpret_cast<double*>(p);
-    const int nbh = (V / 2 + 31) / 32, nbb = (V + kBlas - 1) / kBlas;
+    const int nbh = (V / 2 + 127) / 128, nbb = (V + kBlas - 1) / kBlas;
     const size_t used = (size_t)(p - (base + sb * k)) + align256((std::max(nbh, nbb) + kRedGroups) * sizeof(double));
     if (used > sb) return -set_err(c_, SFH_E_INVALID, "slot layout exceeds its allocation");
     s.pool.push_back((int)slots_.size());
@@ -278,6 +437,8 @@ int Engine<T>::admit(int sys, const double2* b, double2* x, long long ticket) {
   h.t_admit = std::chrono::steady_clock::now();
   BSlot<T>& L = h.dev;
   if (!kMixed) L.x = reinterpret_cast<S*>(x);
+  L.b = b;
+  L.x64 = x;
   hipMemsetAsync(x, 0, n * sizeof(double2), c_->stream);
   const double phi_sq = dev_dot(c_, b, b, n);
   h.phi_sq = phi_sq;
@@ -292,11 +453,12 @@ int Engine<T>::admit(int sys, const double2* b, double2* x, long long ticket) {
   z.ctr = h.ctr;
   if (phi_sq < 1e-28) {  // x = 0 (Fermions.cpp:562-564)
     z.done = 1;
+    z.converged = 1;
     h.finished = true;
   }
+  h.last = z;
   if (kMixed) {
-    hipMemcpyAsync(h.r64, b, n * sizeof(double2), hipMemcpyDeviceToDevice, c_->stream);
-    f64_to_f32(c_->stream, h.r64, reinterpret_cast<float2*>(L.r), n);
+    f64_to_f32(c_->stream, b, reinterpret_cast<float2*>(L.r), n);
     hipMemsetAsync(L.x, 0, n * sizeof(S), c_->stream);
   } else {
     hipMemcpyAsync(L.r, b, n * sizeof(S), hipMemcpyDeviceToDevice, c_->stream);
@@ -317,12 +479,15 @@ int Engine<T>::rebuild() {
   std::vector<int2> mh, mb;
   std::vector<BSlot<T>> hl(slots_.size());
   for (size_t k = 0; k < slots_.size(); ++k) hl[k] = slots_[k].dev;
+  mapped_.clear();
   for (size_t s = 0; s < sys_.size(); ++s) {
     BSys<T> d{};
     d.g = sys_[s].gauge->g;
     d.U = sys_[s].U;
+    d.U64 = sys_[s].gauge->U;
     d.mass = (R)sys_[s].mass;
-    d.nb_hop = (d.g.Vh + 31) / 32;
+    d.mass64 = sys_[s].mass;
+    d.nb_hop = (d.g.Vh + 127) / 128;
     d.nslot = 0;
     for (int k : sys_[s].pool)
       if (slots_[k].used && !slots_[k].finished) d.slot[d.nslot++] = k;
@@ -331,12 +496,14 @@ int Engine<T>::rebuild() {
     if (d.nslot == 0) continue;
     for (int b = 0; b < d.nb_hop; ++b) mh.push_back(make_int2(sidx, b));
     for (int j = 0; j < d.nslot; ++j) {
+      mapped_.push_back(d.slot[j]);
       const int nbb = (d.g.V + kBlas - 1) / kBlas;
       for (int b = 0; b < nbb; ++b) mb.push_back(make_int2(d.slot[j], b));
     }
   }
   nbh_ = (int)mh.size();
   nbb_ = (int)mb.size();
+  finished_mapped_ = 0;
   dirty_ = false;
   if (gexec_) {
     hipGraphExecDestroy(gexec_);
@@ -345,20 +512,34 @@ int Engine<T>::rebuild() {
     graph_ = nullptr;
   }
   if (!nbh_) return SFH_OK;
-  dsys_ = static_cast<BSys<T>*>(scratch(c_, kMixed ? "eng32_sys" : "eng64_sys", hs.size() * sizeof(BSys<T>)));
-  dslot_ = static_cast<BSlot<T>*>(scratch(c_, kMixed ? "eng32_slot" : "eng64_slot", hl.size() * sizeof(BSlot<T>)));
-  dmh_ = static_cast<int2*>(scratch(c_, kMixed ? "eng32_mh" : "eng64_mh", mh.size() * sizeof(int2)));
-  dmb_ = static_cast<int2*>(scratch(c_, kMixed ? "eng32_mb" : "eng64_mb", mb.size() * sizeof(int2)));
-  if (!dsys_ || !dslot_ || !dmh_ || !dmb_) return SFH_E_NOMEM;
+  const char* tag = kMixed ? "eng32_" : "eng64_";
+  auto key = [&](const char* what) { return std::string(tag) + what; };
+  dsys_ = static_cast<BSys<T>*>(scratch(c_, key("sys"), hs.size() * sizeof(BSys<T>)));
+  dslot_ = static_cast<BSlot<T>*>(scratch(c_, key("slot"), hl.size() * sizeof(BSlot<T>)));
+  dmh_ = static_cast<int2*>(scratch(c_, key("mh"), mh.size() * sizeof(int2)));
+  dmb_ = static_cast<int2*>(scratch(c_, key("mb"), mb.size() * sizeof(int2)));
+  dids_ = static_cast<int*>(scratch(c_, key("ids"), mapped_.size() * sizeof(int)));
+  dstates_ = static_cast<CGState*>(scratch(c_, key("states"), mapped_.size() * sizeof(CGState)));
+  if (!dsys_ || !dslot_ || !dmh_ || !dmb_ || !dids_ || !dstates_) return SFH_E_NOMEM;
+  if (mapped_.size() > hstates_cap_) {
+    if (hstates_) hipHostFree(hstates_);
+    hstates_ = nullptr;
+    hstates_cap_ = 0;
+    if (hipHostMalloc(reinterpret_cast<void**>(&hstates_), slots_.size() * sizeof(CGState)) != hipSuccess)
+      return set_err(c_, SFH_E_NOMEM, "engine state staging");
+    hstates_cap_ = slots_.size();
+  }
   hipMemcpyAsync(dsys_, hs.data(), hs.size() * sizeof(BSys<T>), hipMemcpyHostToDevice, c_->stream);
   hipMemcpyAsync(dslot_, hl.data(), hl.size() * sizeof(BSlot<T>), hipMemcpyHostToDevice, c_->stream);
   hipMemcpyAsync(dmh_, mh.data(), mh.size() * sizeof(int2), hipMemcpyHostToDevice, c_->stream);
   hipMemcpyAsync(dmb_, mb.data(), mb.size() * sizeof(int2), hipMemcpyHostToDevice, c_->stream);
+  hipMemcpyAsync(dids_, mapped_.data(), mapped_.size() * sizeof(int), hipMemcpyHostToDevice, c_->stream);
   if (int e = hip_check(c_, hipStreamSynchronize(c_->stream), "engine tables")) return e;
   if (!use_graph_) return SFH_OK;
   if (hipStreamBeginCapture(c_->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
     return set_err(c_, SFH_E_HIP, "engine graph capture");
   launch_chunk();
+  launch_tail();
   if (hipStreamEndCapture(c_->stream, &graph_) != hipSuccess) return set_err(c_, SFH_E_HIP, "engine graph end");
   if (hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0) != hipSuccess)
     return set_err(c_, SFH_E_HIP, "engine graph instantiate");
@@ -372,6 +553,11 @@ int Engine<T>::active() const {
   return n;
 }
 
+// One chunk for every mapped solve, its reliable updates and the state copy;
+// then completions on the host.  Finished solves stay in the launch tables
+// (their kernels return at once) until a quarter of the tables has finished
+// or a solve is admitted, so the captured chunk is replayed, not rebuilt,
+// as solves complete.
 template <class T>
 int Engine<T>::step() {
   if (dirty_)
@@ -383,60 +569,28 @@ int Engine<T>::step() {
       if (hipGraphLaunch(gexec_, c_->stream) != hipSuccess) return set_err(c_, SFH_E_HIP, "engine graph launch");
     } else {
       launch_chunk();
+      launch_tail();
     }
   }
   ++chunks_;
-  std::vector<CGState> hst(slots_.size());
-  for (size_t k = 0; k < slots_.size(); ++k)
-    if (slots_[k].used && !slots_[k].finished)
-      hipMemcpyAsync(&hst[k], slots_[k].dev.st, sizeof(CGState), hipMemcpyDeviceToHost, c_->stream);
   if (int e = hip_check(c_, hipStreamSynchronize(c_->stream), "engine chunk")) return e;
-  for (size_t k = 0; k < slots_.size(); ++k) {
+  for (size_t j = 0; j < mapped_.size(); ++j) {
+    const int k = mapped_[j];
     SlotH& h = slots_[k];
-    if (!h.used || h.finished || !hst[k].done) continue;
-    CGState& z = hst[k];
+    const CGState& z = hstates_[j];
+    if (!h.used || h.finished || !z.done) continue;
+    // stopped: converged on the true residual (mixed), met the iterated
+    // target (FP64), or an error status
     h.last = z;
-    if (z.status != SFH_OK || !kMixed) {
-      h.finished = true;
-      h.status = z.status;
-      h.rsq_true = z.rsq;
-      done_.push_back((int)k);
-      dirty_ = true;
-      continue;
-    }
-    // FP64 reliable update
-    const SysH& s = sys_[h.sys];
-    const size_t n = 3 * (size_t)s.gauge->g.V;
-    acc_f32_into_f64(c_->stream, h.xh, reinterpret_cast<float2*>(h.dev.x), n);
-    double2* ax = static_cast<double2*>(scratch(c_, "engine_ax", n * sizeof(double2)));
-    if (!ax) return SFH_E_NOMEM;
-    if (int e = apply_ddag_dev(c_, s.gauge, s.mass, h.xh, ax)) return e;
-    sub_f64(c_->stream, h.r64, h.b, ax, n);
-    const double rt = dev_dot(c_, h.r64, h.r64, n);
-    h.rsq_true = rt;
-    h.rupd++;
-    if (rt < z.target || z.iter >= max_iter_) {
-      h.finished = true;
-      h.status = rt < z.target ? SFH_OK : SFH_E_NOCONV;
-      h.last.rsq = rt;
-      if (h.status) h.last.status = h.status;
-      done_.push_back((int)k);
-      dirty_ = true;
-      continue;
-    }
-    f64_to_f32(c_->stream, h.r64, reinterpret_cast<float2*>(h.dev.r), n);
-    CGState ns = z;
-    ns.beta = rt / z.rsq_prev;
-    ns.rsq = rt;
-    ns.rmax = std::sqrt(rt);
-    ns.done = 0;
-    ns.pause_reason = 0;
-    ns.ru_pending = 0;
-    hipMemcpyAsync(h.dev.st, &ns, sizeof(CGState), hipMemcpyHostToDevice, c_->stream);
-    cg_p_f32(c_->stream, s.gauge->g, reinterpret_cast<float2*>(h.dev.p), reinterpret_cast<const float2*>(h.dev.r),
-             h.dev.st);
-    if (int e = hip_check(c_, hipStreamSynchronize(c_->stream), "engine reliable update")) return e;
+    h.finished = true;
+    h.rupd = z.rupd;
+    h.status = z.status;
+    h.rsq_true = kMixed ? z.rsq_true : z.rsq;
+    if (kMixed && z.status == SFH_OK && !z.converged) h.status = h.last.status = SFH_E_NOCONV;
+    done_.push_back(k);
+    ++finished_mapped_;
   }
+  if (4 * finished_mapped_ >= (int)mapped_.size()) dirty_ = true;
   return SFH_OK;
 }
 
@@ -461,7 +615,6 @@ std::vector<typename Engine<T>::Done> Engine<T>::take_done() {
     out.push_back(d);
     h.used = false;
   }
-  if (!done_.empty()) dirty_ = true;
   done_.clear();
   return out;
 }
