@@ -124,6 +124,9 @@ def bench_batch(args, dev):
     nrhs = sum(len(s["rf"]) for s in systems)
     spec = [(s["g"], s["mass"], s["rf"], s["xf"]) for s in systems]
     dev.cg_batch(ctx, spec[:2], tol=1e-10, precision="f32")  # warm-up
+    # autotuner: launch shape of the batched hop for this batch (untimed)
+    shape_threads, shape_tile, shape_us = dev.tune_batch(ctx, spec, tol=1e-10, precision="f32")
+    log(f"[bench] batch: tuned hop shape {shape_threads} sites/block, {shape_tile} RHS/block ({shape_us:.1f} us)")
     ctx.synchronize()
     ctx.set_profiling(1)
     t0 = time.perf_counter()
@@ -162,7 +165,9 @@ def bench_batch(args, dev):
         "dtype": "f32 inner / f64 reliable updates and reductions", "data": "synthetic (seeded Philox)",
         "config": {"workload": "64 systems (16 each of 8^4, 12^4, 16^4, 24^4), m~U[0.01,0.1], eps in "
                                "{0.2,0.3,0.4}, 1-8 RHS each, batched CG to true residual 1e-10, hipGraph chunks",
-                   "rhs_total": nrhs, "parallelism": "single GPU"},
+                   "rhs_total": nrhs, "parallelism": "single GPU",
+                   "tuned_hop_shape": {"sites_per_block": shape_threads, "rhs_per_block": shape_tile,
+                                       "hop_us": round(shape_us, 2)}},
         "sequential_fp64_solves_per_s": round(nrhs / seq["f64"], 3),
         "sequential_mixed_solves_per_s": round(nrhs / seq["f32"], 3),
         "speedup_vs_sequential_fp64": round(seq["f64"] / makespan, 3),

@@ -225,6 +225,17 @@ typedef struct {
 } sfh_batch_system;
 int sfh_cg_batch(sfh_ctx* ctx, int nsys, const sfh_batch_system* systems, const sfh_cg_params* p,
                  sfh_cg_stats* stats);
+/* Autotuner of the batched engine: times the batched hop on this batch for
+ * every launch shape (128, 256 or 512 sites per block; 1, 2, 4 or 8
+ * right-hand sides per block) and records the fastest in the context for
+ * later sfh_cg_batch calls on batches of the same shape (lattices and
+ * right-hand-side counts, in order, and precision).  If *hop_threads and
+ * *rhs_tile are nonzero on input, that shape is recorded instead (and timed).
+ * The shape changes no solve's arithmetic: results are bit-identical tuned
+ * or not.  The x fields are used as work space (left zeroed).  No
+ * counterpart in the reference. */
+int sfh_tune_batch(sfh_ctx* ctx, int nsys, const sfh_batch_system* systems, const sfh_cg_params* p,
+                   int* hop_threads, int* rhs_tile, double* us);
 
 /* ---------------------------------------------------------------- solve service
  * Continuous-batching CG service for concurrent HMC chains (mixed precision:

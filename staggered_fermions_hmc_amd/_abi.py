@@ -89,6 +89,8 @@ SIGNATURES = {
     "sfh_cg": (_I, [_P, _P, ctypes.POINTER(CGParams), _P, _P, ctypes.POINTER(CGStats)]),
     "sfh_cg_field": (_I, [_P, _P, ctypes.POINTER(CGParams), _P, _P, ctypes.POINTER(CGStats)]),
     "sfh_cg_batch": (_I, [_P, _I, ctypes.POINTER(BatchSystem), ctypes.POINTER(CGParams), ctypes.POINTER(CGStats)]),
+    "sfh_tune_batch": (_I, [_P, _I, ctypes.POINTER(BatchSystem), ctypes.POINTER(CGParams), ctypes.POINTER(_I),
+                            ctypes.POINTER(_I), ctypes.POINTER(_D)]),
     "sfh_service_create": (_I, [_I, _SZ, _D, _I, ctypes.POINTER(_P)]),
     "sfh_service_add_chain": (_I, [_P, _P, _D, _I, ctypes.POINTER(_I)]),
     "sfh_service_submit": (_I, [_P, _I, _P, _P, ctypes.POINTER(ctypes.c_longlong)]),

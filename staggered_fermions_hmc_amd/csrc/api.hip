@@ -659,13 +659,14 @@ int sfh_cg_field(sfh_ctx* c, const sfh_gauge* g, const sfh_cg_params* p, const s
   return cg_solve(c, const_cast<sfh_gauge*>(g), p, phi->d, x->d, stats);
 }
 
-int sfh_cg_batch(sfh_ctx* c, int nsys, const sfh_batch_system* systems, const sfh_cg_params* p,
-                 sfh_cg_stats* stats) {
-  if (!c || nsys < 1 || !systems || !cg_params_ok(p)) return SFH_E_INVALID;
-  Guard gd(c);
-  std::vector<BatchSys> bs(nsys);
-  std::vector<std::vector<const double2*>> rhs(nsys);
-  std::vector<std::vector<double2*>> xs(nsys);
+namespace {
+
+// Validates the caller's batch and lowers it to device pointers.
+int lower_batch(sfh_ctx* c, int nsys, const sfh_batch_system* systems, std::vector<BatchSys>& bs,
+                std::vector<std::vector<const double2*>>& rhs, std::vector<std::vector<double2*>>& xs) {
+  bs.resize(nsys);
+  rhs.assign(nsys, {});
+  xs.assign(nsys, {});
   for (int k = 0; k < nsys; ++k) {
     const sfh_batch_system& s = systems[k];
     if (!s.gauge || s.nrhs < 1 || s.nrhs > 8 || !s.rhs || !s.x || !std::isfinite(s.mass))
@@ -679,7 +680,31 @@ int sfh_cg_batch(sfh_ctx* c, int nsys, const sfh_batch_system* systems, const sf
     }
     bs[k] = BatchSys{s.gauge, s.mass, s.nrhs, rhs[k].data(), xs[k].data()};
   }
+  return SFH_OK;
+}
+
+}  // namespace
+
+int sfh_cg_batch(sfh_ctx* c, int nsys, const sfh_batch_system* systems, const sfh_cg_params* p,
+                 sfh_cg_stats* stats) {
+  if (!c || nsys < 1 || !systems || !cg_params_ok(p)) return SFH_E_INVALID;
+  Guard gd(c);
+  std::vector<BatchSys> bs;
+  std::vector<std::vector<const double2*>> rhs;
+  std::vector<std::vector<double2*>> xs;
+  if (int e = lower_batch(c, nsys, systems, bs, rhs, xs)) return e;
   return cg_batch(c, nsys, bs.data(), p, stats);
+}
+
+int sfh_tune_batch(sfh_ctx* c, int nsys, const sfh_batch_system* systems, const sfh_cg_params* p, int* hop_threads,
+                   int* rhs_tile, double* us) {
+  if (!c || nsys < 1 || !systems || !cg_params_ok(p)) return SFH_E_INVALID;
+  Guard gd(c);
+  std::vector<BatchSys> bs;
+  std::vector<std::vector<const double2*>> rhs;
+  std::vector<std::vector<double2*>> xs;
+  if (int e = lower_batch(c, nsys, systems, bs, rhs, xs)) return e;
+  return tune_batch(c, nsys, bs.data(), p, hop_threads, rhs_tile, us);
 }
 
 // ---------------------------------------------------------------- forces, MD

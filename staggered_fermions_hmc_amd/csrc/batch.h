@@ -12,6 +12,18 @@ namespace sfh {
 
 constexpr int kMaxRhs = 8;  // right-hand sides in flight per system
 
+// Launch shape of the batched hop, chosen by the autotuner.  hop_threads
+// (128, 256 or 512 lanes, one site each, half per parity) applies to the
+// first hop of an iteration, t = D^dag p, which has no reduction; rhs_tile
+// (1, 2, 4 or 8) is how many right-hand sides of a system one block applies
+// its links to, for both hops.  Neither changes any solve's arithmetic or
+// reduction order: the second hop (with the reduction) always runs 256 lanes
+// per block.
+struct BatchConfig {
+  int hop_threads = 256;
+  int rhs_tile = kMaxRhs;
+};
+
 template <class T> struct BSys {
   Geom g;
   const typename Prec<T>::L* U;
@@ -72,6 +84,14 @@ template <class T> class Engine {
   int active() const;
   std::vector<Done> take_done();
   size_t bytes() const { return bytes_; }
+  void set_config(const BatchConfig& cfg) {
+    cfg_ = cfg;
+    dirty_ = true;
+  }
+  const BatchConfig& config() const { return cfg_; }
+  // Average microseconds of the first hop over the current tables with
+  // launch shape cfg (n launches between HIP events).  Writes only t.
+  int time_hop(const BatchConfig& cfg, int n, double* us);
   long chunks() const { return chunks_; }
   static size_t slot_bytes(int V);
 
@@ -98,6 +118,7 @@ template <class T> class Engine {
   };
   int rebuild();
   void launch_tail();
+  void hop_map(int threads, int tile, std::vector<int4>& out) const;
 
   sfh_ctx* c_;
   double tol_;
@@ -112,13 +133,16 @@ template <class T> class Engine {
   std::vector<void*> owned_;
   size_t bytes_ = 0;
   bool dirty_ = true;
-  int nbh_ = 0, nbb_ = 0;
+  int nbh0_ = 0, nbh_ = 0, nbb_ = 0;  // blocks of the first hop, the second hop, the vector kernels
   long chunks_ = 0;
+  BatchConfig cfg_;
   BSys<T>* dsys_ = nullptr;
   BSlot<T>* dslot_ = nullptr;
-  int2 *dmh_ = nullptr, *dmb_ = nullptr;
+  int4 *dmh0_ = nullptr, *dmh_ = nullptr;
+  int2* dmb_ = nullptr;
   // slots in the current launch tables, their gathered states (device and
   // pinned host copies), and how many of them have finished
+  std::vector<BSys<T>> hsys_;  // host copy of the system table
   std::vector<int> mapped_;
   int* dids_ = nullptr;
   CGState* dstates_ = nullptr;

@@ -70,10 +70,10 @@ __device__ __forceinline__ void site_hop(const LT* __restrict__ U, const typenam
 // come from HBM once and from cache for the other right-hand sides.
 // phase DOT=false: t = D^dag p ; DOT=true: v = D t and <p, v>.
 template <class T, bool DOT>
-__global__ __launch_bounds__(256) void kb_hop(const BSys<T>* __restrict__ sys, const BSlot<T>* __restrict__ slots,
-                                              const int2* __restrict__ bmap, typename Prec<T>::R hb) {
+__global__ __launch_bounds__(512) void kb_hop(const BSys<T>* __restrict__ sys, const BSlot<T>* __restrict__ slots,
+                                              const int4* __restrict__ bmap, typename Prec<T>::R hb) {
   using R = typename Prec<T>::R;
-  const int2 m = bmap[blockIdx.x];
+  const int4 m = bmap[blockIdx.x];  // system, block, right-hand sides [z, w) of the system
   const BSys<T>& S = sys[m.x];  // read in place: a private copy with a dynamically indexed slot[] spills
   const int lb = m.y;
   const Geom& g = S.g;
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256) void kb_hop(const BSys<T>* __restrict__ sys, c
     for (int mu = 0; mu < 4; ++mu) neighbours(g, cb, n, mu, nb.fw[mu], nb.bw[mu], nb.sf[mu], nb.sb[mu]);
   }
   __shared__ int sdone;
-  for (int j = 0; j < S.nslot; ++j) {
+  for (int j = m.z; j < m.w; ++j) {
     const BSlot<T> L = slots[S.slot[j]];
     if (threadIdx.x == 0) sdone = L.st->done;
     __syncthreads();
@@ -206,11 +206,11 @@ __global__ __launch_bounds__(kBlas) void kb_ru_acc(const BSlot<T>* __restrict__ 
 
 template <class T, int PHASE>
 __global__ __launch_bounds__(256) void kb_ru_hop(const BSys<T>* __restrict__ sys, const BSlot<T>* __restrict__ slots,
-                                                 const int2* __restrict__ bmap) {
-  const int2 m = bmap[blockIdx.x];
+                                                 const int4* __restrict__ bmap) {
+  const int4 m = bmap[blockIdx.x];
   const BSys<T>& S = sys[m.x];  // read in place: a private copy with a dynamically indexed slot[] spills
   int any = 0;
-  for (int j = 0; j < S.nslot; ++j) any |= slots[S.slot[j]].st->ru_pending;
+  for (int j = m.z; j < m.w; ++j) any |= slots[S.slot[j]].st->ru_pending;
   if (!any) return;  // uniform over the block
   const int lb = m.y;
   const Geom& g = S.g;
@@ -229,7 +229,7 @@ __global__ __launch_bounds__(256) void kb_ru_hop(const BSys<T>* __restrict__ sys
     for (int mu = 0; mu < 4; ++mu) neighbours(g, cb, n, mu, nb.fw[mu], nb.bw[mu], nb.sf[mu], nb.sb[mu]);
   }
   const double hb = PHASE == 0 ? -0.5 : 0.5;
-  for (int j = 0; j < S.nslot; ++j) {
+  for (int j = m.z; j < m.w; ++j) {
     const BSlot<T> L = slots[S.slot[j]];
     if (!valid || !L.st->ru_pending) continue;  // no barrier in this loop
     const double2* in = PHASE == 0 ? L.x64 : L.t64;
@@ -315,7 +315,8 @@ Engine<T>::Engine(sfh_ctx* c, double tol, int max_iter, double delta, int chunk,
 template <class T>
 void Engine<T>::launch_chunk() {
   for (int it = 0; it < chunk_; ++it) {
-    hipLaunchKernelGGL((kb_hop<T, false>), dim3(nbh_), dim3(256), 0, c_->stream, dsys_, dslot_, dmh_, (R)(-0.5));
+    hipLaunchKernelGGL((kb_hop<T, false>), dim3(nbh0_), dim3(cfg_.hop_threads), 0, c_->stream, dsys_, dslot_, dmh0_,
+                       (R)(-0.5));
     hipLaunchKernelGGL((kb_hop<T, true>), dim3(nbh_), dim3(256), 0, c_->stream, dsys_, dslot_, dmh_, (R)(0.5));
     hipLaunchKernelGGL((kb_xr<T>), dim3(nbb_), dim3(kBlas), 0, c_->stream, dslot_, dmb_);
     hipLaunchKernelGGL((kb_p<T>), dim3(nbb_), dim3(kBlas), 0, c_->stream, dslot_, dmb_);
@@ -473,13 +474,31 @@ int Engine<T>::admit(int sys, const double2* b, double2* x, long long ticket) {
   return id;
 }
 
+// Block map of one hop: (system, block, right-hand-side range) entries, one
+// lane per site at `threads` lanes per block, right-hand sides in tiles of
+// `tile`; the tiles of one block index are adjacent, so a block's links are
+// still in L2 for the next tile.
+template <class T>
+void Engine<T>::hop_map(int threads, int tile, std::vector<int4>& out) const {
+  out.clear();
+  const int hw = threads / 2;
+  for (size_t s = 0; s < hsys_.size(); ++s) {
+    const BSys<T>& d = hsys_[s];
+    if (d.nslot == 0) continue;
+    const int nb = (d.g.Vh + hw - 1) / hw;
+    for (int b = 0; b < nb; ++b)
+      for (int j0 = 0; j0 < d.nslot; j0 += tile) out.push_back(make_int4((int)s, b, j0, std::min(j0 + tile, d.nslot)));
+  }
+}
+
 template <class T>
 int Engine<T>::rebuild() {
-  std::vector<BSys<T>> hs;
-  std::vector<int2> mh, mb;
+  std::vector<int4> mh0, mh;
+  std::vector<int2> mb;
   std::vector<BSlot<T>> hl(slots_.size());
   for (size_t k = 0; k < slots_.size(); ++k) hl[k] = slots_[k].dev;
   mapped_.clear();
+  hsys_.clear();
   for (size_t s = 0; s < sys_.size(); ++s) {
     BSys<T> d{};
     d.g = sys_[s].gauge->g;
@@ -491,16 +510,17 @@ int Engine<T>::rebuild() {
     d.nslot = 0;
     for (int k : sys_[s].pool)
       if (slots_[k].used && !slots_[k].finished) d.slot[d.nslot++] = k;
-    const int sidx = (int)hs.size();
-    hs.push_back(d);
-    if (d.nslot == 0) continue;
-    for (int b = 0; b < d.nb_hop; ++b) mh.push_back(make_int2(sidx, b));
+    hsys_.push_back(d);
     for (int j = 0; j < d.nslot; ++j) {
       mapped_.push_back(d.slot[j]);
       const int nbb = (d.g.V + kBlas - 1) / kBlas;
       for (int b = 0; b < nbb; ++b) mb.push_back(make_int2(d.slot[j], b));
     }
   }
+  const std::vector<BSys<T>>& hs = hsys_;
+  hop_map(cfg_.hop_threads, cfg_.rhs_tile, mh0);
+  hop_map(256, cfg_.rhs_tile, mh);
+  nbh0_ = (int)mh0.size();
   nbh_ = (int)mh.size();
   nbb_ = (int)mb.size();
   finished_mapped_ = 0;
@@ -516,11 +536,12 @@ int Engine<T>::rebuild() {
   auto key = [&](const char* what) { return std::string(tag) + what; };
   dsys_ = static_cast<BSys<T>*>(scratch(c_, key("sys"), hs.size() * sizeof(BSys<T>)));
   dslot_ = static_cast<BSlot<T>*>(scratch(c_, key("slot"), hl.size() * sizeof(BSlot<T>)));
-  dmh_ = static_cast<int2*>(scratch(c_, key("mh"), mh.size() * sizeof(int2)));
+  dmh0_ = static_cast<int4*>(scratch(c_, key("mh0"), mh0.size() * sizeof(int4)));
+  dmh_ = static_cast<int4*>(scratch(c_, key("mh"), mh.size() * sizeof(int4)));
   dmb_ = static_cast<int2*>(scratch(c_, key("mb"), mb.size() * sizeof(int2)));
   dids_ = static_cast<int*>(scratch(c_, key("ids"), mapped_.size() * sizeof(int)));
   dstates_ = static_cast<CGState*>(scratch(c_, key("states"), mapped_.size() * sizeof(CGState)));
-  if (!dsys_ || !dslot_ || !dmh_ || !dmb_ || !dids_ || !dstates_) return SFH_E_NOMEM;
+  if (!dsys_ || !dslot_ || !dmh0_ || !dmh_ || !dmb_ || !dids_ || !dstates_) return SFH_E_NOMEM;
   if (mapped_.size() > hstates_cap_) {
     if (hstates_) hipHostFree(hstates_);
     hstates_ = nullptr;
@@ -531,7 +552,8 @@ int Engine<T>::rebuild() {
   }
   hipMemcpyAsync(dsys_, hs.data(), hs.size() * sizeof(BSys<T>), hipMemcpyHostToDevice, c_->stream);
   hipMemcpyAsync(dslot_, hl.data(), hl.size() * sizeof(BSlot<T>), hipMemcpyHostToDevice, c_->stream);
-  hipMemcpyAsync(dmh_, mh.data(), mh.size() * sizeof(int2), hipMemcpyHostToDevice, c_->stream);
+  hipMemcpyAsync(dmh0_, mh0.data(), mh0.size() * sizeof(int4), hipMemcpyHostToDevice, c_->stream);
+  hipMemcpyAsync(dmh_, mh.data(), mh.size() * sizeof(int4), hipMemcpyHostToDevice, c_->stream);
   hipMemcpyAsync(dmb_, mb.data(), mb.size() * sizeof(int2), hipMemcpyHostToDevice, c_->stream);
   hipMemcpyAsync(dids_, mapped_.data(), mapped_.size() * sizeof(int), hipMemcpyHostToDevice, c_->stream);
   if (int e = hip_check(c_, hipStreamSynchronize(c_->stream), "engine tables")) return e;
@@ -543,6 +565,34 @@ int Engine<T>::rebuild() {
   if (hipStreamEndCapture(c_->stream, &graph_) != hipSuccess) return set_err(c_, SFH_E_HIP, "engine graph end");
   if (hipGraphInstantiate(&gexec_, graph_, nullptr, nullptr, 0) != hipSuccess)
     return set_err(c_, SFH_E_HIP, "engine graph instantiate");
+  return SFH_OK;
+}
+
+template <class T>
+int Engine<T>::time_hop(const BatchConfig& cfg, int n, double* us) {
+  if (dirty_)
+    if (int e = rebuild()) return e;
+  if (!nbh_) {
+    *us = 0.0;
+    return SFH_OK;
+  }
+  std::vector<int4> m;
+  hop_map(cfg.hop_threads, cfg.rhs_tile, m);
+  int4* dm = static_cast<int4*>(scratch(c_, kMixed ? "eng32_tune" : "eng64_tune", m.size() * sizeof(int4)));
+  if (!dm) return SFH_E_NOMEM;
+  hipMemcpyAsync(dm, m.data(), m.size() * sizeof(int4), hipMemcpyHostToDevice, c_->stream);
+  auto launch = [&]() {
+    hipLaunchKernelGGL((kb_hop<T, false>), dim3((int)m.size()), dim3(cfg.hop_threads), 0, c_->stream, dsys_, dslot_, dm,
+                       (R)(-0.5));
+  };
+  launch();
+  hipEventRecord(c_->ev0, c_->stream);
+  for (int k = 0; k < n; ++k) launch();
+  hipEventRecord(c_->ev1, c_->stream);
+  if (int e = hip_check(c_, hipEventSynchronize(c_->ev1), "batch hop timing")) return e;
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, c_->ev0, c_->ev1);
+  *us = 1e3 * ms / n;
   return SFH_OK;
 }
 
@@ -622,8 +672,76 @@ std::vector<typename Engine<T>::Done> Engine<T>::take_done() {
 template class Engine<float>;
 template class Engine<double>;
 
+namespace {
+
+// Batch shape: precision, then lattice extents and right-hand-side count of
+// every system, in order.
+std::string batch_shape(int nsys, const BatchSys* sys, int precision) {
+  std::string k = std::to_string(precision);
+  for (int j = 0; j < nsys; ++j) {
+    const Geom& g = sys[j].gauge->g;
+    k += "|" + std::to_string(g.L[0]) + "," + std::to_string(g.L[1]) + "," + std::to_string(g.L[2]) + "," +
+         std::to_string(g.L[3]) + "x" + std::to_string(sys[j].nrhs);
+  }
+  return k;
+}
+
+}  // namespace
+
+int tune_batch(sfh_ctx* c, int nsys, const BatchSys* sys, const sfh_cg_params* p, int* hop_threads, int* rhs_tile,
+               double* us) {
+  auto tune = [&](auto& eng) -> int {
+    long long t = 0;
+    for (int k = 0; k < nsys; ++k) {
+      const int id = eng.add_system(sys[k].gauge, sys[k].mass, sys[k].nrhs);
+      if (id < 0) return -id;
+      for (int j = 0; j < sys[k].nrhs; ++j, ++t)
+        if (eng.admit(id, sys[k].rhs[j], sys[k].x[j], t) < 0) return SFH_E_HIP;
+    }
+    BatchConfig best;
+    double best_us = 1e30;
+    // a shape given on input is recorded as is (timed once for the caller)
+    const bool fixed = hop_threads && rhs_tile && *hop_threads > 0 && *rhs_tile > 0;
+    for (int threads : {128, 256, 512})
+      for (int tile : {1, 2, 4, 8}) {
+        if (fixed && (threads != *hop_threads || tile != *rhs_tile)) continue;
+        BatchConfig cfg;
+        cfg.hop_threads = threads;
+        cfg.rhs_tile = tile;
+        double u = 0.0;
+        if (int e = eng.time_hop(cfg, 5, &u)) return e;
+        if (u < best_us) {
+          best_us = u;
+          best = cfg;
+        }
+      }
+    if (best_us >= 1e30) return set_err(c, SFH_E_INVALID, "batch shape: 128, 256 or 512 sites; 1, 2, 4 or 8 RHS");
+    c->batch_cfg[batch_shape(nsys, sys, p->precision)] = {best.hop_threads, best.rhs_tile};
+    if (hop_threads) *hop_threads = best.hop_threads;
+    if (rhs_tile) *rhs_tile = best.rhs_tile;
+    if (us) *us = best_us;
+    return SFH_OK;
+  };
+  if (p->precision == SFH_PREC_F32) {
+    Engine<float> eng(c, p->tol, p->max_iter, p->reliable_delta, 8);
+    return tune(eng);
+  }
+  if (p->precision == SFH_PREC_F64) {
+    Engine<double> eng(c, p->tol, p->max_iter, p->reliable_delta, 8);
+    return tune(eng);
+  }
+  return set_err(c, SFH_E_INVALID, "batched CG supports FP64 and FP32-inner precision");
+}
+
 int cg_batch(sfh_ctx* c, int nsys, const BatchSys* sys, const sfh_cg_params* p, sfh_cg_stats* stats) {
+  const auto tuned = c->batch_cfg.find(batch_shape(nsys, sys, p->precision));
   auto run = [&](auto& eng) -> int {
+    if (tuned != c->batch_cfg.end()) {
+      BatchConfig cfg;
+      cfg.hop_threads = tuned->second.first;
+      cfg.rhs_tile = tuned->second.second;
+      eng.set_config(cfg);
+    }
     std::vector<long long> first(nsys);
     long long t = 0;
     for (int k = 0; k < nsys; ++k) {

@@ -82,6 +82,9 @@ struct sfh_ctx {
   int prof_every = 1;   // time one launch in prof_every per kernel name
   std::map<std::string, long> prof_seen;
   std::map<std::string, std::pair<double, long>> ktime;
+  // batched-hop launch shape (threads per block, right-hand sides per block)
+  // chosen by the tuner, per batch shape
+  std::map<std::string, std::pair<int, int>> batch_cfg;
   // event pool for kernel timing: pending (name, start, stop) resolved lazily
   std::vector<hipEvent_t> ev_pool;
   struct Pending { std::string name; hipEvent_t a, b; };
@@ -196,6 +199,10 @@ struct BatchSys {
   double2* const* x;
 };
 int cg_batch(sfh_ctx* c, int nsys, const BatchSys* sys, const sfh_cg_params* p, sfh_cg_stats* stats);
+// Times the batched hop's launch shapes on this batch and records the
+// fastest for batches of the same shape (lattices and right-hand-side counts).
+int tune_batch(sfh_ctx* c, int nsys, const BatchSys* sys, const sfh_cg_params* p, int* hop_threads, int* rhs_tile,
+               double* us);
 
 // host generators (gen.cpp)
 void gen_links(const int* dims, double eps, unsigned long long seed, double* out);

@@ -294,10 +294,7 @@ def load_links(ctx, gauges, path):
     ctx.check(ctx._lib.sfh_gauge_load(ctx.h, arr, len(gauges), str(path).encode()))
 
 
-def cg_batch(ctx, systems, tol=1e-10, max_iter=100000, precision="f32", reliable_delta=0.1):
-    """Solve many systems together.  systems: list of (gauge, mass, [rhs
-    Field, ...], [x Field, ...]).  Returns one stats dict per right-hand side,
-    systems in order."""
+def _batch_array(systems):
     n = len(systems)
     arr = (_abi.BatchSystem * n)()
     keep = []
@@ -313,10 +310,31 @@ def cg_batch(ctx, systems, tol=1e-10, max_iter=100000, precision="f32", reliable
         arr[k].rhs = ctypes.cast(r, ctypes.POINTER(ctypes.c_void_p))
         arr[k].x = ctypes.cast(x, ctypes.POINTER(ctypes.c_void_p))
         total += len(rhs)
+    return arr, keep, total
+
+
+def cg_batch(ctx, systems, tol=1e-10, max_iter=100000, precision="f32", reliable_delta=0.1):
+    """Solve many systems together.  systems: list of (gauge, mass, [rhs
+    Field, ...], [x Field, ...]).  Returns one stats dict per right-hand side,
+    systems in order."""
+    arr, keep, total = _batch_array(systems)
     prm = _abi.CGParams(0.0, tol, max_iter, _abi.PREC[precision], reliable_delta, 0)
     stats = (_abi.CGStats * total)()
-    ctx.check(ctx._lib.sfh_cg_batch(ctx.h, n, arr, ctypes.byref(prm), stats))
+    ctx.check(ctx._lib.sfh_cg_batch(ctx.h, len(systems), arr, ctypes.byref(prm), stats))
     return [s.as_dict() for s in stats]
+
+
+def tune_batch(ctx, systems, tol=1e-10, max_iter=100000, precision="f32", reliable_delta=0.1, shape=None):
+    """Autotune the batched hop's launch shape for this batch (kept in the
+    context for batches of the same shape); the x fields are work space.
+    shape=(sites per block, RHS per block) records that shape instead.
+    Returns (sites per block, right-hand sides per block, microseconds)."""
+    arr, keep, _ = _batch_array(systems)
+    prm = _abi.CGParams(0.0, tol, max_iter, _abi.PREC[precision], reliable_delta, 0)
+    th, tile, us = ctypes.c_int(shape[0] if shape else 0), ctypes.c_int(shape[1] if shape else 0), ctypes.c_double()
+    ctx.check(ctx._lib.sfh_tune_batch(ctx.h, len(systems), arr, ctypes.byref(prm), ctypes.byref(th),
+                                      ctypes.byref(tile), ctypes.byref(us)))
+    return th.value, tile.value, us.value
 
 
 class Service:

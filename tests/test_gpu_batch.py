@@ -108,3 +108,32 @@ def test_batch_zero_source_and_errors(ctx):
     with pytest.raises(RuntimeError, match="failed to converge"):
         dev.cg_batch(ctx, [(systems[0]["g"], 0.1, systems[0]["rf"][:1], systems[0]["xf"][:1])], max_iter=3,
                      precision="f32")
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_batch_launch_shapes_give_identical_results(precision):
+    """The autotuner picks a launch shape for the batched hop; no shape
+    changes any arithmetic, so solutions are bit-identical under every
+    shape (forced here) and under the tuner's own choice."""
+    specs = [((8, 8, 8, 8), 0.04, 3), ((4, 4, 4, 8), 0.07, 2), ((16, 16, 16, 16), 0.1, 1)]
+    ref_ctx = dev.Context(0)
+    systems = make_systems(ref_ctx, specs, seed0=700)
+    run(ref_ctx, systems, precision)
+    ref = [[x.download() for x in s["xf"]] for s in systems]
+    for shape in ((128, 1), (512, 2), (256, 4), None):
+        c = dev.Context(0)
+        systems2 = make_systems(c, specs, seed0=700)
+        spec = [(s["g"], s["mass"], s["rf"], s["xf"]) for s in systems2]
+        threads, tile, us = dev.tune_batch(c, spec, precision=precision, shape=shape)
+        assert threads in (128, 256, 512) and tile in (1, 2, 4, 8) and us > 0
+        if shape:
+            assert (threads, tile) == shape
+        stats = run(c, systems2, precision)
+        assert all(st["status"] == 0 for st in stats)
+        for s, xs in zip(systems2, ref):
+            for xf, x0 in zip(s["xf"], xs):
+                assert np.array_equal(xf.download(), x0), (shape, s["dims"])
+        c.close()
+    with pytest.raises(dev.SfhError):
+        dev.tune_batch(ref_ctx, [(s["g"], s["mass"], s["rf"], s["xf"]) for s in systems], precision=precision,
+                       shape=(300, 3))
