@@ -220,16 +220,18 @@ int sfh_create(int device, sfh_ctx** out) {
   sfh_ctx* c = new sfh_ctx();
   c->device = device;
   bool ok = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess;
-  ok = ok && hipMalloc(&c->d_state, 2 * sizeof(CGState)) == hipSuccess;
+  ok = ok && hipMalloc(&c->d_state, 4 * sizeof(CGState)) == hipSuccess;
   ok = ok && hipMalloc(&c->d_partials, 8 * (size_t)kPartialSlot * sizeof(double)) == hipSuccess;
   ok = ok && hipHostMalloc(&c->h_state, 3 * sizeof(CGState), hipHostMallocDefault) == hipSuccess;
   ok = ok && hipHostMalloc(&c->h_scalar, 8 * sizeof(double), hipHostMallocDefault) == hipSuccess;
   ok = ok && hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess;
   ok = ok && hipMalloc(&c->d_ctr, 2 * kCtrBytes) == hipSuccess;
   if (ok) {
-    CGState z[2] = {};
+    CGState z[4] = {};
     z[0].ctr = c->d_ctr;
     z[1].ctr = c->d_ctr + kCtrKinds * kCtrPerKind;
+    z[2].ctr = c->d_ctr;  // second copy of the mixed solve's state (shares state 0's counters)
+    z[3].ctr = z[1].ctr;
     ok = hipMemsetAsync(c->d_ctr, 0, 2 * kCtrBytes, c->stream) == hipSuccess &&
          hipMemcpyAsync(c->d_state, z, sizeof z, hipMemcpyHostToDevice, c->stream) == hipSuccess &&
          hipStreamSynchronize(c->stream) == hipSuccess;

@@ -92,18 +92,74 @@ __device__ __forceinline__ Site site_of(const Geom& g, int nb) {
   return s;
 }
 
-template <class T>
+// Sum of the four partials per workgroup of the previous kernel B, in one
+// fixed order for a 256-lane block: every workgroup that calls it gets the
+// same bits.
+// Split in two so the loads can be issued before the state is known.
+__device__ __forceinline__ void load_partials4(const double* __restrict__ partials, int nb, double (&d)[4]) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j) d[j] = 0.0;
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) d[j] += partials[(size_t)j * nb + k];
+  }
+}
+__device__ __forceinline__ void sum_partials4(const double* __restrict__ partials, int nb, double (&d)[4]) {
+  load_partials4(partials, nb, d);
+  block_sum_n<4>(d);
+}
+
+// DEFER (small grids): the previous B left its dots as plain per-workgroup
+// partials; every workgroup of A sums them and runs the finisher itself (no
+// grid-wide atomics on the iteration's critical path).  A reads the state
+// from st_in and writes the advanced state to st_out, a different copy, so
+// no workgroup reads what another writes; successive iterations alternate
+// the two copies.  Otherwise st_in == st_out and B has run the finisher.
+template <class T, bool DEFER>
 __global__ __launch_bounds__(kBlk) void k_mix_a(Geom g, const typename Prec<T>::L* __restrict__ U, MixBufs<T> b,
-                                                float mass, CGState* st, int nb) {
-  if (st->done) return;
+                                                float mass, const CGState* st_in, CGState* st_out,
+                                                const double* __restrict__ partials, int nb) {
+  int iter;
+  float al, be;
+  if constexpr (DEFER) {
+    __shared__ CGState sst;
+    // the partials are read together with the state (one round trip); they
+    // are used only if the state says a finisher is pending
+    double d[4];
+    load_partials4(partials, nb, d);
+    if (threadIdx.x == 0) sst = *st_in;
+    __syncthreads();
+    if (sst.fin_pending) {
+      block_sum_n<4>(d);
+      if (threadIdx.x == 0) {
+        finish_fused(&sst, d);
+        sst.fin_pending = 0;
+      }
+      __syncthreads();
+    }
+    const bool stop = sst.done;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      CGState o = sst;
+      o.fin_pending = stop ? 0 : 1;  // the B that follows leaves partials
+      *st_out = o;
+    }
+    if (stop) return;  // uniform over the block
+    iter = sst.iter;
+    al = (float)sst.alpha;
+    be = (float)sst.beta;
+  } else {
+    if (st_in->done) return;
+    iter = st_in->iter;
+    al = (float)st_in->alpha;
+    be = (float)st_in->beta;
+  }
   const Site s = site_of(g, nb);
   if (s.cb >= g.Vh) return;
-  const int cur = st->iter & 1, prv = cur ^ 1;
+  const int cur = iter & 1, prv = cur ^ 1;
   const typename Prec<T>::S* __restrict__ rp = b.r[prv];
   const typename Prec<T>::S* __restrict__ pp = b.p[prv];
   const float* __restrict__ rpn = b.rn[prv];
   const float* __restrict__ ppn = b.pn[prv];
-  const float al = (float)st->alpha, be = (float)st->beta;
   const int V = g.V;
   int n[4];
   cb_coords(g, s.cb, s.p, n);
@@ -158,7 +214,7 @@ __global__ __launch_bounds__(kBlk) void k_mix_a(Geom g, const typename Prec<T>::
   store_spinor<T>(b.t, b.tn, V, s.i, res);
 }
 
-template <class T>
+template <class T, bool DEFER>
 __global__ __launch_bounds__(kBlk) void k_mix_b(Geom g, const typename Prec<T>::L* __restrict__ U, MixBufs<T> b,
                                                 float mass, CGState* st, double* partials, int nb) {
   if (st->done) return;
@@ -208,7 +264,31 @@ __global__ __launch_bounds__(kBlk) void k_mix_b(Geom g, const typename Prec<T>::
       d[3] += rr * rr + ri * ri;
     }
   }
-  grid_reduce_n<4>(d, partials, ctr_of(st->ctr, CNT_PV), nb, s.lb, FinFused{st});
+  if constexpr (DEFER) {
+    // per-workgroup partials only: the next kernel A sums them
+    block_sum_n<4>(d);
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) partials[(size_t)j * nb + s.lb] = d[j];
+    }
+  } else {
+    grid_reduce_n<4>(d, partials, ctr_of(st->ctr, CNT_PV), nb, s.lb, FinFused{st});
+  }
+}
+
+// Finisher of the last B of a chunk (DEFER): one workgroup, in place.
+__global__ __launch_bounds__(kBlk) void k_mix_fin(CGState* st, const double* __restrict__ partials, int nb) {
+  __shared__ CGState sst;
+  if (threadIdx.x == 0) sst = *st;
+  __syncthreads();
+  if (!sst.fin_pending) return;  // uniform
+  double d[4];
+  sum_partials4(partials, nb, d);
+  if (threadIdx.x == 0) {
+    finish_fused(&sst, d);
+    sst.fin_pending = 0;
+    *st = sst;
+  }
 }
 
 // Split form, own-site part of kernel A: r_k, p_k and x updated in place
@@ -281,16 +361,28 @@ inline int mix_blocks(const Geom& g) { return (g.Vh + kBlk / 2 - 1) / (kBlk / 2)
 
 }  // namespace
 
+bool mix_defers(const Geom& g) { return mix_blocks(g) <= kDirectSum; }
+
 template <class T>
-void mix_a(hipStream_t s, const Geom& g, const typename Prec<T>::L* U, const MixBufs<T>& b, float mass, CGState* st) {
+void mix_a(hipStream_t s, const Geom& g, const typename Prec<T>::L* U, const MixBufs<T>& b, float mass,
+           const CGState* st_in, CGState* st_out, const double* partials) {
   const int nb = mix_blocks(g);
-  hipLaunchKernelGGL(k_mix_a<T>, dim3(nb), dim3(kBlk), 0, s, g, U, b, mass, st, nb);
+  if (st_in != st_out)
+    hipLaunchKernelGGL((k_mix_a<T, true>), dim3(nb), dim3(kBlk), 0, s, g, U, b, mass, st_in, st_out, partials, nb);
+  else
+    hipLaunchKernelGGL((k_mix_a<T, false>), dim3(nb), dim3(kBlk), 0, s, g, U, b, mass, st_in, st_out, partials, nb);
 }
 template <class T>
 void mix_b(hipStream_t s, const Geom& g, const typename Prec<T>::L* U, const MixBufs<T>& b, float mass, CGState* st,
-           double* partials) {
+           double* partials, bool defer) {
   const int nb = mix_blocks(g);
-  hipLaunchKernelGGL(k_mix_b<T>, dim3(nb), dim3(kBlk), 0, s, g, U, b, mass, st, partials, nb);
+  if (defer)
+    hipLaunchKernelGGL((k_mix_b<T, true>), dim3(nb), dim3(kBlk), 0, s, g, U, b, mass, st, partials, nb);
+  else
+    hipLaunchKernelGGL((k_mix_b<T, false>), dim3(nb), dim3(kBlk), 0, s, g, U, b, mass, st, partials, nb);
+}
+void mix_fin(hipStream_t s, const Geom& g, CGState* st, const double* partials) {
+  hipLaunchKernelGGL(k_mix_fin, dim3(1), dim3(kBlk), 0, s, st, partials, mix_blocks(g));
 }
 template <class T> void mix_upd(hipStream_t s, const Geom& g, const MixBufs<T>& b, CGState* st) {
   hipLaunchKernelGGL(k_mix_upd<T>, dim3((g.V + kBlk - 1) / kBlk), dim3(kBlk), 0, s, g.V, b, st);
@@ -302,10 +394,14 @@ template <class T> void mix_apply(hipStream_t s, const Geom& g, const double2* r
   hipLaunchKernelGGL(k_mix_apply<T>, dim3((g.V + kBlk - 1) / kBlk), dim3(kBlk), 0, s, g.V, r64, b, st);
 }
 
-template void mix_a<float>(hipStream_t, const Geom&, const float2*, const MixBufs<float>&, float, CGState*);
-template void mix_a<Half>(hipStream_t, const Geom&, const __half2*, const MixBufs<Half>&, float, CGState*);
-template void mix_b<float>(hipStream_t, const Geom&, const float2*, const MixBufs<float>&, float, CGState*, double*);
-template void mix_b<Half>(hipStream_t, const Geom&, const __half2*, const MixBufs<Half>&, float, CGState*, double*);
+template void mix_a<float>(hipStream_t, const Geom&, const float2*, const MixBufs<float>&, float, const CGState*,
+                           CGState*, const double*);
+template void mix_a<Half>(hipStream_t, const Geom&, const __half2*, const MixBufs<Half>&, float, const CGState*,
+                          CGState*, const double*);
+template void mix_b<float>(hipStream_t, const Geom&, const float2*, const MixBufs<float>&, float, CGState*, double*,
+                           bool);
+template void mix_b<Half>(hipStream_t, const Geom&, const __half2*, const MixBufs<Half>&, float, CGState*, double*,
+                          bool);
 template void mix_upd<float>(hipStream_t, const Geom&, const MixBufs<float>&, CGState*);
 template void mix_upd<Half>(hipStream_t, const Geom&, const MixBufs<Half>&, CGState*);
 template void mix_acc<float>(hipStream_t, const Geom&, double2*, const MixBufs<float>&, CGState*);

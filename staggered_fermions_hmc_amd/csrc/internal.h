@@ -29,6 +29,7 @@ struct CGState {
   int ru_apply;      // mixed precision: restart the sloppy residual and direction
   int rupd;          // reliable updates done
   int converged;     // mixed precision: the FP64 true residual met the target
+  int fin_pending;   // mixed iteration, small grids: the last B's dots still await their finisher
   unsigned* ctr;     // grid-reduction arrival counters (kCtrKinds x kCtrPerKind, device)
 };
 
@@ -73,6 +74,9 @@ struct sfh_ctx {
   std::map<std::string, std::pair<void*, size_t>> scratch;
   // pinned host staging for scalars
   sfh::CGState* h_state = nullptr;
+  // d_state[0]: solves; [1]: reductions, timings; [2]: second copy of a
+  // mixed solve's state (the small-lattice iteration alternates copies);
+  // [3]: scratch of the mixed-kernel timings
   sfh::CGState* d_state = nullptr;
   double* d_partials = nullptr;  // reduction partials (8 slots of kPartialSlot)
   unsigned* d_ctr = nullptr;     // reduction counters of d_state[0] and d_state[1] (2 x kCtrBytes)

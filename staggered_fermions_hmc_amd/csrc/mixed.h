@@ -19,9 +19,18 @@ template <class T> struct MixBufs {
 
 // Kernel A of iteration k: r_k = r_{k-1} - alpha v_{k-1}, p_k = r_k +
 // beta p_{k-1} and x += alpha p_{k-1} at each site, with r_k and p_k
-// recomputed at the neighbours, then t = D^dag p_k.
+// recomputed at the neighbours, then t = D^dag p_k.  With st_in != st_out
+// (small grids, see mix_defers) it first finishes the previous B's dots from
+// `partials` and writes the advanced state to st_out.
 template <class T>
-void mix_a(hipStream_t s, const Geom& g, const typename Prec<T>::L* U, const MixBufs<T>& b, float mass, CGState* st);
+void mix_a(hipStream_t s, const Geom& g, const typename Prec<T>::L* U, const MixBufs<T>& b, float mass,
+           const CGState* st_in, CGState* st_out, const double* partials);
+// Whether B leaves its dots as per-workgroup partials for the next A (and
+// mix_fin) to finish, instead of a grid reduction with its own finisher:
+// true when the grid is small enough for every workgroup to read them all.
+bool mix_defers(const Geom& g);
+// Finisher of the last B of a chunk when B defers (one workgroup).
+void mix_fin(hipStream_t s, const Geom& g, CGState* st, const double* partials);
 // Split form of kernel A (for lattices whose neighbour vectors do not stay
 // in cache): the own-site update alone, in place; the caller then runs the
 // plain hop t = D^dag p_k.
@@ -31,7 +40,7 @@ constexpr int kMixFusedMaxV = 1 << 18;
 // Kernel B of iteration k: v = D t and the four dots of FinFused.
 template <class T>
 void mix_b(hipStream_t s, const Geom& g, const typename Prec<T>::L* U, const MixBufs<T>& b, float mass, CGState* st,
-           double* partials);
+           double* partials, bool defer);
 // Reliable update, first and last steps (gated on the solve state):
 // x64 += x + alpha p_k and x = 0; then r_{k} = r64 in sloppy storage.
 template <class T> void mix_acc(hipStream_t s, const Geom& g, double2* x64, const MixBufs<T>& b, CGState* st);

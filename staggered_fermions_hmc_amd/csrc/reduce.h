@@ -202,30 +202,32 @@ struct FinRR {
 // <v,v>, <r,r>} of the stored p_k, r_k and v = DDdag p_k.  alpha = |r_k|^2 /
 // <p,v> (breakdown checks of Fermions.cpp:573-586) and |r_{k+1}|^2 =
 // |r_k - alpha v|^2 expanded, from this iteration's dots only.
+__device__ __forceinline__ void finish_fused(CGState* st, const double* s) {
+  const double pv = s[0], rv = s[1], vv = s[2], rr = s[3];
+  st->pv = pv;
+  st->rsq = rr;
+  if (!isfinite(pv) || fabs(pv) < 1e-30) {
+    st->status = SFH_E_BREAKDOWN;
+    st->done = 1;
+    return;
+  }
+  const double a = rr / pv;
+  if (!isfinite(a)) {
+    st->status = SFH_E_NONFINITE_ALPHA;
+    st->done = 1;
+    return;
+  }
+  st->alpha = a;
+  double rn = rr - 2.0 * a * rv + a * a * vv;
+  // cancellation can only take it below zero when r_{k+1} is tiny: the
+  // reliable update that this triggers measures the true residual
+  if (!(rn > 0.0)) rn = 0.0;
+  finish_rr(st, rn);
+}
+
 struct FinFused {
   CGState* st;
-  __device__ void operator()(const double* s) const {
-    const double pv = s[0], rv = s[1], vv = s[2], rr = s[3];
-    st->pv = pv;
-    st->rsq = rr;
-    if (!isfinite(pv) || fabs(pv) < 1e-30) {
-      st->status = SFH_E_BREAKDOWN;
-      st->done = 1;
-      return;
-    }
-    const double a = rr / pv;
-    if (!isfinite(a)) {
-      st->status = SFH_E_NONFINITE_ALPHA;
-      st->done = 1;
-      return;
-    }
-    st->alpha = a;
-    double rn = rr - 2.0 * a * rv + a * a * vv;
-    // cancellation can only take it below zero when r_{k+1} is tiny: the
-    // reliable update that this triggers measures the true residual
-    if (!(rn > 0.0)) rn = 0.0;
-    finish_rr(st, rn);
-  }
+  __device__ void operator()(const double* s) const { finish_fused(st, s); }
 };
 
 // Finisher of the FP64 true residual |phi - DDdag x|^2 of a reliable update

@@ -227,17 +227,21 @@ template <class T> int mix_bufs(sfh_ctx* c, const Geom& g, bool fused, MixBufs<T
 
 // Step A of one mixed iteration: the fused kernel, or the in-place update
 // and the plain hop t = D^dag p_k.
+// st_in / st_out: the state copies the step reads and writes (different
+// only for the fused form on small grids, see mix_defers).
 template <class T>
-void mix_step_a(sfh_ctx* c, const sfh_gauge* gauge, bool fused, const MixBufs<T>& b, float m, CGState* st) {
+void mix_step_a(sfh_ctx* c, const sfh_gauge* gauge, bool fused, const MixBufs<T>& b, float m, const CGState* st_in,
+                CGState* st_out, const double* partials) {
   constexpr bool half = std::is_same<T, Half>::value;
   const Geom& g = gauge->g;
   if (fused) {
     KTimer kt(c, half ? "mix_a_f16" : "mix_a_f32");
     mix_a<T>(c->stream, g, reinterpret_cast<const typename Prec<T>::L*>(half ? (const void*)gauge->lp.u16
                                                                              : (const void*)gauge->lp.u32),
-             b, m, st);
+             b, m, st_in, st_out, partials);
     return;
   }
+  CGState* st = st_out;
   {
     KTimer kt(c, "mix_upd");
     mix_upd<T>(c->stream, g, b, st);
@@ -360,13 +364,19 @@ static int cg_mixed(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* prm, cons
   const float m = (float)prm->mass;
   const typename Prec<T>::L* U = reinterpret_cast<const typename Prec<T>::L*>(HALF ? (const void*)gauge->lp.u16
                                                                                   : (const void*)gauge->lp.u32);
-  auto iter = [&]() {
-    mix_step_a<T>(c, gauge, fused, b, m, st);
+  // small grids: iteration j of a chunk reads state copy j & 1 and writes
+  // copy (j + 1) & 1; the chunk length is even, so the chunk ends in copy 0
+  const bool defer = fused && mix_defers(g);
+  static_assert(kChunkMixed % 2 == 0, "state copies alternate per iteration");
+  CGState* sts[2] = {st, defer ? c->d_state + 2 : st};
+  auto iter = [&](int j) {
+    mix_step_a<T>(c, gauge, fused, b, m, sts[j & 1], sts[(j + 1) & 1], part);
     KTimer kt(c, HALF ? "mix_b_f16" : "mix_b_f32");
-    mix_b<T>(c->stream, g, U, b, m, st, part);
+    mix_b<T>(c->stream, g, U, b, m, sts[(j + 1) & 1], part, defer);
   };
   auto tail = [&]() {
     KTimer kt(c, "reliable_update");
+    if (defer) mix_fin(c->stream, g, st, part);
     mix_acc<T>(c->stream, g, x, b, st);
     hop_f64(c->stream, g, gauge->U, x, t64, prm->mass, -1.0, nullptr, st, nullptr, 1);
     hop_f64(c->stream, g, gauge->U, t64, ax, prm->mass, +1.0, nullptr, st, nullptr, 1);
@@ -374,12 +384,12 @@ static int cg_mixed(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* prm, cons
     mix_apply<T>(c->stream, g, r64, b, st);
   };
   auto direct = [&]() {
-    for (int j = 0; j < kChunkMixed; ++j) iter();
+    for (int j = 0; j < kChunkMixed; ++j) iter(j);
     tail();
   };
   const std::string key = graph_key(HALF ? (fused ? "mix16f" : "mix16s") : (fused ? "mix32f" : "mix32s"), g,
                                     {gauge->U, (const void*)U, x, r64, phi, t64, ax, b.r[0], b.r[1], b.p[0], b.p[1],
-                                     b.v, b.t, b.rn[0], b.rn[1], b.pn[0], b.pn[1], b.vn, b.tn, b.x, st, part},
+                                     b.v, b.t, b.rn[0], b.rn[1], b.pn[0], b.pn[1], b.vn, b.tn, b.x, st, sts[1], part},
                                     prm->mass, get_hop_variant(0, g) * 4 + get_hop_variant(HALF ? 2 : 1, g));
   CGState fin{};
   if (int e = run_chunks(c, [&]() { return launch_chunk(c, key, direct); }, kChunkMixed, s, fin)) return e;
@@ -485,9 +495,12 @@ static int time_mixed_t(sfh_ctx* c, sfh_gauge* gauge, int n, double* us_a, doubl
     *us = 1e3 * ms / n;
     return (int)SFH_OK;
   };
-  if (int e = timed([&]() { mix_step_a<T>(c, gauge, fused, b, m, st); }, us_a)) return e;
-  // B changes alpha and beta; A is not run again
-  return timed([&]() { mix_b<T>(c->stream, g, U, b, m, st, part); }, us_b);
+  // A reads st and (small grids) writes a scratch copy, so every launch
+  // sees the same state; B changes alpha and beta or only leaves partials
+  const bool defer = fused && mix_defers(g);
+  CGState* st_out = defer ? &c->d_state[3] : st;
+  if (int e = timed([&]() { mix_step_a<T>(c, gauge, fused, b, m, st, st_out, part); }, us_a)) return e;
+  return timed([&]() { mix_b<T>(c->stream, g, U, b, m, st, part, defer); }, us_b);
 }
 
 void set_mix_form(int precision, int form) {
