@@ -120,9 +120,11 @@ int sfh_set_profiling(sfh_ctx* ctx, int on);
 int sfh_kernel_time(sfh_ctx* ctx, const char* name, double* total_ms, long* launches);
 
 /* Hop kernel form, per precision (0 f64, 1 f32, 2 f16): 0 = one lane per
- * site, 1 = four lanes per site (one per direction), -1 = choose by lattice
- * size (the default).  Process-wide.  No counterpart in the reference: a
- * tuning knob of this build. */
+ * site with all four directions' loads in flight, 1 = four lanes per site
+ * (one per direction), 2 / 3 = one lane per site stepping through the
+ * directions one / two at a time (fewer registers, more waves), -1 = the
+ * default form.  Process-wide.  No counterpart in the reference: a tuning
+ * knob of this build. */
 int sfh_set_hop_variant(int precision, int variant);
 /* Times each hop form on g's lattice (n launches each, HIP events on the
  * context stream), fixes the faster one for that precision and returns it

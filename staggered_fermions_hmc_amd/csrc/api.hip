@@ -297,7 +297,7 @@ int sfh_kernel_time(sfh_ctx* c, const char* name, double* total_ms, long* launch
 }
 
 int sfh_set_hop_variant(int precision, int variant) {
-  if (precision < 0 || precision > 2 || variant < -1 || variant > 1) return SFH_E_INVALID;
+  if (precision < 0 || precision > 2 || variant < -1 || variant > 3) return SFH_E_INVALID;
   set_hop_variant(precision, variant);
   return SFH_OK;
 }
@@ -317,7 +317,7 @@ int sfh_tune_hop(sfh_ctx* c, const sfh_gauge* g, int precision, int n, int* vari
   hipMemsetAsync(na, 0, gg.V * sizeof(float), c->stream);
   double best_us = 1e30;
   int best = 0;
-  for (int v = 0; v < 2; ++v) {
+  for (int v = 0; v < 4; ++v) {
     set_hop_variant(precision, v);
     auto launch = [&]() {
       if (precision == 0)

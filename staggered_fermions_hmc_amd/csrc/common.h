@@ -40,17 +40,28 @@ inline Geom make_geom(const int* d) {
 // First checkerboard index of the range of `half` sites that logical
 // workgroup lb covers (on each parity).  When a column of fixed (n0, n1)
 // (S1 checkerboard sites) splits into whole ranges, workgroups walk the
-// columns fastest: lb = chunk * L0 L1 + n0 L1 + n1.  Consecutive workgroups
-// of one XCD are then y-neighbours and x-neighbours are L1 workgroups apart,
-// so the backward links and spinors they share are still in that XCD's L2;
-// in plain checkerboard order x-neighbours are L1 L2 L3 / (2 half)
-// workgroups apart and their links come from HBM a second time.  Otherwise
-// ranges follow checkerboard order.  Either way a bijection onto [0, Vh).
+// columns fastest, in tiles of kTileX x kTileY columns (y fastest inside a
+// tile, then y-tiles, then x-tiles), one z-slab of `half` sites per column
+// at a time.  Inside a tile, y-neighbour ranges are adjacent in the walk and
+// x-neighbour ranges kTileY workgroups apart, so the backward links and
+// spinors they share are still in the XCD's L2 (4 MB) even in FP64, where a
+// workgroup touches about 240 KB; in plain checkerboard order x-neighbours
+// are L1 L2 L3 / (2 half) workgroups apart and their links come from HBM a
+// second time.  Extents that do not tile fall back to the plain column walk
+// or to checkerboard order.  Always a bijection onto [0, Vh).
+constexpr int kTileX = 8, kTileY = 4;
 __device__ __forceinline__ int range_start(const Geom& g, int lb, int half) {
   if (g.S1 % half == 0) {
     const int ncol = g.L[0] * g.L[1];
     const int chunk = lb / ncol;
-    return (lb - chunk * ncol) * g.S1 + chunk * half;
+    int col = lb - chunk * ncol;
+    if (g.L[0] % kTileX == 0 && g.L[1] % kTileY == 0) {
+      const int tile = col / (kTileX * kTileY), w = col - tile * (kTileX * kTileY);
+      const int ytiles = g.L[1] / kTileY;
+      const int tx = tile / ytiles, ty = tile - tx * ytiles;
+      col = (tx * kTileX + w / kTileY) * g.L[1] + ty * kTileY + w % kTileY;
+    }
+    return col * g.S1 + chunk * half;
   }
   return lb * half;
 }

@@ -17,7 +17,9 @@
 
 namespace sfh {
 
-template <class T, bool DOT>
+// UNR: directions processed per step of the mu loop (4: all links of the
+// site in flight at once; 1 or 2: fewer registers, more waves per SIMD).
+template <class T, bool DOT, int UNR>
 __global__ __launch_bounds__(256) void k_hop(Geom g, const typename Prec<T>::L* __restrict__ U,
                                              const typename Prec<T>::S* __restrict__ in,
                                              const float* __restrict__ in_norm, typename Prec<T>::S* __restrict__ out,
@@ -43,11 +45,10 @@ __global__ __launch_bounds__(256) void k_hop(Geom g, const typename Prec<T>::L* 
     vec3<R> acc;
 #pragma unroll
     for (int c = 0; c < 3; ++c) acc.c[c] = {R(0), R(0)};
-    int esum = 0;
-#pragma unroll
+#pragma unroll UNR
     for (int mu = 0; mu < 4; ++mu) {
+      const int esum = (mu > 0 ? n[0] : 0) + (mu > 1 ? n[1] : 0) + (mu > 2 ? n[2] : 0);
       const R eta = (esum & 1) ? R(-1) : R(1);
-      esum += n[mu];
       int fw, bw;
       R sf = eta, sb = eta;
       if (mu == 3) {
@@ -59,10 +60,12 @@ __global__ __launch_bounds__(256) void k_hop(Geom g, const typename Prec<T>::L* 
         if (n[3] == g.L[3] - 1) sf = -sf;
         if (n[3] == 0) sb = -sb;
       } else {
+        // selects, not indexing: mu is not a constant when the loop is not unrolled
         const int S = mu == 0 ? g.S0 : (mu == 1 ? g.S1 : g.S2);
-        const int L = g.L[mu];
-        fw = cb + (n[mu] + 1 == L ? -(L - 1) * S : S);
-        bw = cb + (n[mu] == 0 ? (L - 1) * S : -S);
+        const int L = mu == 0 ? g.L[0] : (mu == 1 ? g.L[1] : g.L[2]);
+        const int nm = mu == 0 ? n[0] : (mu == 1 ? n[1] : n[2]);
+        fw = cb + (nm + 1 == L ? -(L - 1) * S : S);
+        bw = cb + (nm == 0 ? (L - 1) * S : -S);
       }
       mat3<R> Uf, Ub;
       vec3<R> vf, vb;
@@ -204,7 +207,9 @@ constexpr int kHopBlock = 256;
 static inline int hop_blocks(const Geom& g) { return (g.Vh + kHopBlock / 2 - 1) / (kHopBlock / 2); }
 static inline int hop4_blocks(const Geom& g) { return (g.Vh + 31) / 32; }
 
-// Variant 0: one lane per site; variant 1: four lanes per site.
+// Variant 0: one lane per site, all four directions unrolled; 1: four lanes
+// per site; 2, 3: one lane per site, the direction loop stepped one or two
+// directions at a time.
 template <class T>
 static void hop_launch(hipStream_t s, int variant, const Geom& g, const typename Prec<T>::L* U,
                        const typename Prec<T>::S* in, const float* in_norm, typename Prec<T>::S* out, float* out_norm,
@@ -220,12 +225,20 @@ static void hop_launch(hipStream_t s, int variant, const Geom& g, const typename
                          dw, dw_norm, st, partials, nb, gate);
   } else {
     const int nb = hop_blocks(g);
-    if (dw)
-      hipLaunchKernelGGL((k_hop<T, true>), dim3(nb), dim3(kHopBlock), 0, s, g, U, in, in_norm, out, out_norm, a, hb,
-                         dw, dw_norm, st, partials, nb, gate);
-    else
-      hipLaunchKernelGGL((k_hop<T, false>), dim3(nb), dim3(kHopBlock), 0, s, g, U, in, in_norm, out, out_norm, a,
-                         hb, dw, dw_norm, st, partials, nb, gate);
+    auto go = [&](auto kernel) {
+      hipLaunchKernelGGL(kernel, dim3(nb), dim3(kHopBlock), 0, s, g, U, in, in_norm, out, out_norm, a, hb, dw, dw_norm,
+                         st, partials, nb, gate);
+    };
+    if (variant == 2) {
+      if (dw) go(k_hop<T, true, 1>);
+      else go(k_hop<T, false, 1>);
+    } else if (variant == 3) {
+      if (dw) go(k_hop<T, true, 2>);
+      else go(k_hop<T, false, 2>);
+    } else {
+      if (dw) go(k_hop<T, true, 4>);
+      else go(k_hop<T, false, 4>);
+    }
   }
 }
 

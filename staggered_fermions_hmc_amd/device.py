@@ -390,7 +390,8 @@ class Service:
 
 
 def set_hop_variant(precision, variant):
-    """0 one lane per site, 1 four lanes per site, -1 by size (default)."""
+    """0 one lane per site (default), 1 four lanes per site, 2 / 3 one lane
+    per site stepping one / two directions at a time, -1 the default."""
     st = _abi.lib().sfh_set_hop_variant(_abi.PREC[precision], int(variant))
     if st:
         raise SfhError(st, "sfh_set_hop_variant")

@@ -62,25 +62,27 @@ def test_ddag_matches_oracle(ctx, dims):
 @pytest.mark.parametrize("dims", [(4, 4, 4, 4), (6, 4, 4, 2), (8, 6, 4, 4), (16, 16, 16, 16)])
 @pytest.mark.parametrize("prec", ["f64", "f32", "f16"])
 def test_hop_forms_agree(ctx, dims, prec):
-    """The one-lane and four-lane hop forms give the same CG solves."""
+    """All hop forms (one lane per site with four, one or two directions in
+    flight; four lanes per site) give the same operator and CG solves."""
     m = 0.1
     U = dev.random_links(dims, 0.3, seed=41)
     phi = dev.random_spinor(dims, seed=42)
     g = dev.Gauge(ctx, dims, U)
-    out = []
+    sol, op = [], []
     try:
-        for v in (0, 1):
+        for v in (0, 1, 2, 3):
             dev.set_hop_variant(prec, v)
-            out.append(g.cg(phi, m, tol=1e-10, precision=prec)[0])
+            sol.append(g.cg(phi, m, tol=1e-10, precision=prec)[0])
             if prec == "f64":
-                out.append(g.apply_d(m, phi))
+                op.append(g.apply_d(m, phi))
     finally:
         dev.set_hop_variant(prec, -1)
-    if prec == "f64":
-        assert rel(out[1], out[3]) < 1e-14  # D: the two forms differ only in summation order
-        assert rel(out[0], out[2]) < 1e-9
-    else:
-        assert rel(out[0], out[1]) < 1e-8
+    for k in (1, 2, 3):
+        if prec == "f64":
+            assert rel(op[k], op[0]) < 1e-14, k  # D: the forms differ at most in summation order
+            assert rel(sol[k], sol[0]) < 1e-9, k
+        else:
+            assert rel(sol[k], sol[0]) < 1e-8, k
 
 
 def test_dslash_delta_source_stencil(ctx):
@@ -333,7 +335,7 @@ def test_tune_hop_picks_a_form(ctx):
     try:
         for prec in ("f64", "f32", "f16"):
             v, us = g.tune_hop(prec, n=5)
-            assert v in (0, 1) and us > 0
+            assert v in (0, 1, 2, 3) and us > 0
     finally:
         for prec in ("f64", "f32", "f16"):
             dev.set_hop_variant(prec, -1)
