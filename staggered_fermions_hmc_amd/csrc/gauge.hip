@@ -12,34 +12,34 @@ namespace sfh {
 using M = mat3<double>;
 using C = cpx<double>;
 
-__device__ __forceinline__ int dindex(const Geom& g, const int n[4]) {
-  const int p = (n[0] + n[1] + n[2] + n[3]) & 1;
-  return p * g.Vh + cb_of(g, n);
-}
 
-struct Pos {
-  int n[4];
-  __device__ __forceinline__ Pos step(const Geom& g, int mu, int d) const {
-    Pos r = *this;
-    int v = r.n[mu] + d;
-    const int L = g.L[mu];
-    if (v < 0) v += L;
-    else if (v >= L) v -= L;
-    r.n[mu] = v;
-    return r;
+// A site as four coordinates in registers.  Moves pick the coordinate with
+// selects: a runtime index into a coordinate array (the direction of a move
+// is not a compile-time constant) would put the array in scratch memory.
+struct Site4 {
+  int x0, x1, x2, x3;
+  __device__ __forceinline__ void move(const Geom& g, int d, int s) {
+    const int L = d == 0 ? g.L[0] : (d == 1 ? g.L[1] : (d == 2 ? g.L[2] : g.L[3]));
+    int v = (d == 0 ? x0 : (d == 1 ? x1 : (d == 2 ? x2 : x3))) + s;
+    v = v < 0 ? v + L : (v >= L ? v - L : v);
+    x0 = d == 0 ? v : x0;
+    x1 = d == 1 ? v : x1;
+    x2 = d == 2 ? v : x2;
+    x3 = d == 3 ? v : x3;
+  }
+  __device__ __forceinline__ int index(const Geom& g) const {
+    const int p = (x0 + x1 + x2 + x3) & 1;
+    return p * g.Vh + ((x0 * g.L[1] + x1) * g.L[2] + x2) * g.L3h + (x3 >> 1);
   }
 };
 
-__device__ __forceinline__ M ldU(const double2* __restrict__ U, const Geom& g, const Pos& x, int mu) {
-  const int i = dindex(g, x.n);
-  M m;
-#pragma unroll
-  for (int e = 0; e < 9; ++e) {
-    const double2 v = U[(size_t)(mu * 9 + e) * g.V + i];
-    m.e[e] = {v.x, v.y};
-  }
-  return m;
+__device__ __forceinline__ Site4 site_of(const Geom& g, int i) {
+  const int p = i >= g.Vh;
+  int n[4];
+  cb_coords(g, i - p * g.Vh, p, n);
+  return Site4{n[0], n[1], n[2], n[3]};
 }
+
 __device__ __forceinline__ M ldL(const double2* __restrict__ U, const Geom& g, int i, int mu) {
   M m;
 #pragma unroll
@@ -54,86 +54,93 @@ __device__ __forceinline__ void stL(double2* __restrict__ U, const Geom& g, int 
   for (int e = 0; e < 9; ++e) U[(size_t)(mu * 9 + e) * g.V + i] = make_double2(m.e[e].re, m.e[e].im);
 }
 
-// Plaquette staple sum A1 (Lattice.cpp:249-313): ReTr(U_mu(n) A1) sums the six
-// plaquettes containing the link.
-__device__ M staple_sum(const double2* __restrict__ U, const Geom& g, const Pos& n, int mu) {
-  M sum = mzero<double>();
-#pragma unroll 1
-  for (int nu = 0; nu < 4; ++nu) {
-    if (nu == mu) continue;
-    const Pos a = n.step(g, nu, -1);
-    const Pos b = a.step(g, mu, +1);
-    const M lo = mmul_dn(ldU(U, g, b, nu), mmul_dn(ldU(U, g, a, mu), ldU(U, g, a, nu)));
-    sum = madd(sum, lo);
-    const Pos c = n.step(g, nu, +1);
-    const Pos d = n.step(g, mu, +1);
-    const M hi = mmul(ldU(U, g, d, nu), mmul_nd(mdag(ldU(U, g, c, mu)), ldU(U, g, n, nu)));
-    sum = madd(sum, hi);
+// t <- t m and t <- t m^dag, one row at a time: only t, m and three entries
+// are live, so the staple kernels keep several waves per SIMD.
+__device__ __forceinline__ void rmul_n(M& t, const M& m) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const C a0 = t.e[3 * i], a1 = t.e[3 * i + 1], a2 = t.e[3 * i + 2];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      C s = a0 * m.e[j];
+      cmac(s, a1, m.e[3 + j]);
+      cmac(s, a2, m.e[6 + j]);
+      t.e[3 * i + j] = s;
+    }
   }
-  return sum;
+}
+__device__ __forceinline__ void rmul_d(M& t, const M& m) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const C a0 = t.e[3 * i], a1 = t.e[3 * i + 1], a2 = t.e[3 * i + 2];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      C s = a0 * conj(m.e[3 * j]);
+      cmac(s, a1, conj(m.e[3 * j + 1]));
+      cmac(s, a2, conj(m.e[3 * j + 2]));
+      t.e[3 * i + j] = s;
+    }
+  }
 }
 
-// Rectangle staple sum A2 (Lattice.cpp:315-499): the six 2x1 / 1x2 loops per
-// orthogonal direction that contain the link.
-__device__ M rect_sum(const double2* __restrict__ U, const Geom& g, const Pos& n, int mu) {
-  M sum = mzero<double>();
+// Closed paths through link U_mu(n), as moves from n + mu back to n: d = 0
+// moves along mu, d = 1 along the plane's other direction nu; a + move from
+// x takes U_d(x), a - move takes U_d(x - d)^dag.  The product of a path is
+// the staple whose ReTr(U_mu(n) staple) is the loop.
+struct Move {
+  int d, s;
+};
+// the two plaquettes of the (mu, nu) plane (Lattice.cpp:249-313): below, above
+__constant__ Move kPlaqPath[2][3] = {{{1, -1}, {0, -1}, {1, +1}}, {{1, +1}, {0, -1}, {1, -1}}};
+// the six rectangles (Lattice.cpp:315-499): 2 x 1 in nu below and above
+// (short side on the link), then 1 x 2 with the link first or second along
+// the long side, below and above
+__constant__ Move kRectPath[6][5] = {
+    {{1, -1}, {1, -1}, {0, -1}, {1, +1}, {1, +1}},
+    {{1, +1}, {1, +1}, {0, -1}, {1, -1}, {1, -1}},
+    {{0, +1}, {1, -1}, {0, -1}, {0, -1}, {1, +1}},
+    {{1, +1}, {0, -1}, {0, -1}, {1, -1}, {0, +1}},
+    {{1, -1}, {0, -1}, {0, -1}, {1, +1}, {0, +1}},
+    {{0, +1}, {1, +1}, {0, -1}, {0, -1}, {1, -1}},
+};
+
+template <int N>
+__device__ __forceinline__ M walk(const double2* __restrict__ U, const Geom& g, Site4 x, int mu, int nu,
+                                  const Move (&path)[N]) {
+  M t;
+#pragma unroll 1
+  for (int k = 0; k < N; ++k) {
+    const int d = path[k].d ? nu : mu;
+    const int s = path[k].s;
+    if (s < 0) x.move(g, d, -1);
+    const M m = ldL(U, g, x.index(g), d);
+    if (s > 0) x.move(g, d, +1);
+    if (k == 0) t = s > 0 ? m : mdag(m);
+    else if (s > 0) rmul_n(t, m);
+    else rmul_d(t, m);
+  }
+  return t;
+}
+
+// c11 A1 + c12 A2 for link (n, mu): A1 the six plaquette staples, A2 the
+// eighteen rectangle staples (Lattice.cpp:249-507).
+__device__ M staples(const double2* __restrict__ U, const Geom& g, const Site4& n, int mu, double c11, double c12) {
+  M A = mzero<double>();
+  Site4 x = n;
+  x.move(g, mu, +1);
 #pragma unroll 1
   for (int nu = 0; nu < 4; ++nu) {
     if (nu == mu) continue;
-    const Pos nP = n.step(g, mu, +1), n2P = nP.step(g, mu, +1), nM = n.step(g, mu, -1);
-    const Pos nU = n.step(g, nu, +1), n2U = nU.step(g, nu, +1);
-    const Pos nD = n.step(g, nu, -1), n2D = nD.step(g, nu, -1);
-    M t;
-    {  // short side, below
-      const Pos a = nP.step(g, nu, -1), b = a.step(g, nu, -1);
-      t = mmul_nd(mdag(ldU(U, g, a, nu)), ldU(U, g, b, nu));
-      t = mmul_nd(t, ldU(U, g, n2D, mu));
-      t = mmul(t, ldU(U, g, n2D, nu));
-      t = mmul(t, ldU(U, g, nD, nu));
-      sum = madd(sum, t);
+    if (c11 != 0.0) {
+#pragma unroll 1
+      for (int k = 0; k < 2; ++k) A = madd(A, mscale(c11, walk(U, g, x, mu, nu, kPlaqPath[k])));
     }
-    {  // short side, above
-      const Pos a = nP.step(g, nu, +1);
-      t = mmul(ldU(U, g, nP, nu), ldU(U, g, a, nu));
-      t = mmul_nd(t, ldU(U, g, n2U, mu));
-      t = mmul_nd(t, ldU(U, g, nU, nu));
-      t = mmul_nd(t, ldU(U, g, n, nu));
-      sum = madd(sum, t);
-    }
-    {  // long side, link first, below
-      const Pos a = n2P.step(g, nu, -1), b = nP.step(g, nu, -1);
-      t = mmul_nd(ldU(U, g, nP, mu), ldU(U, g, a, nu));
-      t = mmul_nd(t, ldU(U, g, b, mu));
-      t = mmul_nd(t, ldU(U, g, nD, mu));
-      t = mmul(t, ldU(U, g, nD, nu));
-      sum = madd(sum, t);
-    }
-    {  // long side, link second, above
-      const Pos a = nM.step(g, nu, +1);
-      t = mmul_nd(ldU(U, g, nP, nu), ldU(U, g, nU, mu));
-      t = mmul_nd(t, ldU(U, g, a, mu));
-      t = mmul_nd(t, ldU(U, g, nM, nu));
-      t = mmul(t, ldU(U, g, nM, mu));
-      sum = madd(sum, t);
-    }
-    {  // long side, link second, below
-      const Pos a = nP.step(g, nu, -1), b = nM.step(g, nu, -1);
-      t = mmul_nd(mdag(ldU(U, g, a, nu)), ldU(U, g, nD, mu));
-      t = mmul_nd(t, ldU(U, g, b, mu));
-      t = mmul(t, ldU(U, g, b, nu));
-      t = mmul(t, ldU(U, g, nM, mu));
-      sum = madd(sum, t);
-    }
-    {  // long side, link first, above
-      const Pos a = nP.step(g, nu, +1);
-      t = mmul(ldU(U, g, nP, mu), ldU(U, g, n2P, nu));
-      t = mmul_nd(t, ldU(U, g, a, mu));
-      t = mmul_nd(t, ldU(U, g, nU, mu));
-      t = mmul_nd(t, ldU(U, g, n, nu));
-      sum = madd(sum, t);
+    if (c12 != 0.0) {
+#pragma unroll 1
+      for (int k = 0; k < 6; ++k) A = madd(A, mscale(c12, walk(U, g, x, mu, nu, kRectPath[k])));
     }
   }
-  return sum;
+  return A;
 }
 
 __device__ __forceinline__ M traceless(const M& f) {
@@ -159,37 +166,34 @@ __device__ __forceinline__ M mi_antiherm(const M& A, double s) {
   return r;
 }
 
-__device__ __forceinline__ void site_of(const Geom& g, int i, Pos& x) {
-  const int p = i >= g.Vh;
-  cb_coords(g, i - p * g.Vh, p, x.n);
-}
-
 // Total force on link (i, mu) and optional P -= coef F (Fermions.cpp:372-384,
 // :443-515, :640-659, :690-733).
 __global__ __launch_bounds__(128) void k_force(Geom g, const double2* __restrict__ U, const double2* __restrict__ X,
                                                const double2* __restrict__ Y, double beta, double c11, double c12,
                                                unsigned which, double2* __restrict__ F, double2* __restrict__ P,
-                                               double coef) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int mu = blockIdx.y;
+                                               double coef, int nb) {
+  // the four directions of one site range run as consecutive workgroups of
+  // one XCD, so the links around those sites are read from HBM once and
+  // from that XCD's L2 for the other directions
+  const int lb = xcd_block(nb);
+  const int mu = lb & 3;
+  const int i = (lb >> 2) * blockDim.x + threadIdx.x;
   if (i >= g.V) return;
-  Pos n;
-  site_of(g, i, n);
+  const Site4 n = site_of(g, i);
   M f = mzero<double>();
-  const M u = ldL(U, g, i, mu);
   if (which & SFH_FORCE_GAUGE) {
-    M A = mscale(c11, staple_sum(U, g, n, mu));
-    if (c12 != 0.0) A = madd(A, mscale(c12, rect_sum(U, g, n, mu)));
-    const M Q = mmul(u, A);
+    const M A = staples(U, g, n, mu, c11, c12);
+    const M Q = mmul(ldL(U, g, i, mu), A);
     f = traceless(mi_antiherm(Q, beta / 12.0));
   }
   if (which & SFH_FORCE_FERMION) {
-    const Pos up = n.step(g, mu, +1);
-    const int j = dindex(g, up.n);
-    int es = 0;
-    for (int k = 0; k < mu; ++k) es += n.n[k];
+    const M u = ldL(U, g, i, mu);
+    Site4 up = n;
+    up.move(g, mu, +1);
+    const int j = up.index(g);
+    const int es = (mu > 0 ? n.x0 : 0) + (mu > 1 ? n.x1 : 0) + (mu > 2 ? n.x2 : 0);
     double c = (es & 1) ? -0.25 : 0.25;
-    if (mu == 3 && n.n[3] == g.L[3] - 1) c = -c;
+    if (mu == 3 && n.x3 == g.L[3] - 1) c = -c;
     C x0[3], x1[3], y0[3], y1[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
@@ -229,8 +233,8 @@ __global__ __launch_bounds__(128) void k_force(Geom g, const double2* __restrict
 int force_field(sfh_ctx* c, const Geom& g, const double2* U, const double2* X, const double2* Y, double beta,
                 double c11, double c12, unsigned which, double2* F, double2* P, double coef) {
   KTimer kt(c, "force");
-  dim3 grid((g.V + 127) / 128, 4);
-  hipLaunchKernelGGL(k_force, grid, dim3(128), 0, c->stream, g, U, X, Y, beta, c11, c12, which, F, P, coef);
+  const int nb = 4 * ((g.V + 127) / 128);
+  hipLaunchKernelGGL(k_force, dim3(nb), dim3(128), 0, c->stream, g, U, X, Y, beta, c11, c12, which, F, P, coef, nb);
   return hip_check(c, hipGetLastError(), "force kernel launch");
 }
 
@@ -240,24 +244,29 @@ __global__ __launch_bounds__(128) void k_link_sum(Geom g, const double2* __restr
                                                   double c12, int kind, double* partials, unsigned* cnt,
                                                   double* out, int nb) {
   double v = 0.0;
-  for (int l = blockIdx.x * blockDim.x + threadIdx.x; l < 4 * g.V; l += nb * blockDim.x) {
-    const int i = l >> 2, mu = l & 3;
+  // blocks of 128 links of one direction; the four directions of a site
+  // range are consecutive blocks of one XCD (shared L2)
+  const int lb = xcd_block(nb);
+  const int nvb = 4 * ((g.V + blockDim.x - 1) / blockDim.x);
+  for (int vb = lb; vb < nvb; vb += nb) {
+    const int mu = vb & 3;
+    const int i = (vb >> 2) * blockDim.x + threadIdx.x;
+    if (i >= g.V) continue;
     const M u = ldL(U, g, i, mu);
     if (kind == 2) {
       v += retr_mul(u, u);
     } else {
-      Pos n;
-      site_of(g, i, n);
-      const double pl = retr_mul(u, staple_sum(U, g, n, mu));
+      const Site4 n = site_of(g, i);
+      const double pl = retr_mul(u, staples(U, g, n, mu, 1.0, 0.0));
       if (kind == 1) {
         v += pl / 3.0;
       } else {
-        const double rc = c12 != 0.0 ? retr_mul(u, rect_sum(U, g, n, mu)) : 0.0;
+        const double rc = c12 != 0.0 ? retr_mul(u, staples(U, g, n, mu, 0.0, 1.0)) : 0.0;
         v += -(beta / 3.0) * (c11 * (pl / 4.0) + c12 * (rc / 6.0));
       }
     }
   }
-  grid_reduce(v, partials, cnt, nb, blockIdx.x, FinStore{out});
+  grid_reduce(v, partials, cnt, nb, lb, FinStore{out});
 }
 
 static double link_sum(sfh_ctx* c, const Geom& g, const double2* U, double beta, double c11, double c12, int kind) {
@@ -287,15 +296,11 @@ __global__ void k_polyakov(Geom g, const double2* __restrict__ U, double2* __res
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int ns = g.L[0] * g.L[1] * g.L[2];
   if (s >= ns) return;
-  Pos x;
-  x.n[2] = s % g.L[2];
-  x.n[1] = (s / g.L[2]) % g.L[1];
-  x.n[0] = s / (g.L[2] * g.L[1]);
-  x.n[3] = 0;
-  M w = ldU(U, g, x, 3);
+  Site4 x{s / (g.L[2] * g.L[1]), (s / g.L[2]) % g.L[1], s % g.L[2], 0};
+  M w = ldL(U, g, x.index(g), 3);
   for (int t = 1; t < g.L[3]; ++t) {
-    x.n[3] = t;
-    w = mmul(w, ldU(U, g, x, 3));
+    x.x3 = t;
+    rmul_n(w, ldL(U, g, x.index(g), 3));
   }
   const C tr = mtrace(w);
   out[s] = make_double2(tr.re, tr.im);
