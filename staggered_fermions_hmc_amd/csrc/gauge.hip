@@ -108,7 +108,7 @@ template <int N>
 __device__ __forceinline__ M walk(const double2* __restrict__ U, const Geom& g, Site4 x, int mu, int nu,
                                   const Move (&path)[N]) {
   M t;
-#pragma unroll 1
+#pragma unroll
   for (int k = 0; k < N; ++k) {
     const int d = path[k].d ? nu : mu;
     const int s = path[k].s;
