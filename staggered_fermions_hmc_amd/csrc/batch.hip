@@ -36,6 +36,7 @@ namespace sfh {
 namespace {
 
 constexpr int kBlas = 256;
+template <class T> constexpr bool kMixedT = std::is_same<T, float>::value;
 
 // Neighbour indices and signs of one site in all four directions.
 template <class R> struct SiteNb {
@@ -101,7 +102,8 @@ __global__ __launch_bounds__(512) void kb_hop(const BSys<T>* __restrict__ sys, c
     if (dn) continue;
     const typename Prec<T>::S* in = DOT ? L.t : L.p;
     typename Prec<T>::S* out = DOT ? L.v : L.t;
-    double dotv = 0.0;
+    // mixed precision: <p,v>, <r,v>, <v,v>, <r,r> for FinFused; FP64: <p,v>
+    double d[4] = {0.0, 0.0, 0.0, 0.0};
     if (valid) {
       vec3<R> acc, self, res;
       site_hop<T>(S.U, in, V, i, q, nb, acc);
@@ -113,14 +115,55 @@ __global__ __launch_bounds__(512) void kb_hop(const BSys<T>* __restrict__ sys, c
       }
       store_spinor<T>(out, nullptr, V, i, res);
       if constexpr (DOT) {
-        vec3<R> w;
+        vec3<R> w, rk;
         load_spinor<T>(L.p, nullptr, V, i, w);
+        if constexpr (kMixedT<T>) load_spinor<T>(L.r, nullptr, V, i, rk);
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-          dotv += (double)w.c[c].re * (double)res.c[c].re + (double)w.c[c].im * (double)res.c[c].im;
+        for (int c = 0; c < 3; ++c) {
+          const double vr = res.c[c].re, vi = res.c[c].im;
+          d[0] += (double)w.c[c].re * vr + (double)w.c[c].im * vi;
+          if constexpr (kMixedT<T>) {
+            const double rr = rk.c[c].re, ri = rk.c[c].im;
+            d[1] += rr * vr + ri * vi;
+            d[2] += vr * vr + vi * vi;
+            d[3] += rr * rr + ri * ri;
+          }
+        }
       }
     }
-    if constexpr (DOT) grid_reduce(dotv, L.part, ctr_of(L.st->ctr, CNT_PV), S.nb_hop, lb, FinPV{L.st});
+    if constexpr (DOT) {
+      if constexpr (kMixedT<T>) grid_reduce_n<4>(d, L.part, ctr_of(L.st->ctr, CNT_PV), S.nb_hop, lb, FinFused{L.st});
+      else grid_reduce(d[0], L.part, ctr_of(L.st->ctr, CNT_PV), S.nb_hop, lb, FinPV{L.st});
+    }
+  }
+}
+
+// Mixed precision, first kernel of an iteration (all updates of the previous
+// one, in place): x += alpha p, r -= alpha v, p = r + beta p.  With alpha =
+// beta = 0 (a fresh solve, p = v = 0) it leaves p = r; after a reliable
+// update (alpha = 0, r the true residual) it forms p = r + beta p.
+template <class T>
+__global__ __launch_bounds__(kBlas) void kb_upd(const BSlot<T>* __restrict__ slots, const int2* __restrict__ bmap) {
+  const int2 m = bmap[blockIdx.x];
+  const BSlot<T> L = slots[m.x];
+  if (L.st->done) return;
+  const int i = m.y * kBlas + threadIdx.x;
+  if (i >= L.V) return;
+  const float al = (float)L.st->alpha, be = (float)L.st->beta;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const size_t k = (size_t)c * L.V + i;
+    float2 xx = L.x[k], rv = L.r[k], pp = L.p[k];
+    const float2 vv = L.v[k];
+    xx.x = __fmaf_rn(al, pp.x, xx.x);
+    xx.y = __fmaf_rn(al, pp.y, xx.y);
+    rv.x = __fmaf_rn(-al, vv.x, rv.x);
+    rv.y = __fmaf_rn(-al, vv.y, rv.y);
+    pp.x = __fmaf_rn(be, pp.x, rv.x);
+    pp.y = __fmaf_rn(be, pp.y, rv.y);
+    L.x[k] = xx;
+    L.r[k] = rv;
+    L.p[k] = pp;
   }
 }
 
@@ -179,11 +222,11 @@ __global__ __launch_bounds__(kBlas) void kb_p(const BSlot<T>* __restrict__ slots
 
 // Reliable update of every paused mixed solve, on the device (gated on each
 // slot's state, so solves that did not pause cost one state read):
-//   kb_ru_acc   x64 += x, x = 0
+//   kb_ru_acc   x64 += x + alpha p, x = 0
 //   kb_ru_hop   t64 = D^dag x64, then ax = D t64 (FP64 links, loaded once per
 //               block for all paused right-hand sides of the system)
 //   kb_ru_sub   r64 = b - ax, |r64|^2 -> FinRU (restart, converged or out of budget)
-//   kb_ru_apply r = r64 in FP32, p = r + beta p
+//   kb_ru_apply r = r64 in FP32 (the next kb_upd restarts p from it)
 template <class T>
 __global__ __launch_bounds__(kBlas) void kb_ru_acc(const BSlot<T>* __restrict__ slots, const int2* __restrict__ bmap) {
   const int2 m = bmap[blockIdx.x];
@@ -192,13 +235,15 @@ __global__ __launch_bounds__(kBlas) void kb_ru_acc(const BSlot<T>* __restrict__ 
   if (!L.st->ru_pending) return;
   const int i = m.y * kBlas + threadIdx.x;
   if (i >= L.V) return;
+  const double al = L.st->alpha;  // the last iteration's step, not yet in x
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const size_t k = (size_t)c * L.V + i;
     const auto xl = L.x[k];
+    const auto pk = L.p[k];
     double2 h = L.x64[k];
-    h.x += (double)xl.x;
-    h.y += (double)xl.y;
+    h.x += (double)xl.x + al * (double)pk.x;
+    h.y += (double)xl.y + al * (double)pk.y;
     L.x64[k] = h;
     L.x[k] = {0, 0};
   }
@@ -274,17 +319,13 @@ __global__ __launch_bounds__(kBlas) void kb_ru_apply(const BSlot<T>* __restrict_
   if (!L.st->ru_apply) return;
   const int i = m.y * kBlas + threadIdx.x;
   if (i >= L.V) return;
-  const float be = (float)L.st->beta;
+  // alpha = 0 and beta = |r_true|^2 / |r_k|^2 in the state: the next kb_upd
+  // forms p = r + beta p from this r
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const size_t k = (size_t)c * L.V + i;
     const double2 t = L.r64[k];
-    const float2 rv = make_float2((float)t.x, (float)t.y);
-    float2 pp = L.p[k];
-    pp.x = rv.x + be * pp.x;
-    pp.y = rv.y + be * pp.y;
-    L.r[k] = rv;
-    L.p[k] = pp;
+    L.r[k] = make_float2((float)t.x, (float)t.y);
   }
 }
 
@@ -305,7 +346,8 @@ size_t Engine<T>::slot_bytes(int V) {
   const size_t n = 3 * (size_t)V;
   const int nbh = (V / 2 + 127) / 128, nbb = (V + kBlas - 1) / kBlas;
   return align256(n * sizeof(S)) * (kMixed ? 5 : 4) + (kMixed ? 3 * align256(n * sizeof(double2)) : 0) +
-         align256((std::max(nbh, nbb) + kRedGroups) * sizeof(double)) + align256(sizeof(CGState)) + align256(kCtrBytes);
+         align256(4 * (std::max(nbh, nbb) + kRedGroups) * sizeof(double)) + align256(sizeof(CGState)) +
+         align256(kCtrBytes);
 }
 
 template <class T>
@@ -314,12 +356,17 @@ Engine<T>::Engine(sfh_ctx* c, double tol, int max_iter, double delta, int chunk,
 
 template <class T>
 void Engine<T>::launch_chunk() {
+  // mixed: update, hop, hop + four dots (three kernels per iteration);
+  // FP64: the reference recurrence, hop, hop + <p,v>, x/r update, p update
   for (int it = 0; it < chunk_; ++it) {
+    if constexpr (kMixed) hipLaunchKernelGGL((kb_upd<T>), dim3(nbb_), dim3(kBlas), 0, c_->stream, dslot_, dmb_);
     hipLaunchKernelGGL((kb_hop<T, false>), dim3(nbh0_), dim3(cfg_.hop_threads), 0, c_->stream, dsys_, dslot_, dmh0_,
                        (R)(-0.5));
     hipLaunchKernelGGL((kb_hop<T, true>), dim3(nbh_), dim3(256), 0, c_->stream, dsys_, dslot_, dmh_, (R)(0.5));
-    hipLaunchKernelGGL((kb_xr<T>), dim3(nbb_), dim3(kBlas), 0, c_->stream, dslot_, dmb_);
-    hipLaunchKernelGGL((kb_p<T>), dim3(nbb_), dim3(kBlas), 0, c_->stream, dslot_, dmb_);
+    if constexpr (!kMixed) {
+      hipLaunchKernelGGL((kb_xr<T>), dim3(nbb_), dim3(kBlas), 0, c_->stream, dslot_, dmb_);
+      hipLaunchKernelGGL((kb_p<T>), dim3(nbb_), dim3(kBlas), 0, c_->stream, dslot_, dmb_);
+    }
   }
 }
 
@@ -399,7 +446,8 @@ int Engine<T>::add_system(const sfh_gauge* gauge, double mass, int pool) {
     h.ctr = reinterpret_cast<unsigned*>(take(kCtrBytes));
     L.part = reinterpret_cast<double*>(p);
     const int nbh = (V / 2 + 127) / 128, nbb = (V + kBlas - 1) / kBlas;
-    const size_t used = (size_t)(p - (base + sb * k)) + align256((std::max(nbh, nbb) + kRedGroups) * sizeof(double));
+    const size_t used =
+        (size_t)(p - (base + sb * k)) + align256(4 * (std::max(nbh, nbb) + kRedGroups) * sizeof(double));
     if (used > sb) return -set_err(c_, SFH_E_INVALID, "slot layout exceeds its allocation");
     s.pool.push_back((int)slots_.size());
     slots_.push_back(h);
@@ -461,6 +509,7 @@ int Engine<T>::admit(int sys, const double2* b, double2* x, long long ticket) {
   if (kMixed) {
     f64_to_f32(c_->stream, b, reinterpret_cast<float2*>(L.r), n);
     hipMemsetAsync(L.x, 0, n * sizeof(S), c_->stream);
+    hipMemsetAsync(L.v, 0, n * sizeof(S), c_->stream);  // first kb_upd: r - 0 v must stay finite
   } else {
     hipMemcpyAsync(L.r, b, n * sizeof(S), hipMemcpyDeviceToDevice, c_->stream);
   }
@@ -660,7 +709,7 @@ std::vector<typename Engine<T>::Done> Engine<T>::take_done() {
     o.rsq = h.last.rsq;
     o.true_rel_resid = h.phi_sq > 0 ? std::sqrt(h.rsq_true / h.phi_sq) : 0.0;
     o.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - h.t_admit).count();
-    o.hbm_bytes = (double)h.last.iter * (kMixed ? 912.0 : 1824.0) * h.dev.V;
+    o.hbm_bytes = (double)h.last.iter * (kMixed ? 888.0 : 1824.0) * h.dev.V;
     d.state = h.last;
     out.push_back(d);
     h.used = false;
