@@ -18,7 +18,8 @@ sample on the host).
 instead; ``batch`` the 64 uneven batched systems, ``serve`` the solve service
 under Poisson arrivals, ``precision`` the errors of the mixed-precision
 results against the FP64 oracle at 16^4, ``traj32`` a 32^4 Symanzik HMC
-trajectory in FP64 and FP16-mixed.
+trajectory in FP64 and FP16-mixed, ``heatbath`` Cabibbo-Marinari heatbath
+sweeps of the Symanzik action at 32^4 and 16^4.
 """
 import argparse
 import json
@@ -388,13 +389,84 @@ def bench_traj32(args, dev):
          "force_rel_linf_difference": dF / float(np.max(np.abs(F["f64"])))})))
 
 
+# One link update of the heatbath: 12 plaquette and 72 rectangle staple
+# products, about 3 more for the three SU(2) subgroup rotations (the 2 x 2
+# block of U A and two rotated rows each): 87 SU(3) products x 198 flop.
+HEATBATH_FLOP_PER_LINK = 87 * 198
+PEAK_FP64_TFS = 78.6
+
+
+def bench_heatbath(args, dev):
+    """Cabibbo-Marinari heatbath sweeps of the Symanzik action (c11 = 5/3,
+    c12 = -1/12, beta = 4.1) on 32^4 device-resident links from a hot start
+    (links exp(i 0.6 H)), after `warmup` untimed sweeps; 16^4 beside it.  The
+    roofline is the FP64 rate of the sweep's 16 class launches, timed with
+    HIP events on the library stream."""
+    beta, c11, c12 = 4.1, 5.0 / 3.0, -1.0 / 12.0
+    steps = max(args.steps, 4)
+    ctx = dev.Context(0)
+    if args.kernels_only:
+        # two sweeps at 32^4, then one FP32 link conversion (reads 576 B and
+        # writes 288 B per site) that calibrates the traffic counters
+        dims = (32, 32, 32, 32)
+        g = dev.Gauge(ctx, dims, dev.random_links(dims, 0.6, seed=8))
+        g.heatbath(beta, c11, c12, seed=1, sweep0=0, nsweeps=2)
+        g.time_hop("f32", False, 1)
+        ctx.synchronize()
+        log("[bench] heatbath kernels done")
+        return
+    res = {}
+    for dims in [(32, 32, 32, 32), (16, 16, 16, 16)]:
+        V = int(np.prod(dims))
+        g = dev.Gauge(ctx, dims, dev.random_links(dims, 0.6, seed=8))
+        g.heatbath(beta, c11, c12, seed=1, sweep0=0, nsweeps=max(args.warmup, 1))
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        pl, ac = g.heatbath(beta, c11, c12, seed=1, sweep0=100, nsweeps=steps)
+        dt = (time.perf_counter() - t0) / steps
+        ctx.set_profiling(True)
+        g.heatbath(beta, c11, c12, seed=1, sweep0=200, nsweeps=steps)
+        ms, n = ctx.kernel_time("heatbath")
+        ctx.set_profiling(False)
+        per_launch_us = ms * 1e3 / (16 * n)
+        tfs = (V / 4) * HEATBATH_FLOP_PER_LINK / (per_launch_us * 1e-6) / 1e12
+        res["x".join(map(str, dims))] = {"ms_per_sweep": round(dt * 1e3, 3), "sweeps_per_s": round(1 / dt, 2),
+                                         "class_launch_us": round(per_launch_us, 2), "fp64_tflops": round(tfs, 2),
+                                         "plaquette": float(pl[-1]), "action_density": float(ac[-1])}
+        log(f"[bench] heatbath {dims}: {dt * 1e3:.3f} ms/sweep, {per_launch_us:.1f} us per class launch, "
+            f"{tfs:.2f} TF/s")
+    top = res["32x32x32x32"]
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as oc
+        threads = min(16, os.cpu_count() or 1)
+        oc.set_threads(threads)
+        dims = (32, 32, 32, 32)
+        U = dev.random_links(dims, 0.6, seed=8)
+        t0 = time.perf_counter()
+        oc.heatbath(dims, U, beta, c11, c12, seed=1, sweep0=0, nsweeps=1, order=1)
+        tc = time.perf_counter() - t0
+        cpu = {"value": round(1.0 / tc, 4), "unit": "sweeps/s", "cores": threads, "kind": "port",
+               "sample": f"one colour-ordered oracle sweep at 32^4 ({tc:.1f} s, OpenMP over each colour class)"}
+    traffic = traffic_from_profiles("heatbath", "32^4")
+    out = _base_line(
+        top["sweeps_per_s"], "sweeps/s", steps, top["ms_per_sweep"], "f64",
+        "32^4 Cabibbo-Marinari heatbath sweep, Symanzik (5/3, -1/12), beta 4.1, hot start",
+        {"metric": "heatbath sweeps/s (Symanzik action, 32^4)", "sizes": res,
+         "roofline": {"bound": "mfma", "achieved": top["fp64_tflops"], "peak": PEAK_FP64_TFS, "unit": "TFLOP/s",
+                      "frac": round(top["fp64_tflops"] / PEAK_FP64_TFS, 4), "traffic": traffic,
+                      "kernel": "k_heatbath (one colour class, 262144 links, 87 x 198 flop each)"},
+         "cpu_baseline": cpu})
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="16^4",
-                    choices=sorted(WORKLOADS) + ["batch", "serve", "precision", "traj32"])
+                    choices=sorted(WORKLOADS) + ["batch", "serve", "precision", "traj32", "heatbath"])
     ap.add_argument("--rates", default="0.5,1,2", help="serve: arrival rates as multiples of the measured "
                                                      "sequential FP64 capacity")
     ap.add_argument("--requests", type=int, default=200, help="serve: requests per arrival rate")
@@ -403,7 +475,8 @@ def main():
                     help="only the kernel timing loops of the workload (the hop in FP64 and in the inner precision, "
                          "the two steps of the mixed iteration), e.g. under a hardware-counter profiler; the first "
                          "use of the low-precision link copy also runs its conversion kernel once (reads 576 B, "
-                         "writes 288 B (FP32) or 144 B (FP16) per site)")
+                         "writes 288 B (FP32) or 144 B (FP16) per site); for the heatbath workload, two 32^4 sweeps "
+                         "and one FP32 link conversion")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a "
@@ -420,6 +493,8 @@ def main():
         return bench_precision(args, dev)
     if args.workload == "traj32":
         return bench_traj32(args, dev)
+    if args.workload == "heatbath":
+        return bench_heatbath(args, dev)
 
     wl = WORKLOADS[args.workload]
     dims, mass = wl["dims"], wl["mass"]

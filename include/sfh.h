@@ -37,7 +37,13 @@ extern "C" {
  *                                      (Fermions.cpp:712-716, :752-756)
  * EXP_QCUBED / EXP_NEG_TRACE         <- std::invalid_argument in
  *                                      numerically_stable_matrix_exponential
- *                                      (Fermions.cpp:848-857)                 */
+ *                                      (Fermions.cpp:848-857)
+ * SAMPLER                            <- no counterpart: a heatbath rejection
+ *                                      loop (SU3_Sampling.cpp:17-29, :53-68,
+ *                                      :175-191) that found no sample in 2^20
+ *                                      tries (non-finite links, or a*beta so
+ *                                      small that the reference would loop for
+ *                                      ever)                                  */
 typedef enum {
   SFH_OK = 0,
   SFH_E_INVALID = 1,
@@ -50,7 +56,8 @@ typedef enum {
   SFH_E_NOCONV = 8,
   SFH_E_NOT_HERMITIAN = 9,
   SFH_E_EXP_QCUBED = 10,
-  SFH_E_EXP_NEG_TRACE = 11
+  SFH_E_EXP_NEG_TRACE = 11,
+  SFH_E_SAMPLER = 12
 } sfh_status;
 
 /* Arithmetic of the CG inner iterations.  F64 follows the reference's
@@ -280,6 +287,23 @@ int sfh_avg_plaq(sfh_ctx* ctx, const sfh_gauge* g, double* out);
  * poly_loop_at_spat_coord (Lattice.cpp:849-884); the correlators of
  * Lattice.cpp:888-948 are cheap host sums over this grid. */
 int sfh_polyakov_grid(sfh_ctx* ctx, const sfh_gauge* g, double* out);
+/* nsweeps Cabibbo-Marinari heatbath sweeps of the Symanzik gauge action
+ * (beta, c11, c12) on the device-resident links.  Replaces heatbath_update
+ * (Lattice.cpp:721-757) with single_link_heatbath (:697-719), the subgroup
+ * updates R, S, T (:571-695) and the SU(2) sampler (SU3_Sampling.cpp:17-199).
+ * Every extent must be a multiple of 4 and beta > 0.  The sweep visits the
+ * links direction by direction, each direction in four colour classes
+ * (2 n_mu + sum of the other coordinates) mod 4 whose links are updated in
+ * parallel; the reference visits sites lexicographically.  Uniforms are
+ * Philox4x32-10 draws addressed by (seed; draw, link, sweep index), the
+ * sweeps of one call taking indices sweep0, sweep0 + 1, ...: a result
+ * depends on the seed and sweep indices only.  avg_plaq[k] and
+ * action_density[k] (either may be NULL) receive sweep k's
+ * (sum of ReTr(U A1)/3 / 4) / (6 V) and (sum of -(beta/3) ReTr(U A) / 4) / V,
+ * each link's term taken right after its update, as the reference returns
+ * them. */
+int sfh_heatbath(sfh_ctx* ctx, sfh_gauge* g, double beta, double c11, double c12, unsigned long long seed,
+                 unsigned sweep0, int nsweeps, double* avg_plaq, double* action_density);
 /* sum over links of ReTr(P^2).  Replaces compute_Tr_P2 (Fermions.cpp:662-678). */
 int sfh_tr_p2(sfh_ctx* ctx, const sfh_gauge* g, const double* P, double* out);
 /* U <- exp(i eps P) U on every device link (Fermions.cpp:703-723, exponential

@@ -40,7 +40,7 @@ using cd = std::complex<double>;
 enum {
   OK = 0, E_INVALID = 1, E_BREAKDOWN = 5, E_NONFINITE_ALPHA = 6,
   E_NONFINITE_BETA = 7, E_NOCONV = 8, E_NOT_HERMITIAN = 9, E_EXP_QCUBED = 10,
-  E_EXP_NEG_TRACE = 11
+  E_EXP_NEG_TRACE = 11, E_SAMPLER = 12
 };
 
 struct M3 { cd a[9]; };
@@ -635,6 +635,317 @@ int oc_polyakov_grid(const int* dims, const double* U, double* out) {
     out[2 * k + 1] = tr.imag();
   }
   return OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ heatbath
+// Cabibbo-Marinari heatbath with Symanzik staples: Lattice.cpp:507-757 and
+// SU3_Sampling.cpp:4-199.  The reference draws its uniforms from a per-thread
+// std::mt19937 seeded from std::random_device (rng.cpp:9-27), which no run
+// can reproduce; here every uniform comes from a counter-based Philox4x32-10
+// stream (Salmon, Moraes, Dror, Shaw, SC'11) addressed by
+// (seed; draw pair, link, sweep, tag), in the order the reference's sampler
+// consumes them, so the device sweep can be checked draw for draw.
+namespace {
+
+struct Philox4x32 {
+  uint32_t k0, k1;
+  void block(uint32_t c[4]) const {
+    uint32_t a = k0, b = k1;
+    for (int r = 0; r < 10; ++r) {
+      const uint64_t p0 = uint64_t(0xD2511F53u) * c[0];
+      const uint64_t p1 = uint64_t(0xCD9E8D57u) * c[2];
+      const uint32_t n0 = uint32_t(p1 >> 32) ^ c[1] ^ a;
+      const uint32_t n2 = uint32_t(p0 >> 32) ^ c[3] ^ b;
+      c[1] = uint32_t(p1);
+      c[3] = uint32_t(p0);
+      c[0] = n0;
+      c[2] = n2;
+      a += 0x9E3779B9u;
+      b += 0xBB67AE85u;
+    }
+  }
+};
+
+constexpr uint32_t kHeatbathTag = 0x68623031u;  // separates these streams from the field generators'
+// Tries before a rejection loop gives up (the reference's loops have no
+// bound and would not end on non-finite links); the device uses the same.
+constexpr int kMaxTries = 1 << 20;
+
+// 53-bit uniform in [0, 1) from two 32-bit words.
+inline double u53(uint32_t a, uint32_t b) {
+  return (double(a >> 5) * 67108864.0 + double(b >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+// The uniforms of one link update: draw k is half (k & 1) of Philox block
+// {k >> 1, link, sweep, tag}.
+struct LinkStream {
+  Philox4x32 ph;
+  uint32_t link, sweep, q = 0;
+  int have = 0;
+  double next_u = 0.0;
+  bool fail = false;
+  LinkStream(unsigned long long seed, uint32_t l, uint32_t s)
+      : ph{uint32_t(seed), uint32_t(seed >> 32)}, link(l), sweep(s) {}
+  double uniform() {
+    if (have) { have = 0; return next_u; }
+    uint32_t c[4] = {q++, link, sweep, kHeatbathTag};
+    ph.block(c);
+    next_u = u53(c[2], c[3]);
+    have = 1;
+    return u53(c[0], c[1]);
+  }
+};
+
+constexpr double kPi = 3.14159265358979323846;  // SU3_Sampling.h:10
+
+// SU3_Sampling.cpp:17-29 (Gattringer-Lang): x0 with density
+// sqrt(1 - x0^2) exp(a beta x0).
+double gen_x0(double a, double beta, LinkStream& rs) {
+  for (int tries = 0; tries < kMaxTries; ++tries) {
+    const double r1 = 1.0 - rs.uniform();
+    const double r2 = 1.0 - rs.uniform();
+    const double r3 = 1.0 - rs.uniform();
+    const double c = std::cos(2 * kPi * r2);
+    const double lam2 = -1 / (2 * a * beta) * (std::log(r1) + c * c * std::log(r3));
+    const double x = 1 - lam2;
+    const double r = rs.uniform();
+    if (r * r <= x && x >= -1 && x <= 1) return 1 - 2 * lam2;
+  }
+  rs.fail = true;
+  return 1.0;
+}
+
+// SU3_Sampling.cpp:53-68: unit vector by rejection from the cube [-1, 1)^3.
+void unit3(LinkStream& rs, double v[3]) {
+  for (int tries = 0; tries < kMaxTries; ++tries) {
+    for (int k = 0; k < 3; ++k) v[k] = 2.0 * rs.uniform() - 1.0;
+    const double len = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (len <= 1.0) {
+      for (int k = 0; k < 3; ++k) v[k] /= len;
+      return;
+    }
+  }
+  rs.fail = true;
+  v[0] = 1.0; v[1] = v[2] = 0.0;
+}
+
+// SU(2) as [[x0 + i x3, x2 + i x1], [-x2 + i x1, x0 - i x3]]
+// (SU3_Sampling.cpp:82-95, :193-199).
+struct M2 { cd a[4]; };
+inline M2 su2_of(const double x[4]) {
+  return M2{{cd(x[0], x[3]), cd(x[2], x[1]), cd(-x[2], x[1]), cd(x[0], -x[3])}};
+}
+inline M2 mul2(const M2& x, const M2& y) {
+  return M2{{x.a[0] * y.a[0] + x.a[1] * y.a[2], x.a[0] * y.a[1] + x.a[1] * y.a[3],
+             x.a[2] * y.a[0] + x.a[3] * y.a[2], x.a[2] * y.a[1] + x.a[3] * y.a[3]}};
+}
+inline M2 dag2(const M2& x) {
+  return M2{{std::conj(x.a[0]), std::conj(x.a[2]), std::conj(x.a[1]), std::conj(x.a[3])}};
+}
+
+// SU3_Sampling.cpp:161-170 with :82-95.
+M2 su2_heat(double a, double beta, LinkStream& rs, double* x0_out) {
+  double x[4];
+  x[0] = gen_x0(a, beta, rs);
+  double v[3];
+  unit3(rs, v);
+  const double nrm = std::sqrt(1 - x[0] * x[0]);
+  for (int k = 0; k < 3; ++k) x[k + 1] = v[k] * nrm;
+  if (x0_out) *x0_out = x[0];
+  return su2_of(x);
+}
+
+// SU3_Sampling.cpp:175-199.
+M2 su2_random(LinkStream& rs) {
+  for (int tries = 0; tries < kMaxTries; ++tries) {
+    double r[4];
+    for (auto& v : r) v = 2.0 * rs.uniform() - 1.0;
+    const double len = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2] + r[3] * r[3]);
+    if (len <= 1.0) {
+      for (auto& v : r) v /= len;
+      return su2_of(r);
+    }
+  }
+  rs.fail = true;
+  const double one[4] = {1.0, 0.0, 0.0, 0.0};
+  return su2_of(one);
+}
+
+// The subgroups in the reference's update order (Lattice.cpp:507-569,
+// :697-719): R = rows/columns (0, 1), S = (0, 2), T = (1, 2).
+constexpr int kSub[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+
+// One subgroup update U <- embed(R) U (Lattice.cpp:571-695): the 2 x 2 block
+// of U A projected onto the quaternions (Lattice.cpp:571-610), heatbath
+// sample X at a = sqrt(det), R = X (block / a)^dagger; below a = 1e-14 a
+// Haar-random R.  Returns a and the drawn x0 (0 on the random branch).
+M3 subgroup_heatbath(const M3& U, const M3& A, double beta, int type, LinkStream& rs, double* a_out,
+                     double* x0_out) {
+  if (rs.fail) return U;
+  const int i = kSub[type][0], j = kSub[type][1];
+  const M3 W = mul(U, A);
+  const M2 B{{W.a[3 * i + i], W.a[3 * i + j], W.a[3 * j + i], W.a[3 * j + j]}};
+  // b0 = Re tr(B) / 2 and b_l = i Im tr(B sigma_l) / 2
+  const double b0 = 0.5 * (B.a[0].real() + B.a[3].real());
+  const double k1 = 0.5 * (B.a[1].imag() + B.a[2].imag());
+  const double k2 = 0.5 * (B.a[1].real() - B.a[2].real());
+  const double k3 = 0.5 * (B.a[0].imag() - B.a[3].imag());
+  const double q[4] = {b0, k1, k2, k3};
+  const M2 P = su2_of(q);
+  const double a = std::sqrt((P.a[0] * P.a[3] - P.a[1] * P.a[2]).real());
+  M2 R;
+  double x0 = 0.0;
+  if (a <= 1e-14) {
+    R = su2_random(rs);
+  } else {
+    M2 Vn = P;
+    for (auto& v : Vn.a) v /= a;
+    R = mul2(su2_heat(a, 2 * beta / 3, rs, &x0), dag2(Vn));
+  }
+  M3 out = U;
+  for (int c = 0; c < 3; ++c) {
+    out.a[3 * i + c] = R.a[0] * U.a[3 * i + c] + R.a[1] * U.a[3 * j + c];
+    out.a[3 * j + c] = R.a[2] * U.a[3 * i + c] + R.a[3] * U.a[3 * j + c];
+  }
+  if (a_out) *a_out = a;
+  if (x0_out) *x0_out = x0;
+  return out;
+}
+
+inline uint32_t link_id(int s, int mu) { return uint32_t(s) * 4u + uint32_t(mu); }
+
+// Lattice.cpp:697-719: staples from the current links, then R, S, T; the
+// link's plaquette and action terms ReTr(U A1) / 3 and -(beta/3) ReTr(U A).
+// A link whose sampler ran out of tries keeps its value, contributes 0 and
+// returns false.
+bool single_link_heatbath(const Geo& g, double* U, int s, int mu, double beta, double c11, double c12,
+                          unsigned long long seed, uint32_t sweep, double* plaq, double* act) {
+  const M3 A1 = staple(g, U, s, mu);
+  const M3 A2 = rect_staple(g, U, s, mu);
+  const M3 A = add(scale(c11, A1), scale(c12, A2));
+  LinkStream rs(seed, link_id(s, mu), sweep);
+  M3 u = load_link(U, s, mu);
+  for (int t = 0; t < 3; ++t) u = subgroup_heatbath(u, A, beta, t, rs, nullptr, nullptr);
+  if (rs.fail) {
+    *plaq = *act = 0.0;
+    return false;
+  }
+  store_link(U, s, mu, u);
+  *plaq = trace(mul(u, A1)).real() / 3.0;
+  *act = -(beta / 3.0) * trace(mul(u, A)).real();
+  return true;
+}
+
+// Colour of link (s, mu) in the parallel sweep: (2 n_mu + sum of the other
+// coordinates) & 3.  The mu-links in the staples of (n, mu) sit at offsets
+// +-mu, +-nu, +-2 nu and +-mu +-nu (nu != mu), which change the colour by 2,
+// 1, 2 and 1 or 3 (mod 4), so no staple of a link holds another mu-link of its
+// colour when every extent is a multiple of 4.  Four colours is the fewest
+// possible: links at n, n + mu, n + nu, n + mu + nu are pairwise coupled.
+inline int link_colour(const Geo& g, int s, int mu) {
+  const int* n = &g.coord[4 * s];
+  return (n[0] + n[1] + n[2] + n[3] + n[mu]) & 3;
+}
+
+// One sweep, Lattice.cpp:721-757.  order 0: the reference's order (sites
+// lexicographic, direction innermost), serial.  order 1: direction by
+// direction, each in 4 colour classes, every class in parallel (the device
+// order).  Returns the reference's (average plaquette, action density).
+bool heatbath_sweep(const Geo& g, double* U, double beta, double c11, double c12, unsigned long long seed,
+                    uint32_t sweep, int order, double* avg_plaq, double* act_density) {
+  std::vector<double> pl(4 * size_t(g.V)), ac(4 * size_t(g.V));
+  int failed = 0;
+  if (order == 0) {
+    for (int s = 0; s < g.V; ++s)
+      for (int mu = 0; mu < 4; ++mu)
+        failed += !single_link_heatbath(g, U, s, mu, beta, c11, c12, seed, sweep, &pl[4 * s + mu],
+                                        &ac[4 * s + mu]);
+  } else {
+    for (int mu = 0; mu < 4; ++mu)
+      for (int col = 0; col < 4; ++col) {
+#pragma omp parallel for schedule(static) reduction(+ : failed)
+        for (int s = 0; s < g.V; ++s)
+          if (link_colour(g, s, mu) == col)
+            failed += !single_link_heatbath(g, U, s, mu, beta, c11, c12, seed, sweep, &pl[4 * s + mu],
+                                            &ac[4 * s + mu]);
+      }
+  }
+  double tp = 0.0, ta = 0.0;
+  for (size_t l = 0; l < pl.size(); ++l) {
+    tp += pl[l];
+    ta += ac[l];
+  }
+  *avg_plaq = (tp / 4.0) / (6.0 * g.V);
+  *act_density = (ta / 4.0) / g.V;
+  return failed == 0;
+}
+
+bool heatbath_dims_ok(const int* d) {
+  for (int k = 0; k < 4; ++k)
+    if (d[k] < 4 || d[k] % 4) return false;
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Philox4x32-10 block: out = philox(key, ctr).
+int oc_philox(const unsigned* key, const unsigned* ctr, unsigned* out) {
+  const Philox4x32 ph{key[0], key[1]};
+  uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+  ph.block(c);
+  for (int k = 0; k < 4; ++k) out[k] = c[k];
+  return OK;
+}
+
+// n samples of x0 (SU3_Sampling.cpp:17-29) at (a, beta); sample k uses the
+// stream of link k at the given sweep.
+int oc_heatbath_x0(double a, double beta, unsigned long long seed, unsigned sweep, int n, double* out) {
+  if (!(a > 0) || !(beta > 0) || n < 0) return E_INVALID;
+#pragma omp parallel for
+  for (int k = 0; k < n; ++k) {
+    LinkStream rs(seed, uint32_t(k), sweep);
+    out[k] = gen_x0(a, beta, rs);
+  }
+  return OK;
+}
+
+// One subgroup update (type 0, 1, 2 = R, S, T) of U18 at staple sum A18,
+// with the stream of (link, sweep); returns a and x0 through ax[2].
+int oc_heatbath_subgroup(const double* U18, const double* A18, double beta, int type, unsigned long long seed,
+                         unsigned link, unsigned sweep, double* out18, double* ax) {
+  if (type < 0 || type > 2 || !(beta > 0)) return E_INVALID;
+  M3 u, a;
+  for (int k = 0; k < 9; ++k) {
+    u.a[k] = cd(U18[2 * k], U18[2 * k + 1]);
+    a.a[k] = cd(A18[2 * k], A18[2 * k + 1]);
+  }
+  LinkStream rs(seed, link, sweep);
+  const M3 r = subgroup_heatbath(u, a, beta, type, rs, &ax[0], &ax[1]);
+  for (int k = 0; k < 9; ++k) {
+    out18[2 * k] = r.a[k].real();
+    out18[2 * k + 1] = r.a[k].imag();
+  }
+  return rs.fail ? E_SAMPLER : OK;
+}
+
+// nsweeps heatbath sweeps with sweep indices sweep0, sweep0 + 1, ...;
+// plaq / act (may be null) receive each sweep's averages.
+int oc_heatbath(const int* dims, double* U, double beta, double c11, double c12, unsigned long long seed,
+                unsigned sweep0, int nsweeps, int order, double* plaq, double* act) {
+  if (!heatbath_dims_ok(dims) || !(beta > 0) || nsweeps < 0 || order < 0 || order > 1) return E_INVALID;
+  Geo g(dims);
+  bool ok = true;
+  for (int k = 0; k < nsweeps; ++k) {
+    double p, a;
+    ok &= heatbath_sweep(g, U, beta, c11, c12, seed, sweep0 + uint32_t(k), order, &p, &a);
+    if (plaq) plaq[k] = p;
+    if (act) act[k] = a;
+  }
+  return ok ? OK : E_SAMPLER;
 }
 
 }  // extern "C"

@@ -50,6 +50,11 @@ def lib():
             "oc_hmc_trajectory": (_I, [_P, _D, _D, _I, _D, _D, _D, _D, _I, _P, _P, _D, _P, _P, _P]),
             "oc_set_threads": (_I, [_I]),
             "oc_polyakov_grid": (_I, [_P, _P, _P]),
+            "oc_philox": (_I, [_P, _P, _P]),
+            "oc_heatbath_x0": (_I, [_D, _D, ctypes.c_ulonglong, ctypes.c_uint, _I, _P]),
+            "oc_heatbath_subgroup": (_I, [_P, _P, _D, _I, ctypes.c_ulonglong, ctypes.c_uint, ctypes.c_uint, _P,
+                                          _P]),
+            "oc_heatbath": (_I, [_P, _P, _D, _D, _D, ctypes.c_ulonglong, ctypes.c_uint, _I, _I, _P, _P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -204,3 +209,48 @@ def polyakov_grid(dims, U):
     out = np.empty(2 * int(d[0] * d[1] * d[2]))
     lib().oc_polyakov_grid(_p(d), _p(U), _p(out))
     return out.view(np.complex128)
+
+
+def philox(key, ctr):
+    """Philox4x32-10 block of a 2-word key and 4-word counter (uint32)."""
+    k = np.ascontiguousarray(np.asarray(key, dtype=np.uint32))
+    c = np.ascontiguousarray(np.asarray(ctr, dtype=np.uint32))
+    out = np.empty(4, dtype=np.uint32)
+    lib().oc_philox(_p(k), _p(c), _p(out))
+    return out
+
+
+def heatbath_x0(a, beta, n, seed=1, sweep=0):
+    """n draws of the SU(2) heatbath x0 at (a, beta): density
+    sqrt(1 - x0^2) exp(a beta x0) (SU3_Sampling.cpp:17-29)."""
+    out = np.empty(int(n))
+    st = lib().oc_heatbath_x0(float(a), float(beta), int(seed), int(sweep), int(n), _p(out))
+    if st:
+        raise OracleError(st, "heatbath_x0")
+    return out
+
+
+def heatbath_subgroup(U18, A18, beta, sub, seed=1, link=0, sweep=0):
+    """One subgroup update U <- embed(R) U of the Cabibbo-Marinari step (sub
+    0, 1, 2 = rows/columns (0,1), (0,2), (1,2)); returns (U', a, x0)."""
+    U = _f64(U18); A = _f64(A18)
+    out = np.empty(18); ax = np.zeros(2)
+    st = lib().oc_heatbath_subgroup(_p(U), _p(A), float(beta), int(sub), int(seed), int(link), int(sweep),
+                                    _p(out), _p(ax))
+    if st:
+        raise OracleError(st, "heatbath_subgroup")
+    return out, ax[0], ax[1]
+
+
+def heatbath(dims, U, beta, c11, c12, seed, sweep0=0, nsweeps=1, order=1):
+    """nsweeps Cabibbo-Marinari sweeps (Lattice.cpp:721-757).  order 1 is
+    the device's colour order, 0 the reference's lexicographic order.
+    Returns (U', per-sweep average plaquette, per-sweep action density)."""
+    d = np.ascontiguousarray(np.asarray(dims, dtype=np.int32))
+    U = _f64(U).copy()
+    pl = np.empty(int(nsweeps)); ac = np.empty(int(nsweeps))
+    st = lib().oc_heatbath(_p(d), _p(U), float(beta), float(c11), float(c12), int(seed), int(sweep0),
+                           int(nsweeps), int(order), _p(pl), _p(ac))
+    if st:
+        raise OracleError(st, "heatbath")
+    return U, pl, ac

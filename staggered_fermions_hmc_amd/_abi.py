@@ -23,7 +23,7 @@ SFH_OK = 0
 STATUS_NAMES = {
     0: "SFH_OK", 1: "SFH_E_INVALID", 2: "SFH_E_NOMEM", 3: "SFH_E_HIP", 4: "SFH_E_NODEVICE",
     5: "SFH_E_BREAKDOWN", 6: "SFH_E_NONFINITE_ALPHA", 7: "SFH_E_NONFINITE_BETA", 8: "SFH_E_NOCONV",
-    9: "SFH_E_NOT_HERMITIAN", 10: "SFH_E_EXP_QCUBED", 11: "SFH_E_EXP_NEG_TRACE",
+    9: "SFH_E_NOT_HERMITIAN", 10: "SFH_E_EXP_QCUBED", 11: "SFH_E_EXP_NEG_TRACE", 12: "SFH_E_SAMPLER",
 }
 PREC = {"f64": 0, "f32": 1, "f16": 2}
 FORCE_GAUGE = 1
@@ -104,6 +104,7 @@ SIGNATURES = {
     "sfh_avg_plaq": (_I, [_P, _P, ctypes.POINTER(_D)]),
     "sfh_tr_p2": (_I, [_P, _P, _P, ctypes.POINTER(_D)]),
     "sfh_polyakov_grid": (_I, [_P, _P, _P]),
+    "sfh_heatbath": (_I, [_P, _P, _D, _D, _D, ctypes.c_ulonglong, ctypes.c_uint, _I, _P, _P]),
     "sfh_link_update": (_I, [_P, _P, _D, _P, _I]),
     "sfh_exp_iq": (_I, [_P, _P, _P]),
     "sfh_leapfrog": (_I, [_P, _P, ctypes.POINTER(MDParams), _P, _P, _P]),

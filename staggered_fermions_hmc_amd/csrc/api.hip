@@ -206,6 +206,7 @@ const char* sfh_status_string(int s) {
     case SFH_E_NOT_HERMITIAN: return "momentum not hermitian traceless";
     case SFH_E_EXP_QCUBED: return "exp: Im Tr Q^3 non-zero";
     case SFH_E_EXP_NEG_TRACE: return "exp: Tr Q^2 negative";
+    case SFH_E_SAMPLER: return "heatbath sampler found no sample in 2^20 tries";
     default: return "unknown status";
   }
 }
@@ -749,6 +750,18 @@ int sfh_polyakov_grid(sfh_ctx* c, const sfh_gauge* g, double* out) {
   if (!c || !g || !out) return SFH_E_INVALID;
   Guard gd(c);
   return polyakov_grid(c, g->g, g->U, out);
+}
+
+int sfh_heatbath(sfh_ctx* c, sfh_gauge* g, double beta, double c11, double c12, unsigned long long seed,
+                 unsigned sweep0, int nsweeps, double* avg_plaq, double* action_density) {
+  if (!c || !g || nsweeps < 0 || !(beta > 0) || !std::isfinite(beta) || !std::isfinite(c11) ||
+      !std::isfinite(c12))
+    return SFH_E_INVALID;
+  if (!heatbath_dims_ok(g->g))
+    return set_err(c, SFH_E_INVALID, "heatbath needs every lattice extent to be a multiple of 4");
+  Guard gd(c);
+  invalidate_lp(g);
+  return heatbath(c, g->g, g->U, beta, c11, c12, seed, sweep0, nsweeps, avg_plaq, action_density);
 }
 
 int sfh_tr_p2(sfh_ctx* c, const sfh_gauge* g, const double* P, double* out) {

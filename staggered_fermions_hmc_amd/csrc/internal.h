@@ -178,6 +178,11 @@ int link_update(sfh_ctx* c, const Geom& g, double2* U, const double2* P, double 
 int exp_iq_one(sfh_ctx* c, const double* Q18, double* out18);
 int polyakov_grid(sfh_ctx* c, const Geom& g, const double2* U, double* out);
 
+// ------------------------------------------------------------------ heatbath (heatbath.hip)
+bool heatbath_dims_ok(const Geom& g);
+int heatbath(sfh_ctx* c, const Geom& g, double2* U, double beta, double c11, double c12, unsigned long long seed,
+             unsigned sweep0, int nsweeps, double* plaq, double* act);
+
 // ------------------------------------------------------------------ solver (solver.cpp)
 int cg_solve(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* p, const double2* phi, double2* x,
              sfh_cg_stats* stats);

@@ -11,7 +11,7 @@ from ._abi import SfhError, f64, ptr
 # Reference exception classes by status (Fermions.cpp: std::runtime_error for
 # CG and the hermiticity guard, std::invalid_argument for the exponential).
 _EXC = {5: RuntimeError, 6: RuntimeError, 7: RuntimeError, 8: RuntimeError, 9: RuntimeError,
-        10: ValueError, 11: ValueError, 1: ValueError}
+        10: ValueError, 11: ValueError, 12: RuntimeError, 1: ValueError}
 
 
 class SolverError(SfhError):
@@ -210,6 +210,17 @@ class Gauge:
         out = np.empty(2 * self.dims[0] * self.dims[1] * self.dims[2])
         self.ctx.check(self.ctx._lib.sfh_polyakov_grid(self.ctx.h, self.h, ptr(out)))
         return out.view(np.complex128)
+
+    def heatbath(self, beta, c11, c12, seed, sweep0=0, nsweeps=1):
+        """nsweeps Cabibbo-Marinari heatbath sweeps of the Symanzik action on
+        the device-resident links (heatbath_update, Lattice.cpp:721-757),
+        sweep indices sweep0, sweep0 + 1, ...  Returns the per-sweep
+        (average plaquette, action density) arrays the reference returns."""
+        pl = np.empty(max(int(nsweeps), 1))
+        ac = np.empty(max(int(nsweeps), 1))
+        self.ctx.check(self.ctx._lib.sfh_heatbath(self.ctx.h, self.h, float(beta), float(c11), float(c12), int(seed),
+                                                  int(sweep0), int(nsweeps), ptr(pl), ptr(ac)))
+        return pl[:nsweeps], ac[:nsweeps]
 
     def tr_p2(self, P):
         a = f64(P)

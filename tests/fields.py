@@ -146,3 +146,17 @@ def gauge_transform(dims, U, psi, seed):
         out.append(vec_to_flat(np.einsum("vij,vj->vi", G, flat_to_vec(psi))))
     out.append(G)
     return out
+
+
+def u_exact(beta, n=256):
+    """<ReTr U / 3> under exp((beta/3) ReTr U) dU on SU(3), by Weyl's formula
+    over the eigenphases (the integrand is smooth and periodic, so the
+    rectangle rule converges exponentially)."""
+    th = np.linspace(0, 2 * np.pi, n, endpoint=False)
+    t1, t2 = np.meshgrid(th, th, indexing="ij")
+    t3 = -t1 - t2
+    e = [np.exp(1j * t) for t in (t1, t2, t3)]
+    vdm = abs(e[0] - e[1]) ** 2 * abs(e[0] - e[2]) ** 2 * abs(e[1] - e[2]) ** 2
+    p = (np.cos(t1) + np.cos(t2) + np.cos(t3)) / 3
+    w = vdm * np.exp(beta * p)
+    return float((w * p).sum() / w.sum())
