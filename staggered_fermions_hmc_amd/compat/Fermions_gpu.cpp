@@ -24,6 +24,7 @@ struct ThreadState {
   sfh_gauge* gauge = nullptr;
   unsigned long long generation = 0;
   unsigned long long draws = 0;  // Philox stream position of this thread
+  unsigned hb_sweeps = 0;        // heatbath sweep index of this thread
   std::mt19937_64 uniform{0x9e3779b97f4a7c15ULL};
   ~ThreadState() {
     if (gauge) sfh_gauge_destroy(ctx, gauge);
@@ -308,6 +309,17 @@ double compute_avg_plaq(const Link_array& arr) {
   double out = 0.0;
   check(sfh_avg_plaq(ctx(), g, &out), "compute_avg_plaq");
   return out;
+}
+
+// Lattice.cpp:721-757.
+std::pair<double, double> heatbath_update(Link_array& arr, double beta) {
+  sfh_gauge* g = gauge_from(arr);
+  double plaq = 0.0, act = 0.0;
+  const unsigned long long key = config().seed ^ (std::hash<const void*>{}(&ts) * 0x9e3779b97f4a7c15ULL);
+  check(sfh_heatbath(ctx(), g, beta, config().c_11, config().c_12, key, ts.hb_sweeps++, 1, &plaq, &act),
+        "heatbath_update");
+  check(sfh_gauge_download(ctx(), g, arr.data()), "sfh_gauge_download");
+  return {plaq, act};
 }
 
 static double uniform01() { return std::uniform_real_distribution<double>(0.0, 1.0)(ts.uniform); }

@@ -2,7 +2,7 @@
 // functions, implemented on the MI355X library through the C ABI (sfh.h).
 //
 // Same names, signatures, argument meaning, ownership and exceptions as the
-// reference (Fermions.h:31-70, :105, :130; Lattice.h:76-77):
+// reference (Fermions.h:31-70, :105, :130; Lattice.h:75-77):
 //   - inputs are const& host vectors in the reference layouts, results are
 //     returned by value with exact sizes 6V or 4V*18;
 //   - apply_DDdagger writes into the caller's pre-sized out_field;
@@ -15,6 +15,7 @@
 // from inside an OpenMP loop over coupling values, Main.cpp:174).
 #pragma once
 
+#include <utility>
 #include <vector>
 
 #include "sfh_compat.h"
@@ -57,3 +58,10 @@ std::vector<double> local_final_step(std::vector<double>& P_field, double beta, 
 
 double compute_action(const Link_array& arr, double beta);
 double compute_avg_plaq(const Link_array& arr);
+// One Cabibbo-Marinari heatbath sweep of the Symanzik action (c_11, c_12 of
+// the configuration) in place; returns (average plaquette, action density)
+// as the reference does.  Sweeps of one thread take consecutive sweep
+// indices of the configured seed's Philox streams; the links are visited in
+// the device's colour order, so a sweep differs from the reference's
+// lexicographic one draw for draw but samples the same distribution.
+std::pair<double, double> heatbath_update(Link_array& arr, double beta);

@@ -187,6 +187,31 @@ static int run() {
     report("HMC update keeps links unitary", err < 1e-12, err);
   }
 
+  // heatbath sweeps (Lattice.cpp:721-757) from a cold start thermalise the
+  // links and keep them in SU(3); the returned averages are those of the
+  // reference (plaquette normalised to 1 on a cold lattice)
+  {
+    Link_array W = cold();
+    std::pair<double, double> r{};
+    for (int k = 0; k < 10; ++k) r = heatbath_update(W, 4.1);
+    double err = 0;
+    for (size_t l = 0; l < W.size() / 18; ++l) {
+      std::complex<double> m[9];
+      for (int k = 0; k < 9; ++k) m[k] = {W[18 * l + 2 * k], W[18 * l + 2 * k + 1]};
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          std::complex<double> s = 0;
+          for (int k = 0; k < 3; ++k) s += m[3 * i + k] * std::conj(m[3 * j + k]);
+          err = std::max(err, std::abs(s - (i == j ? 1.0 : 0.0)));
+        }
+    }
+    report("heatbath keeps links unitary", err < 1e-12, err);
+    report("heatbath plaquette in (0, 1) and below cold", r.first > 0.2 && r.first < 0.99, r.first);
+    report("heatbath plaquette matches compute_avg_plaq / 6 (10%)",
+           std::abs(compute_avg_plaq(W) / 6.0 - r.first) < 0.1 * r.first, compute_avg_plaq(W) / 6.0 - r.first);
+    report("heatbath action density negative", r.second < 0, r.second);
+  }
+
   std::printf("%s: %d failure(s)\n", failures ? "FAIL" : "PASS", failures);
   return failures ? 1 : 0;
 }

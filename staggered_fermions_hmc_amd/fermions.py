@@ -2,7 +2,7 @@
 MI355X library.
 
 Same function names, argument meaning and error behaviour as the reference
-(Fermions.h:31-70, :105, :130; Lattice.h:76-77); host arrays are float64 in
+(Fermions.h:31-70, :105, :130; Lattice.h:75-77); host arrays are float64 in
 the reference layouts.  The reference's compile-time constants
 (Parameters.h:9-24, :51-52) are the fields of :class:`Parameters`, set once
 with :func:`configure`.  Each call hands its host arrays to the device and
@@ -141,6 +141,21 @@ def compute_action(arr, beta):
 def compute_avg_plaq(arr):
     """Lattice.cpp:784-804."""
     return _gauge(arr).avg_plaq()
+
+
+_hb_sweep = [0]
+
+
+def heatbath_update(arr, beta):
+    """One Cabibbo-Marinari heatbath sweep of the Symanzik action in place
+    (Lattice.cpp:721-757); returns (average plaquette, action density).  The
+    links are visited in the device's colour order; sweeps take consecutive
+    sweep indices of the Philox streams."""
+    g = _gauge(arr)
+    pl, ac = g.heatbath(beta, params.c_11, params.c_12, seed=_seed[0], sweep0=_hb_sweep[0], nsweeps=1)
+    _hb_sweep[0] += 1
+    arr[:] = g.download()
+    return float(pl[0]), float(ac[0])
 
 
 def numerically_stable_matrix_exponential(Q):
