@@ -19,7 +19,8 @@ instead; ``batch`` the 64 uneven batched systems, ``serve`` the solve service
 under Poisson arrivals, ``precision`` the errors of the mixed-precision
 results against the FP64 oracle at 16^4, ``traj32`` a 32^4 Symanzik HMC
 trajectory in FP64 and FP16-mixed, ``heatbath`` Cabibbo-Marinari heatbath
-sweeps of the Symanzik action at 32^4 and 16^4.
+sweeps of the Symanzik action at 32^4 and 16^4, ``hex`` HEX smearing steps
+at 32^4 and 16^4.
 """
 import argparse
 import json
@@ -460,13 +461,86 @@ def bench_heatbath(args, dev):
     print(json.dumps(out))
 
 
+# One HEX step per site: level 1 makes 12 links (4 staple products, the
+# stout map G U^dagger, Q^2, Q^3 and exp(iQ) U: 8 SU(3) products each),
+# level 2 makes 12 (8 staple products + 4), level 3 makes 4 (12 + 4).
+HEX_FLOP_PER_SITE = (12 * 8 + 12 * 12 + 4 * 16) * 198
+# Compulsory bytes per site: level 1 reads U and writes 12 level-1 links,
+# level 2 reads those and U and writes 12 level-2 links, level 3 reads those
+# and U and writes U.
+HEX_BYTES_PER_SITE = 576 + 1728 + 1728 + 576 + 1728 + 1728 + 576 + 576
+
+
+def bench_hex(args, dev):
+    """HEX smearing steps (Smearing.cpp:304-311, alpha 0.95, 0.76, 0.38) of
+    32^4 device-resident links (links exp(i 0.6 H)), after `warmup` untimed
+    steps; 16^4 beside it.  The roofline is the FP64 rate of one step's three
+    level launches, timed with HIP events on the library stream."""
+    alpha = (0.95, 0.76, 0.38)
+    steps = max(args.steps, 4)
+    ctx = dev.Context(0)
+    if args.kernels_only:
+        dims = (32, 32, 32, 32)
+        g = dev.Gauge(ctx, dims, dev.random_links(dims, 0.6, seed=8))
+        g.hex_smear(alpha, 2)
+        g.time_hop("f32", False, 1)
+        ctx.synchronize()
+        log("[bench] hex kernels done")
+        return
+    res = {}
+    for dims in [(32, 32, 32, 32), (16, 16, 16, 16)]:
+        V = int(np.prod(dims))
+        g = dev.Gauge(ctx, dims, dev.random_links(dims, 0.6, seed=8))
+        g.hex_smear(alpha, max(args.warmup, 1))
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        g.hex_smear(alpha, steps)
+        dt = (time.perf_counter() - t0) / steps
+        ctx.set_profiling(True)
+        g.hex_smear(alpha, steps)
+        lv = {k: ctx.kernel_time(k) for k in ("hex1", "hex2", "hex3")}
+        ctx.set_profiling(False)
+        us = {k: ms * 1e3 / n for k, (ms, n) in lv.items()}
+        step_us = sum(us.values())
+        tfs = V * HEX_FLOP_PER_SITE / (step_us * 1e-6) / 1e12
+        res["x".join(map(str, dims))] = {
+            "ms_per_step": round(dt * 1e3, 3), "steps_per_s": round(1 / dt, 2),
+            "level_us": {k: round(v, 1) for k, v in us.items()}, "fp64_tflops": round(tfs, 2),
+            "compulsory_tb_s": round(V * HEX_BYTES_PER_SITE / (step_us * 1e-6) / 1e12, 2),
+            "avg_plaquette": g.avg_plaq()}
+        log(f"[bench] hex {dims}: {dt * 1e3:.3f} ms/step, levels {us}, {tfs:.2f} TF/s")
+    top = res["32x32x32x32"]
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as oc
+        threads = min(16, os.cpu_count() or 1)
+        oc.set_threads(threads)
+        dims = (32, 32, 32, 32)
+        U = dev.random_links(dims, 0.6, seed=8)
+        t0 = time.perf_counter()
+        oc.hex_smear(dims, U, alpha=alpha, nsteps=1)
+        tc = time.perf_counter() - t0
+        cpu = {"value": round(1.0 / tc, 4), "unit": "steps/s", "cores": threads, "kind": "port",
+               "sample": f"one oracle HEX step at 32^4 ({tc:.1f} s, OpenMP over sites)"}
+    out = _base_line(
+        top["steps_per_s"], "HEX steps/s", steps, top["ms_per_step"], "f64",
+        "32^4 HEX smearing step (alpha 0.95, 0.76, 0.38), links exp(i 0.6 H)",
+        {"metric": "HEX smearing steps/s (32^4)", "sizes": res,
+         "roofline": {"bound": "mfma", "achieved": top["fp64_tflops"], "peak": PEAK_FP64_TFS, "unit": "TFLOP/s",
+                      "frac": round(top["fp64_tflops"] / PEAK_FP64_TFS, 4),
+                      "traffic": traffic_from_profiles("hex_step", "32^4"),
+                      "kernel": "k_hex1 + k_hex2 + k_hex3 (one step, 304 x 198 flop per site)"},
+         "cpu_baseline": cpu})
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="16^4",
-                    choices=sorted(WORKLOADS) + ["batch", "serve", "precision", "traj32", "heatbath"])
+                    choices=sorted(WORKLOADS) + ["batch", "serve", "precision", "traj32", "heatbath", "hex"])
     ap.add_argument("--rates", default="0.5,1,2", help="serve: arrival rates as multiples of the measured "
                                                      "sequential FP64 capacity")
     ap.add_argument("--requests", type=int, default=200, help="serve: requests per arrival rate")
@@ -476,7 +550,7 @@ def main():
                          "the two steps of the mixed iteration), e.g. under a hardware-counter profiler; the first "
                          "use of the low-precision link copy also runs its conversion kernel once (reads 576 B, "
                          "writes 288 B (FP32) or 144 B (FP16) per site); for the heatbath workload, two 32^4 sweeps "
-                         "and one FP32 link conversion")
+                         "and one FP32 link conversion; for hex, two 32^4 steps and one FP32 link conversion")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a "
@@ -495,6 +569,8 @@ def main():
         return bench_traj32(args, dev)
     if args.workload == "heatbath":
         return bench_heatbath(args, dev)
+    if args.workload == "hex":
+        return bench_hex(args, dev)
 
     wl = WORKLOADS[args.workload]
     dims, mass = wl["dims"], wl["mass"]

@@ -289,8 +289,8 @@ int sfh_avg_plaq(sfh_ctx* ctx, const sfh_gauge* g, double* out);
 int sfh_polyakov_grid(sfh_ctx* ctx, const sfh_gauge* g, double* out);
 /* nsweeps Cabibbo-Marinari heatbath sweeps of the Symanzik gauge action
  * (beta, c11, c12) on the device-resident links.  Replaces heatbath_update
- * (Lattice.cpp:721-757) with single_link_heatbath (:697-719), the subgroup
- * updates R, S, T (:571-695) and the SU(2) sampler (SU3_Sampling.cpp:17-199).
+ * (Lattice.cpp:725-757) with single_link_heatbath (:696-723), the subgroup
+ * updates R, S, T (:570-694) and the SU(2) sampler (SU3_Sampling.cpp:17-199).
  * Every extent must be a multiple of 4 and beta > 0.  The sweep visits the
  * links direction by direction, each direction in four colour classes
  * (2 n_mu + sum of the other coordinates) mod 4 whose links are updated in
@@ -304,6 +304,18 @@ int sfh_polyakov_grid(sfh_ctx* ctx, const sfh_gauge* g, double* out);
  * them. */
 int sfh_heatbath(sfh_ctx* ctx, sfh_gauge* g, double beta, double c11, double c12, unsigned long long seed,
                  unsigned sweep0, int nsweeps, double* avg_plaq, double* action_density);
+/* nsteps HEX (hypercubic stout) smearing steps of the device-resident links
+ * in place.  Replaces Apply_HEX_smearing (Smearing.cpp:304-311) and, with
+ * nsteps = 4, Apply_4HEX_smearing (:313-318); the history variant (:320-331)
+ * is nsteps = 1 four times with a download in between.  alpha = {alpha1,
+ * alpha2, alpha3}, the reference's 0.95, 0.76, 0.38 (Smearing.h:7-9); each
+ * level maps V = exp(proj(alpha/2 Gamma U^dagger)) U as the reference writes
+ * it, proj the traceless anti-hermitian part.  The reference hands that
+ * anti-hermitian matrix to its exp(iQ) routine, which rejects it on any
+ * non-trivial links; here the exponential is exp(A) = exp(iQ), Q = -iA, and
+ * the level-3 staple sum starts at zero.  Non-finite links fail the call
+ * with SFH_E_INVALID (the links are then partly smeared). */
+int sfh_hex_smear(sfh_ctx* ctx, sfh_gauge* g, const double alpha[3], int nsteps);
 /* sum over links of ReTr(P^2).  Replaces compute_Tr_P2 (Fermions.cpp:662-678). */
 int sfh_tr_p2(sfh_ctx* ctx, const sfh_gauge* g, const double* P, double* out);
 /* U <- exp(i eps P) U on every device link (Fermions.cpp:703-723, exponential

@@ -640,9 +640,9 @@ int oc_polyakov_grid(const int* dims, const double* U, double* out) {
 }  // extern "C"
 
 // ------------------------------------------------------------------ heatbath
-// Cabibbo-Marinari heatbath with Symanzik staples: Lattice.cpp:507-757 and
+// Cabibbo-Marinari heatbath with Symanzik staples: Lattice.cpp:509-757 and
 // SU3_Sampling.cpp:4-199.  The reference draws its uniforms from a per-thread
-// std::mt19937 seeded from std::random_device (rng.cpp:9-27), which no run
+// std::mt19937 seeded from std::random_device (rng.cpp:8-27), which no run
 // can reproduce; here every uniform comes from a counter-based Philox4x32-10
 // stream (Salmon, Moraes, Dror, Shaw, SC'11) addressed by
 // (seed; draw pair, link, sweep, tag), in the order the reference's sampler
@@ -773,12 +773,12 @@ M2 su2_random(LinkStream& rs) {
   return su2_of(one);
 }
 
-// The subgroups in the reference's update order (Lattice.cpp:507-569,
-// :697-719): R = rows/columns (0, 1), S = (0, 2), T = (1, 2).
+// The subgroups in the reference's update order (Lattice.cpp:509-568,
+// :696-723): R = rows/columns (0, 1), S = (0, 2), T = (1, 2).
 constexpr int kSub[3][2] = {{0, 1}, {0, 2}, {1, 2}};
 
-// One subgroup update U <- embed(R) U (Lattice.cpp:571-695): the 2 x 2 block
-// of U A projected onto the quaternions (Lattice.cpp:571-610), heatbath
+// One subgroup update U <- embed(R) U (Lattice.cpp:570-694): the 2 x 2 block
+// of U A projected onto the quaternions (Lattice.cpp:570-612), heatbath
 // sample X at a = sqrt(det), R = X (block / a)^dagger; below a = 1e-14 a
 // Haar-random R.  Returns a and the drawn x0 (0 on the random branch).
 M3 subgroup_heatbath(const M3& U, const M3& A, double beta, int type, LinkStream& rs, double* a_out,
@@ -816,7 +816,7 @@ M3 subgroup_heatbath(const M3& U, const M3& A, double beta, int type, LinkStream
 
 inline uint32_t link_id(int s, int mu) { return uint32_t(s) * 4u + uint32_t(mu); }
 
-// Lattice.cpp:697-719: staples from the current links, then R, S, T; the
+// Lattice.cpp:696-723: staples from the current links, then R, S, T; the
 // link's plaquette and action terms ReTr(U A1) / 3 and -(beta/3) ReTr(U A).
 // A link whose sampler ran out of tries keeps its value, contributes 0 and
 // returns false.
@@ -849,7 +849,7 @@ inline int link_colour(const Geo& g, int s, int mu) {
   return (n[0] + n[1] + n[2] + n[3] + n[mu]) & 3;
 }
 
-// One sweep, Lattice.cpp:721-757.  order 0: the reference's order (sites
+// One sweep, Lattice.cpp:725-757.  order 0: the reference's order (sites
 // lexicographic, direction innermost), serial.  order 1: direction by
 // direction, each in 4 colour classes, every class in parallel (the device
 // order).  Returns the reference's (average plaquette, action density).
@@ -946,6 +946,137 @@ int oc_heatbath(const int* dims, double* U, double beta, double c11, double c12,
     if (act) act[k] = a;
   }
   return ok ? OK : E_SAMPLER;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ HEX smearing
+// Hypercubic stout smearing, Smearing.cpp:76-331 (three nested stout levels,
+// Smearing.h:7-9 alpha1 = 0.95, alpha2 = 0.76, alpha3 = 0.38).  Two places
+// where the reference cannot run are restated as it intends:
+//   * each level maps V = exp(proj(alpha/2 Gamma U^dagger)) U with proj the
+//     traceless anti-hermitian part (Smearing.cpp:126-129, :143-145); the reference
+//     passes that anti-hermitian matrix to its exp(iQ) routine for hermitian
+//     Q (Fermions.cpp:843-902), whose guards reject it on any non-trivial
+//     links; here exp(A) = exp(iQ) with Q = -iA;
+//   * the level-3 staple sum starts at zero (Smearing.cpp:243 leaves it
+//     uninitialised).
+// The level-1 link V1[mu][nu][rho] depends on (nu, rho) only through the
+// one direction sigma outside {mu, nu, rho} that its staples use
+// (Smearing.cpp:89), so it is stored once per (mu, sigma): 12 per site.
+// Likewise V2[mu][nu] (mu != nu) excludes nu: 12 per site.  Entries with
+// repeated indices, which the reference also computes, feed nothing.
+namespace {
+
+// Slot of an ordered pair of distinct directions (a, b): 3 a + (b < a ? b : b - 1).
+inline int pair_slot(int a, int b) { return 3 * a + (b < a ? b : b - 1); }
+
+inline M3 antiherm_traceless(const M3& m) {
+  M3 a = scale(0.5, sub(m, dag(m)));
+  const cd t = trace(a) / 3.0;
+  a.a[0] -= t; a.a[4] -= t; a.a[8] -= t;
+  return a;
+}
+
+// exp(proj(alpha/2 Gamma U^dagger)) U (Smearing.cpp:126-129, :143-145, :223-225, :291-293).
+int stout_step(const M3& Gamma, const M3& U, double alpha, M3& out) {
+  const M3 A = scale(alpha * 0.5, antiherm_traceless(mul(Gamma, dag(U))));
+  if (!std::isfinite(fnorm(A))) return E_INVALID;
+  M3 e;
+  const int st = exp_iq(scale(cd(0.0, -1.0), A), e);
+  if (st != OK) return st;
+  out = mul(e, U);
+  return OK;
+}
+
+// Up plus down staple of link (s, mu) in the (mu, sig) plane, paths from s to
+// s + mu: L(s, d) is the link of direction d at site s for the pair at hand.
+template <class Link>
+M3 staple_pair(const Geo& g, int s, int mu, int sig, Link L) {
+  const int sp = g.step(s, sig, +1), smu = g.step(s, mu, +1);
+  const int sm = g.step(s, sig, -1), smmu = g.step(sm, mu, +1);
+  const M3 up = mul(mul(L(s, sig), L(sp, mu)), dag(L(smu, sig)));
+  const M3 dn = mul(mul(dag(L(sm, sig)), L(sm, mu)), L(smmu, sig));
+  return add(up, dn);
+}
+
+int hex_step(const Geo& g, double* U, const double alpha[3]) {
+  const size_t V = g.V;
+  std::vector<M3> V1(12 * V), V2(12 * V);
+  int status = OK;
+  // level 1 (Smearing.cpp:76-123, :132-155): V1 of direction mu, staples in sig
+#pragma omp parallel for collapse(2)
+  for (int s = 0; s < g.V; ++s)
+    for (int k = 0; k < 12; ++k) {
+      const int mu = k / 3, r = k % 3, sig = r < mu ? r : r + 1;
+      auto thin = [&](int x, int d) { return load_link(U, x, d); };
+      const M3 G = staple_pair(g, s, mu, sig, thin);
+      const int st = stout_step(G, load_link(U, s, mu), alpha[2], V1[12 * s + k]);
+      if (st) {
+#pragma omp critical
+        status = st;
+      }
+    }
+  if (status) return status;
+  // level 2 (Smearing.cpp:157-234): V2 of direction mu excluding nu, staples
+  // in sig outside {mu, nu} built from level-1 links that exclude the fourth
+  // direction rho
+#pragma omp parallel for collapse(2)
+  for (int s = 0; s < g.V; ++s)
+    for (int k = 0; k < 12; ++k) {
+      const int mu = k / 3, r = k % 3, nu = r < mu ? r : r + 1;
+      M3 G = zero3();
+      for (int sig = 0; sig < 4; ++sig) {
+        if (sig == mu || sig == nu) continue;
+        const int rho = 6 - mu - nu - sig;
+        auto lvl1 = [&](int x, int d) { return V1[12 * x + pair_slot(d, rho)]; };
+        G = add(G, staple_pair(g, s, mu, sig, lvl1));
+      }
+      const int st = stout_step(G, load_link(U, s, mu), alpha[1], V2[12 * s + k]);
+      if (st) {
+#pragma omp critical
+        status = st;
+      }
+    }
+  if (status) return status;
+  // level 3 (Smearing.cpp:236-302): staples in every nu != mu from level-2
+  // links that exclude the other direction of their plane
+  std::vector<M3> out(4 * V);
+#pragma omp parallel for collapse(2)
+  for (int s = 0; s < g.V; ++s)
+    for (int mu = 0; mu < 4; ++mu) {
+      M3 G = zero3();
+      for (int nu = 0; nu < 4; ++nu) {
+        if (nu == mu) continue;
+        auto lvl2 = [&](int x, int d) { return V2[12 * x + pair_slot(d, d == mu ? nu : mu)]; };
+        G = add(G, staple_pair(g, s, mu, nu, lvl2));
+      }
+      const int st = stout_step(G, load_link(U, s, mu), alpha[0], out[4 * s + mu]);
+      if (st) {
+#pragma omp critical
+        status = st;
+      }
+    }
+  if (status) return status;
+  for (int s = 0; s < g.V; ++s)
+    for (int mu = 0; mu < 4; ++mu) store_link(U, s, mu, out[4 * s + mu]);
+  return OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// nsteps HEX smearing steps in place (Apply_HEX_smearing, Smearing.cpp:304-311;
+// Apply_4HEX_smearing :313-318 is nsteps = 4); alpha = {alpha1, alpha2, alpha3}.
+int oc_hex_smear(const int* dims, double* U, const double* alpha, int nsteps) {
+  if (nsteps < 0) return E_INVALID;
+  for (int k = 0; k < 4; ++k)
+    if (dims[k] < 2) return E_INVALID;
+  Geo g(dims);
+  for (int n = 0; n < nsteps; ++n)
+    if (int st = hex_step(g, U, alpha)) return st;
+  return OK;
 }
 
 }  // extern "C"

@@ -55,6 +55,7 @@ def lib():
             "oc_heatbath_subgroup": (_I, [_P, _P, _D, _I, ctypes.c_ulonglong, ctypes.c_uint, ctypes.c_uint, _P,
                                           _P]),
             "oc_heatbath": (_I, [_P, _P, _D, _D, _D, ctypes.c_ulonglong, ctypes.c_uint, _I, _I, _P, _P]),
+            "oc_hex_smear": (_I, [_P, _P, _P, _I]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -243,7 +244,7 @@ def heatbath_subgroup(U18, A18, beta, sub, seed=1, link=0, sweep=0):
 
 
 def heatbath(dims, U, beta, c11, c12, seed, sweep0=0, nsweeps=1, order=1):
-    """nsweeps Cabibbo-Marinari sweeps (Lattice.cpp:721-757).  order 1 is
+    """nsweeps Cabibbo-Marinari sweeps (Lattice.cpp:725-757).  order 1 is
     the device's colour order, 0 the reference's lexicographic order.
     Returns (U', per-sweep average plaquette, per-sweep action density)."""
     d = np.ascontiguousarray(np.asarray(dims, dtype=np.int32))
@@ -254,3 +255,17 @@ def heatbath(dims, U, beta, c11, c12, seed, sweep0=0, nsweeps=1, order=1):
     if st:
         raise OracleError(st, "heatbath")
     return U, pl, ac
+
+
+HEX_ALPHA = (0.95, 0.76, 0.38)  # alpha1, alpha2, alpha3 of Smearing.h:7-9
+
+
+def hex_smear(dims, U, alpha=HEX_ALPHA, nsteps=1):
+    """nsteps HEX smearing steps (Smearing.cpp:76-318); returns new links."""
+    d = _dims(dims)
+    U = _f64(U).copy()
+    a = np.ascontiguousarray(np.asarray(alpha, dtype=np.float64))
+    st = lib().oc_hex_smear(_p(d), _p(U), _p(a), int(nsteps))
+    if st:
+        raise OracleError(st, "hex_smear")
+    return U

@@ -105,6 +105,7 @@ SIGNATURES = {
     "sfh_tr_p2": (_I, [_P, _P, _P, ctypes.POINTER(_D)]),
     "sfh_polyakov_grid": (_I, [_P, _P, _P]),
     "sfh_heatbath": (_I, [_P, _P, _D, _D, _D, ctypes.c_ulonglong, ctypes.c_uint, _I, _P, _P]),
+    "sfh_hex_smear": (_I, [_P, _P, _P, _I]),
     "sfh_link_update": (_I, [_P, _P, _D, _P, _I]),
     "sfh_exp_iq": (_I, [_P, _P, _P]),
     "sfh_leapfrog": (_I, [_P, _P, ctypes.POINTER(MDParams), _P, _P, _P]),

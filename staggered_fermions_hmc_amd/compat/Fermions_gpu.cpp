@@ -311,7 +311,7 @@ double compute_avg_plaq(const Link_array& arr) {
   return out;
 }
 
-// Lattice.cpp:721-757.
+// Lattice.cpp:725-757.
 std::pair<double, double> heatbath_update(Link_array& arr, double beta) {
   sfh_gauge* g = gauge_from(arr);
   double plaq = 0.0, act = 0.0;
@@ -320,6 +320,32 @@ std::pair<double, double> heatbath_update(Link_array& arr, double beta) {
         "heatbath_update");
   check(sfh_gauge_download(ctx(), g, arr.data()), "sfh_gauge_download");
   return {plaq, act};
+}
+
+static void hex_steps(Link_array& U_array, int n) {
+  sfh_gauge* g = gauge_from(U_array);
+  const double alpha[3] = {config().alpha1, config().alpha2, config().alpha3};
+  check(sfh_hex_smear(ctx(), g, alpha, n), "Apply_HEX_smearing");
+  check(sfh_gauge_download(ctx(), g, U_array.data()), "sfh_gauge_download");
+}
+
+// Smearing.cpp:304-311.
+void Apply_HEX_smearing(Link_array& U_array) { hex_steps(U_array, 1); }
+
+// Smearing.cpp:313-318.
+void Apply_4HEX_smearing(Link_array& U_array) { hex_steps(U_array, 4); }
+
+// Smearing.cpp:320-331: the thin links and each of the four smearing levels.
+std::vector<Link_array> Apply_4HEX_smearing_with_history(const Link_array& U_thin) {
+  std::vector<Link_array> history{U_thin};
+  sfh_gauge* g = gauge_from(U_thin);
+  const double alpha[3] = {config().alpha1, config().alpha2, config().alpha3};
+  for (int step = 0; step < 4; ++step) {
+    check(sfh_hex_smear(ctx(), g, alpha, 1), "Apply_4HEX_smearing_with_history");
+    history.emplace_back(U_thin.size());
+    check(sfh_gauge_download(ctx(), g, history.back().data()), "sfh_gauge_download");
+  }
+  return history;
 }
 
 static double uniform01() { return std::uniform_real_distribution<double>(0.0, 1.0)(ts.uniform); }

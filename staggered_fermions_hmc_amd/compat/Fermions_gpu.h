@@ -2,7 +2,7 @@
 // functions, implemented on the MI355X library through the C ABI (sfh.h).
 //
 // Same names, signatures, argument meaning, ownership and exceptions as the
-// reference (Fermions.h:31-70, :105, :130; Lattice.h:75-77):
+// reference (Fermions.h:31-70, :105, :130; Lattice.h:75-77; Smearing.h:29-31):
 //   - inputs are const& host vectors in the reference layouts, results are
 //     returned by value with exact sizes 6V or 4V*18;
 //   - apply_DDdagger writes into the caller's pre-sized out_field;
@@ -65,3 +65,11 @@ double compute_avg_plaq(const Link_array& arr);
 // the device's colour order, so a sweep differs from the reference's
 // lexicographic one draw for draw but samples the same distribution.
 std::pair<double, double> heatbath_update(Link_array& arr, double beta);
+
+// HEX smearing in place with alpha1..3 of the configuration (Smearing.h:7-9),
+// each level V = exp(proj(alpha/2 Gamma U^dagger)) U as the reference writes
+// it; the exponential is exp of that anti-hermitian matrix (the reference's
+// exp(iQ) routine rejects it) and the level-3 sum starts at zero.
+void Apply_HEX_smearing(Link_array& U_array);
+void Apply_4HEX_smearing(Link_array& U_array);
+std::vector<Link_array> Apply_4HEX_smearing_with_history(const Link_array& U_thin);

@@ -187,7 +187,7 @@ static int run() {
     report("HMC update keeps links unitary", err < 1e-12, err);
   }
 
-  // heatbath sweeps (Lattice.cpp:721-757) from a cold start thermalise the
+  // heatbath sweeps (Lattice.cpp:725-757) from a cold start thermalise the
   // links and keep them in SU(3); the returned averages are those of the
   // reference (plaquette normalised to 1 on a cold lattice)
   {
@@ -210,6 +210,36 @@ static int run() {
     report("heatbath plaquette matches compute_avg_plaq / 6 (10%)",
            std::abs(compute_avg_plaq(W) / 6.0 - r.first) < 0.1 * r.first, compute_avg_plaq(W) / 6.0 - r.first);
     report("heatbath action density negative", r.second < 0, r.second);
+  }
+
+  // HEX smearing (Smearing.cpp:304-331): cold links are a fixed point, the
+  // history holds the thin links and four levels, smeared links stay in SU(3)
+  {
+    Link_array C = cold();
+    Apply_4HEX_smearing(C);
+    double d = 0;
+    const Link_array C0 = cold();
+    for (size_t k = 0; k < C.size(); ++k) d = std::max(d, std::abs(C[k] - C0[k]));
+    report("4HEX leaves cold links unchanged", d == 0.0, d);
+    const Link_array H = hot(0.4, 3);
+    const auto hist = Apply_4HEX_smearing_with_history(H);
+    Link_array W = H;
+    Apply_4HEX_smearing(W);
+    double dh = 0;
+    for (size_t k = 0; k < W.size(); ++k) dh = std::max(dh, std::abs(W[k] - hist[4][k]));
+    report("4HEX history: 5 levels, last = Apply_4HEX_smearing", hist.size() == 5 && hist[0] == H && dh == 0.0, dh);
+    double err = 0;
+    for (size_t l = 0; l < W.size() / 18; ++l) {
+      std::complex<double> m[9];
+      for (int k = 0; k < 9; ++k) m[k] = {W[18 * l + 2 * k], W[18 * l + 2 * k + 1]};
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          std::complex<double> s = 0;
+          for (int k = 0; k < 3; ++k) s += m[3 * i + k] * std::conj(m[3 * j + k]);
+          err = std::max(err, std::abs(s - (i == j ? 1.0 : 0.0)));
+        }
+    }
+    report("4HEX keeps links unitary", err < 1e-12, err);
   }
 
   std::printf("%s: %d failure(s)\n", failures ? "FAIL" : "PASS", failures);

@@ -15,6 +15,9 @@ struct Config {
   int max_iter = 50000;              // Parameters.h:24
   double c_11 = 5.0 / 3.0;           // Parameters.h:51
   double c_12 = -1.0 / 12.0;         // Parameters.h:52
+  double alpha1 = 0.95;              // Smearing.h:7
+  double alpha2 = 0.76;              // Smearing.h:8
+  double alpha3 = 0.38;              // Smearing.h:9
   int precision = 0;                 // 0 FP64 (reference), 1 FP32 inner, 2 FP16 inner
   int device = 0;
   unsigned long long seed = 0x5f3759dfULL;  // Philox key of the field generators

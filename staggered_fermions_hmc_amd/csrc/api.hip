@@ -764,6 +764,15 @@ int sfh_heatbath(sfh_ctx* c, sfh_gauge* g, double beta, double c11, double c12, 
   return heatbath(c, g->g, g->U, beta, c11, c12, seed, sweep0, nsweeps, avg_plaq, action_density);
 }
 
+int sfh_hex_smear(sfh_ctx* c, sfh_gauge* g, const double alpha[3], int nsteps) {
+  if (!c || !g || !alpha || nsteps < 0) return SFH_E_INVALID;
+  for (int k = 0; k < 3; ++k)
+    if (!std::isfinite(alpha[k])) return SFH_E_INVALID;
+  Guard gd(c);
+  invalidate_lp(g);
+  return hex_smear(c, g->g, g->U, alpha, nsteps);
+}
+
 int sfh_tr_p2(sfh_ctx* c, const sfh_gauge* g, const double* P, double* out) {
   if (!c || !g || !P || !out) return SFH_E_INVALID;
   Guard gd(c);

@@ -7,6 +7,7 @@
 #include "internal.h"
 #include "reduce.h"
 #include "staples.h"
+#include "su3exp.h"
 
 namespace sfh {
 
@@ -42,11 +43,12 @@ __global__ __launch_bounds__(128) void k_force(Geom g, const double2* __restrict
                                                double coef, int nb) {
   // the four directions of one site range run as consecutive workgroups of
   // one XCD, so the links around those sites are read from HBM once and
-  // from that XCD's L2 for the other directions
+  // from that XCD's L2 for the other directions; the ranges follow the
+  // tiled column walk, so x0- and x1-neighbour ranges run close in time
   const int lb = xcd_block(nb);
   const int mu = lb & 3;
-  const int i = (lb >> 2) * blockDim.x + threadIdx.x;
-  if (i >= g.V) return;
+  const int i = range_site(g, lb >> 2, threadIdx.x);
+  if (i < 0) return;
   const Site4 n = site_of(g, i);
   M f = mzero<double>();
   if (which & SFH_FORCE_GAUGE) {
@@ -101,8 +103,9 @@ __global__ __launch_bounds__(128) void k_force(Geom g, const double2* __restrict
 int force_field(sfh_ctx* c, const Geom& g, const double2* U, const double2* X, const double2* Y, double beta,
                 double c11, double c12, unsigned which, double2* F, double2* P, double coef) {
   KTimer kt(c, "force");
-  const int nb = 4 * ((g.V + 127) / 128);
-  hipLaunchKernelGGL(k_force, dim3(nb), dim3(128), 0, c->stream, g, U, X, Y, beta, c11, c12, which, F, P, coef, nb);
+  const int nb = 4 * ((g.Vh + kRangeHalf - 1) / kRangeHalf);
+  hipLaunchKernelGGL(k_force, dim3(nb), dim3(2 * kRangeHalf), 0, c->stream, g, U, X, Y, beta, c11, c12, which, F, P,
+                     coef, nb);
   return hip_check(c, hipGetLastError(), "force kernel launch");
 }
 
@@ -184,67 +187,6 @@ int polyakov_grid(sfh_ctx* c, const Geom& g, const double2* U, double* out) {
 }
 
 // ------------------------------------------------------------------ exponential and link update
-// exp(iQ) for hermitian traceless Q by the Cayley-Hamilton closed form of
-// Morningstar & Peardon (hep-lat/0311018), with the guards and small-Q branch
-// of Fermions.cpp:843-902.  Returns a status code.
-__device__ int exp_iq(const M& Q, M& out) {
-  const M Q2 = mmul(Q, Q);
-  const M Q3 = mmul(Q2, Q);
-  const C t3 = mtrace(Q3);
-  if (fabs(t3.im) > 1e-10) return SFH_E_EXP_QCUBED;
-  double c0 = t3.re / 3.0;
-  const double c1 = mtrace(Q2).re / 2.0;
-  if (c1 < 0.0) return SFH_E_EXP_NEG_TRACE;
-  if (c1 < 1e-12) {
-    M r = meye<double>();
-#pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      r.e[k].re += -Q.e[k].im - 0.5 * Q2.e[k].re;
-      r.e[k].im += Q.e[k].re - 0.5 * Q2.e[k].im;
-    }
-    out = r;
-    return SFH_OK;
-  }
-  const bool neg = c0 < 0.0;
-  c0 = fabs(c0);
-  const double c0max = 2.0 * sqrt(c1 * c1 * c1 / 27.0);
-  double ratio = c0max > 0.0 ? c0 / c0max : 1.0;
-  ratio = fmin(fmax(ratio, -1.0), 1.0);
-  const double theta = acos(ratio);
-  const double w = sqrt(c1) * sin(theta / 3.0);
-  const double u = sqrt(c1 / 3.0) * cos(theta / 3.0);
-  double s2u, c2u, su, cu;
-  sincos(2.0 * u, &s2u, &c2u);
-  sincos(u, &su, &cu);
-  const C e2u = {c2u, s2u}, emu = {cu, -su};
-  double xi;
-  if (fabs(w) <= 0.05) {
-    const double w2 = w * w;
-    xi = 1.0 - (w2 / 6.0) * (1.0 - (w2 / 20.0) * (1.0 - w2 / 42.0));
-  } else {
-    xi = sin(w) / w;
-  }
-  const double cw = cos(w);
-  const C h0 = (u * u - w * w) * e2u + emu * C{8.0 * u * u * cw, 2.0 * u * (3.0 * u * u + w * w) * xi};
-  const C h1 = (2.0 * u) * e2u - emu * C{2.0 * u * cw, -(3.0 * u * u - w * w) * xi};
-  const C h2 = e2u - emu * C{cw, 3.0 * u * xi};
-  const double den = 9.0 * u * u - w * w;
-  C f0 = {h0.re / den, h0.im / den}, f1 = {h1.re / den, h1.im / den}, f2 = {h2.re / den, h2.im / den};
-  if (neg) {
-    f0 = conj(f0);
-    f1 = {-f1.re, f1.im};
-    f2 = conj(f2);
-  }
-  M r;
-#pragma unroll
-  for (int k = 0; k < 9; ++k) r.e[k] = f1 * Q.e[k] + f2 * Q2.e[k];
-  r.e[0] = r.e[0] + f0;
-  r.e[4] = r.e[4] + f0;
-  r.e[8] = r.e[8] + f0;
-  out = r;
-  return SFH_OK;
-}
-
 __global__ __launch_bounds__(128) void k_link_update(Geom g, double2* __restrict__ U, const double2* __restrict__ P,
                                                      double eps, int check, int* status) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;

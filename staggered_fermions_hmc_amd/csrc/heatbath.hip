@@ -1,5 +1,5 @@
 // Cabibbo-Marinari heatbath of the Symanzik gauge action on device-resident
-// links (Lattice.cpp:507-757, SU3_Sampling.cpp:17-199).
+// links (Lattice.cpp:509-757, SU3_Sampling.cpp:17-199).
 //
 // A sweep updates the links one direction mu at a time, each direction in
 // four colour classes (2 n_mu + sum of the other coordinates) & 3.  The
@@ -8,7 +8,7 @@
 // 1 or 3 (mod 4): a class is a set of links none of whose staples holds
 // another, so one launch updates a whole class, one lane per link, in place.
 // Every extent must be a multiple of 4.  Each link update is the reference's
-// single_link_heatbath (Lattice.cpp:697-719): staple sums A1, A2 from the
+// single_link_heatbath (Lattice.cpp:696-723): staple sums A1, A2 from the
 // current links, A = c11 A1 + c12 A2, then the SU(2) subgroups (0,1), (0,2),
 // (1,2) in turn, then the link's ReTr(U A1) / 3 and -(beta/3) ReTr(U A).
 //
@@ -144,7 +144,7 @@ __device__ Quat su2_random(Draws& rs) {
 }
 
 // One subgroup update U <- embed(R) U on rows/columns (I, J)
-// (Lattice.cpp:571-695): the (I, J) block of U A projected onto the
+// (Lattice.cpp:570-694): the (I, J) block of U A projected onto the
 // quaternions, b0 = Re tr B / 2, b_l = Im tr(B sigma_l) / 2; a = |b|; below
 // a = 1e-14 R is Haar-random, else R = X (b / a)^dagger with X the heatbath
 // sample at (a, 2 beta / 3).

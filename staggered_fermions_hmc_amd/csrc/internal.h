@@ -183,6 +183,9 @@ bool heatbath_dims_ok(const Geom& g);
 int heatbath(sfh_ctx* c, const Geom& g, double2* U, double beta, double c11, double c12, unsigned long long seed,
              unsigned sweep0, int nsweeps, double* plaq, double* act);
 
+// ------------------------------------------------------------------ smearing (smear.hip)
+int hex_smear(sfh_ctx* c, const Geom& g, double2* U, const double alpha[3], int nsteps);
+
 // ------------------------------------------------------------------ solver (solver.cpp)
 int cg_solve(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* p, const double2* phi, double2* x,
              sfh_cg_stats* stats);

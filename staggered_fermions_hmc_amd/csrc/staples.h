@@ -35,6 +35,18 @@ struct EoIndex {
   __device__ __forceinline__ static int of(const Geom& g, const Site4& x) { return x.index(g); }
 };
 
+// Device site of lane t (0 .. 2 kRangeHalf - 1) of site range r: kRangeHalf
+// checkerboard sites from the tiled column walk (range_start) on each parity,
+// the lower half of the lanes even, the upper half odd; -1 past the end.
+// Neighbouring ranges in x0 and x1 come close together in the walk, so
+// kernels that read links around their sites find them in L2.
+constexpr int kRangeHalf = 64;
+__device__ __forceinline__ int range_site(const Geom& g, int r, int t) {
+  const int cb = range_start(g, r, kRangeHalf) + (t & (kRangeHalf - 1));
+  if (cb >= g.Vh) return -1;
+  return (t >= kRangeHalf ? g.Vh : 0) + cb;
+}
+
 __device__ __forceinline__ Site4 site_of(const Geom& g, int i) {
   const int p = i >= g.Vh;
   int n[4];

@@ -213,7 +213,7 @@ class Gauge:
 
     def heatbath(self, beta, c11, c12, seed, sweep0=0, nsweeps=1):
         """nsweeps Cabibbo-Marinari heatbath sweeps of the Symanzik action on
-        the device-resident links (heatbath_update, Lattice.cpp:721-757),
+        the device-resident links (heatbath_update, Lattice.cpp:725-757),
         sweep indices sweep0, sweep0 + 1, ...  Returns the per-sweep
         (average plaquette, action density) arrays the reference returns."""
         pl = np.empty(max(int(nsweeps), 1))
@@ -221,6 +221,13 @@ class Gauge:
         self.ctx.check(self.ctx._lib.sfh_heatbath(self.ctx.h, self.h, float(beta), float(c11), float(c12), int(seed),
                                                   int(sweep0), int(nsweeps), ptr(pl), ptr(ac)))
         return pl[:nsweeps], ac[:nsweeps]
+
+    def hex_smear(self, alpha=(0.95, 0.76, 0.38), nsteps=1):
+        """nsteps HEX smearing steps of the device-resident links in place
+        (Apply_HEX_smearing, Smearing.cpp:304-311); alpha = (alpha1, alpha2,
+        alpha3) as in Smearing.h:7-9."""
+        a = f64(np.asarray(alpha, dtype=np.float64))
+        self.ctx.check(self.ctx._lib.sfh_hex_smear(self.ctx.h, self.h, ptr(a), int(nsteps)))
 
     def tr_p2(self, P):
         a = f64(P)

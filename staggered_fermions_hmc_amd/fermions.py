@@ -2,7 +2,7 @@
 MI355X library.
 
 Same function names, argument meaning and error behaviour as the reference
-(Fermions.h:31-70, :105, :130; Lattice.h:75-77); host arrays are float64 in
+(Fermions.h:31-70, :105, :130; Lattice.h:75-77; Smearing.h:29-31); host arrays are float64 in
 the reference layouts.  The reference's compile-time constants
 (Parameters.h:9-24, :51-52) are the fields of :class:`Parameters`, set once
 with :func:`configure`.  Each call hands its host arrays to the device and
@@ -28,6 +28,9 @@ class Parameters:
     max_iter: int = 50000
     c_11: float = 5.0 / 3.0
     c_12: float = -1.0 / 12.0
+    alpha1: float = 0.95  # Smearing.h:7-9
+    alpha2: float = 0.76
+    alpha3: float = 0.38
     precision: str = "f64"
 
     @property
@@ -148,7 +151,7 @@ _hb_sweep = [0]
 
 def heatbath_update(arr, beta):
     """One Cabibbo-Marinari heatbath sweep of the Symanzik action in place
-    (Lattice.cpp:721-757); returns (average plaquette, action density).  The
+    (Lattice.cpp:725-757); returns (average plaquette, action density).  The
     links are visited in the device's colour order; sweeps take consecutive
     sweep indices of the Philox streams."""
     g = _gauge(arr)
@@ -156,6 +159,32 @@ def heatbath_update(arr, beta):
     _hb_sweep[0] += 1
     arr[:] = g.download()
     return float(pl[0]), float(ac[0])
+
+
+def _hex(arr, nsteps):
+    g = _gauge(arr)
+    g.hex_smear((params.alpha1, params.alpha2, params.alpha3), nsteps)
+    return g
+
+
+def Apply_HEX_smearing(U_array):
+    """One HEX smearing step in place (Smearing.cpp:304-311)."""
+    U_array[:] = _hex(U_array, 1).download()
+
+
+def Apply_4HEX_smearing(U_array):
+    """Four HEX smearing steps in place (Smearing.cpp:313-318)."""
+    U_array[:] = _hex(U_array, 4).download()
+
+
+def Apply_4HEX_smearing_with_history(U_thin):
+    """The thin links and each of four HEX levels (Smearing.cpp:320-331)."""
+    g = _gauge(U_thin)
+    out = [np.array(U_thin, dtype=np.float64, copy=True)]
+    for _ in range(4):
+        g.hex_smear((params.alpha1, params.alpha2, params.alpha3), 1)
+        out.append(g.download())
+    return out
 
 
 def numerically_stable_matrix_exponential(Q):

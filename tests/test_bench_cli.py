@@ -19,7 +19,7 @@ def test_help_lists_workloads():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--help"], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 0
-    for w in ("16^4", "32^4", "batch", "serve", "precision", "traj32", "heatbath", "--kernels-only"):
+    for w in ("16^4", "32^4", "batch", "serve", "precision", "traj32", "heatbath", "hex", "--kernels-only"):
         assert w in r.stdout
 
 

@@ -90,6 +90,19 @@ def test_leapfrog_and_hmc_update_match_oracle():
         assert rel(G, Uh) < 1e-12
 
 
+def test_hex_smearing_entry_points():
+    U = dev.random_links(DIMS, 0.5, seed=11)
+    alpha = (fm.params.alpha1, fm.params.alpha2, fm.params.alpha3)
+    W = U.copy()
+    fm.Apply_HEX_smearing(W)
+    assert np.abs(W - oc.hex_smear(DIMS, U, alpha=alpha, nsteps=1)).max() < 1e-12
+    W = U.copy()
+    fm.Apply_4HEX_smearing(W)
+    assert np.abs(W - oc.hex_smear(DIMS, U, alpha=alpha, nsteps=4)).max() < 1e-12
+    hist = fm.Apply_4HEX_smearing_with_history(U)
+    assert len(hist) == 5 and np.array_equal(hist[0], U) and np.array_equal(hist[4], W)
+
+
 def test_heatbath_update_in_place():
     U = np.zeros(int(np.prod(DIMS)) * 72)
     U[0::18] = U[8::18] = U[16::18] = 1.0

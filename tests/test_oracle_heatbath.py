@@ -1,7 +1,7 @@
-"""Pins of the heatbath oracle (Lattice.cpp:507-757, SU3_Sampling.cpp:17-199).
+"""Pins of the heatbath oracle (Lattice.cpp:509-757, SU3_Sampling.cpp:17-199).
 
 The reference ships no test of its heatbath, and its uniforms come from
-std::random_device seeds (rng.cpp:9-27), so no run of it can be replayed.
+std::random_device seeds (rng.cpp:8-27), so no run of it can be replayed.
 The oracle is pinned instead by what the mathematics fixes:
   * Philox4x32-10 against the published Random123 known-answer vectors;
   * the x0 sampler against its exact law sqrt(1 - x^2) exp(k x): mean
