@@ -378,6 +378,14 @@ def bench_traj32(args, dev):
         traj[prec] = {"seconds": round(dt, 4), "delta_H": float(-H[6]), "plaquette": g.avg_plaq()}
         log(f"[bench] traj32: {prec} trajectory {dt:.3f}s dH {-H[6]:.6f}")
     dF = float(np.max(np.abs(F["f16"] - F["f64"])))
+    # the Symanzik gauge force kernel alone (HIP events on the library stream)
+    ctx.set_profiling(True)
+    for _ in range(3):
+        g.force(4.0, 5.0 / 3.0, -1.0 / 12.0, which=1)
+    fms, fn = ctx.kernel_time("force")
+    ctx.set_profiling(False)
+    force_ms = fms / fn
+    force_tfs = int(np.prod(dims)) * 4 * 85 * 198 / (force_ms * 1e-3) / 1e12
     print(json.dumps(_base_line(
         round(1.0 / traj["f16"]["seconds"], 4), "trajectories/s", 1, round(traj["f16"]["seconds"] * 1e3, 1),
         "f16 inner / f64 reliable updates, reductions, forces and Metropolis",
@@ -387,7 +395,8 @@ def bench_traj32(args, dev):
          "delta_H_difference": abs(traj["f16"]["delta_H"] - traj["f64"]["delta_H"]),
          "plaquette_difference": abs(traj["f16"]["plaquette"] - traj["f64"]["plaquette"]),
          "force_linf_difference": dF, "force_linf": float(np.max(np.abs(F["f64"]))),
-         "force_rel_linf_difference": dF / float(np.max(np.abs(F["f64"])))})))
+         "force_rel_linf_difference": dF / float(np.max(np.abs(F["f64"]))),
+         "gauge_force_kernel_ms": round(force_ms, 3), "gauge_force_fp64_tflops": round(force_tfs, 2)})))
 
 
 # One link update of the heatbath: 12 plaquette and 72 rectangle staple
