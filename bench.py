@@ -693,6 +693,9 @@ def main():
                         "algorithmic_bytes_per_launch": HOP_BYTES_PER_SITE["f64"] * V},
         "timing": "HIP events on the library stream around 50 back-to-back launches of each kernel as the solver "
                   "runs it at this lattice",
+        "bytes_note": "algorithmic bytes count the full 18-real links (72 link reals per site); the links here are "
+                      "SU(3), so the low-precision kernels read two rows per link and rebuild the third, and their "
+                      "measured traffic (traffic) is below the algorithmic bytes",
     }
 
     cpu = None

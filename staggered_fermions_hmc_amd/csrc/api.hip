@@ -324,10 +324,11 @@ int sfh_tune_hop(sfh_ctx* c, const sfh_gauge* g, int precision, int n, int* vari
       if (precision == 0)
         hop_f64(c->stream, gg, g->U, (const double2*)a, (double2*)b, 0.1, 1.0, nullptr, nullptr, nullptr);
       else if (precision == 1)
-        hop_f32(c->stream, gg, g->lp.u32, (const float2*)a, (float2*)b, 0.1f, 1.0f, nullptr, nullptr, nullptr);
+        hop_f32(c->stream, gg, g->lp.u32, (const float2*)a, (float2*)b, 0.1f, 1.0f, nullptr, nullptr, nullptr,
+                g->lp_su3);
       else
         hop_f16(c->stream, gg, g->lp.u16, (const __half2*)a, na, (__half2*)b, nb, 0.1f, 1.0f, nullptr, nullptr,
-                nullptr, nullptr);
+                nullptr, nullptr, g->lp_su3);
     };
     launch();
     hipEventRecord(c->ev0, c->stream);
@@ -385,10 +386,11 @@ int sfh_time_hop(sfh_ctx* c, const sfh_gauge* g, int precision, int with_dot, in
               with_dot ? st : nullptr, part);
     else if (precision == 1)
       hop_f32(c->stream, gg, g->lp.u32, (const float2*)a, (float2*)b, 0.1f, 1.0f,
-              with_dot ? (const float2*)a : nullptr, with_dot ? st : nullptr, part);
+              with_dot ? (const float2*)a : nullptr, with_dot ? st : nullptr, part, g->lp_su3);
     else
       hop_f16(c->stream, gg, g->lp.u16, (const __half2*)a, na, (__half2*)b, nb, 0.1f, 1.0f,
-              with_dot ? (const __half2*)a : nullptr, with_dot ? na : nullptr, with_dot ? st : nullptr, part);
+              with_dot ? (const __half2*)a : nullptr, with_dot ? na : nullptr, with_dot ? st : nullptr, part,
+              g->lp_su3);
   };
   launch();
   hipEventRecord(c->ev0, c->stream);

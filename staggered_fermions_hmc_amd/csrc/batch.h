@@ -128,6 +128,7 @@ template <class T> class Engine {
   bool use_graph_;
   void launch_chunk();
   std::vector<SysH> sys_;
+  bool r12_ = true;  // every system's links are SU(3): the mixed hops read two rows
   std::vector<SlotH> slots_;
   std::vector<int> done_;
   std::vector<void*> owned_;

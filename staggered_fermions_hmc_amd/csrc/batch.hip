@@ -44,8 +44,9 @@ template <class R> struct SiteNb {
   R sf[4], sb[4];
 };
 
-// acc = sum_mu eta [bup U_mu(n) in(n+mu) - bdn U_mu(n-mu)^dag in(n-mu)]
-template <class T, class LT>
+// acc = sum_mu eta [bup U_mu(n) in(n+mu) - bdn U_mu(n-mu)^dag in(n-mu)];
+// R12: SU(3) links read as two rows.
+template <class T, bool R12 = false, class LT>
 __device__ __forceinline__ void site_hop(const LT* __restrict__ U, const typename Prec<T>::S* __restrict__ in, int V,
                                          int i, int q, const SiteNb<typename Prec<T>::R>& nb,
                                          vec3<typename Prec<T>::R>& acc) {
@@ -56,9 +57,9 @@ __device__ __forceinline__ void site_hop(const LT* __restrict__ U, const typenam
   for (int mu = 0; mu < 4; ++mu) {
     mat3<R> Uf, Ub;
     vec3<R> vf, vb;
-    load_link<T>(U, V, mu, i, Uf);
+    load_link<T, R12>(U, V, mu, i, Uf);
     load_spinor<T>(in, nullptr, V, q + nb.fw[mu], vf);
-    load_link<T>(U, V, mu, q + nb.bw[mu], Ub);
+    load_link<T, R12>(U, V, mu, q + nb.bw[mu], Ub);
     load_spinor<T>(in, nullptr, V, q + nb.bw[mu], vb);
     acc_mv(acc, Uf, vf, nb.sf[mu]);
     acc_mdv(acc, Ub, vb, -nb.sb[mu]);
@@ -70,7 +71,7 @@ __device__ __forceinline__ void site_hop(const LT* __restrict__ U, const typenam
 // the hop to every active right-hand side of its system in turn: the links
 // come from HBM once and from cache for the other right-hand sides.
 // phase DOT=false: t = D^dag p ; DOT=true: v = D t and <p, v>.
-template <class T, bool DOT>
+template <class T, bool DOT, bool R12>
 __global__ __launch_bounds__(512) void kb_hop(const BSys<T>* __restrict__ sys, const BSlot<T>* __restrict__ slots,
                                               const int4* __restrict__ bmap, typename Prec<T>::R hb) {
   using R = typename Prec<T>::R;
@@ -106,7 +107,7 @@ __global__ __launch_bounds__(512) void kb_hop(const BSys<T>* __restrict__ sys, c
     double d[4] = {0.0, 0.0, 0.0, 0.0};
     if (valid) {
       vec3<R> acc, self, res;
-      site_hop<T>(S.U, in, V, i, q, nb, acc);
+      site_hop<T, R12>(S.U, in, V, i, q, nb, acc);
       load_spinor<T>(in, nullptr, V, i, self);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -360,9 +361,17 @@ void Engine<T>::launch_chunk() {
   // FP64: the reference recurrence, hop, hop + <p,v>, x/r update, p update
   for (int it = 0; it < chunk_; ++it) {
     if constexpr (kMixed) hipLaunchKernelGGL((kb_upd<T>), dim3(nbb_), dim3(kBlas), 0, c_->stream, dslot_, dmb_);
-    hipLaunchKernelGGL((kb_hop<T, false>), dim3(nbh0_), dim3(cfg_.hop_threads), 0, c_->stream, dsys_, dslot_, dmh0_,
-                       (R)(-0.5));
-    hipLaunchKernelGGL((kb_hop<T, true>), dim3(nbh_), dim3(256), 0, c_->stream, dsys_, dslot_, dmh_, (R)(0.5));
+    if (kMixed && r12_) {
+      hipLaunchKernelGGL((kb_hop<T, false, kMixed>), dim3(nbh0_), dim3(cfg_.hop_threads), 0, c_->stream, dsys_,
+                         dslot_, dmh0_, (R)(-0.5));
+      hipLaunchKernelGGL((kb_hop<T, true, kMixed>), dim3(nbh_), dim3(256), 0, c_->stream, dsys_, dslot_, dmh_,
+                         (R)(0.5));
+    } else {
+      hipLaunchKernelGGL((kb_hop<T, false, false>), dim3(nbh0_), dim3(cfg_.hop_threads), 0, c_->stream, dsys_, dslot_,
+                         dmh0_, (R)(-0.5));
+      hipLaunchKernelGGL((kb_hop<T, true, false>), dim3(nbh_), dim3(256), 0, c_->stream, dsys_, dslot_, dmh_,
+                         (R)(0.5));
+    }
     if constexpr (!kMixed) {
       hipLaunchKernelGGL((kb_xr<T>), dim3(nbb_), dim3(kBlas), 0, c_->stream, dslot_, dmb_);
       hipLaunchKernelGGL((kb_p<T>), dim3(nbb_), dim3(kBlas), 0, c_->stream, dslot_, dmb_);
@@ -404,6 +413,7 @@ int Engine<T>::add_system(const sfh_gauge* gauge, double mass, int pool) {
   if (kMixed) {
     if (int e = ensure_lp(c_, gk, SFH_PREC_F32)) return -e;
     s.U = reinterpret_cast<const typename Prec<T>::L*>(gk->lp.u32);
+    r12_ = r12_ && gk->lp_su3;
   } else {
     s.U = reinterpret_cast<const typename Prec<T>::L*>(gk->U);
   }
@@ -631,8 +641,12 @@ int Engine<T>::time_hop(const BatchConfig& cfg, int n, double* us) {
   if (!dm) return SFH_E_NOMEM;
   hipMemcpyAsync(dm, m.data(), m.size() * sizeof(int4), hipMemcpyHostToDevice, c_->stream);
   auto launch = [&]() {
-    hipLaunchKernelGGL((kb_hop<T, false>), dim3((int)m.size()), dim3(cfg.hop_threads), 0, c_->stream, dsys_, dslot_, dm,
-                       (R)(-0.5));
+    if (kMixed && r12_)
+      hipLaunchKernelGGL((kb_hop<T, false, kMixed>), dim3((int)m.size()), dim3(cfg.hop_threads), 0, c_->stream, dsys_,
+                         dslot_, dm, (R)(-0.5));
+    else
+      hipLaunchKernelGGL((kb_hop<T, false, false>), dim3((int)m.size()), dim3(cfg.hop_threads), 0, c_->stream, dsys_,
+                         dslot_, dm, (R)(-0.5));
   };
   launch();
   hipEventRecord(c_->ev0, c_->stream);

@@ -115,7 +115,7 @@ __device__ __forceinline__ void sum_partials4(const double* __restrict__ partial
 // from st_in and writes the advanced state to st_out, a different copy, so
 // no workgroup reads what another writes; successive iterations alternate
 // the two copies.  Otherwise st_in == st_out and B has run the finisher.
-template <class T, bool DEFER>
+template <class T, bool DEFER, bool R12>
 __global__ __launch_bounds__(kBlk) void k_mix_a(Geom g, const typename Prec<T>::L* __restrict__ U, MixBufs<T> b,
                                                 float mass, const CGState* st_in, CGState* st_out,
                                                 const double* __restrict__ partials, int nb) {
@@ -191,11 +191,11 @@ __global__ __launch_bounds__(kBlk) void k_mix_a(Geom g, const typename Prec<T>::
     neighbours<float>(g, s.cb, n, mu, fw, bw, sf, sb);
     mat3<float> Uf, Ub;
     vec3<float> rf, vf, pf, rb, vb, pb;
-    load_link<T>(U, V, mu, s.i, Uf);
+    load_link<T, R12>(U, V, mu, s.i, Uf);
     load_spinor<T>(rp, rpn, V, s.q + fw, rf);
     load_spinor<T>(b.v, b.vn, V, s.q + fw, vf);
     load_spinor<T>(pp, ppn, V, s.q + fw, pf);
-    load_link<T>(U, V, mu, s.q + bw, Ub);
+    load_link<T, R12>(U, V, mu, s.q + bw, Ub);
     load_spinor<T>(rp, rpn, V, s.q + bw, rb);
     load_spinor<T>(b.v, b.vn, V, s.q + bw, vb);
     load_spinor<T>(pp, ppn, V, s.q + bw, pb);
@@ -214,7 +214,7 @@ __global__ __launch_bounds__(kBlk) void k_mix_a(Geom g, const typename Prec<T>::
   store_spinor<T>(b.t, b.tn, V, s.i, res);
 }
 
-template <class T, bool DEFER>
+template <class T, bool DEFER, bool R12>
 __global__ __launch_bounds__(kBlk) void k_mix_b(Geom g, const typename Prec<T>::L* __restrict__ U, MixBufs<T> b,
                                                 float mass, CGState* st, double* partials, int nb) {
   if (st->done) return;
@@ -235,9 +235,9 @@ __global__ __launch_bounds__(kBlk) void k_mix_b(Geom g, const typename Prec<T>::
       neighbours<float>(g, s.cb, n, mu, fw, bw, sf, sb);
       mat3<float> Uf, Ub;
       vec3<float> tf, tb;
-      load_link<T>(U, V, mu, s.i, Uf);
+      load_link<T, R12>(U, V, mu, s.i, Uf);
       load_spinor<T>(b.t, b.tn, V, s.q + fw, tf);
-      load_link<T>(U, V, mu, s.q + bw, Ub);
+      load_link<T, R12>(U, V, mu, s.q + bw, Ub);
       load_spinor<T>(b.t, b.tn, V, s.q + bw, tb);
       acc_mv(acc, Uf, tf, sf);
       acc_mdv(acc, Ub, tb, -sb);
@@ -365,21 +365,33 @@ bool mix_defers(const Geom& g) { return mix_blocks(g) <= kDirectSum; }
 
 template <class T>
 void mix_a(hipStream_t s, const Geom& g, const typename Prec<T>::L* U, const MixBufs<T>& b, float mass,
-           const CGState* st_in, CGState* st_out, const double* partials) {
+           const CGState* st_in, CGState* st_out, const double* partials, bool r12) {
   const int nb = mix_blocks(g);
-  if (st_in != st_out)
-    hipLaunchKernelGGL((k_mix_a<T, true>), dim3(nb), dim3(kBlk), 0, s, g, U, b, mass, st_in, st_out, partials, nb);
-  else
-    hipLaunchKernelGGL((k_mix_a<T, false>), dim3(nb), dim3(kBlk), 0, s, g, U, b, mass, st_in, st_out, partials, nb);
+  auto go = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, dim3(nb), dim3(kBlk), 0, s, g, U, b, mass, st_in, st_out, partials, nb);
+  };
+  if (st_in != st_out) {
+    if (r12) go(k_mix_a<T, true, true>);
+    else go(k_mix_a<T, true, false>);
+  } else {
+    if (r12) go(k_mix_a<T, false, true>);
+    else go(k_mix_a<T, false, false>);
+  }
 }
 template <class T>
 void mix_b(hipStream_t s, const Geom& g, const typename Prec<T>::L* U, const MixBufs<T>& b, float mass, CGState* st,
-           double* partials, bool defer) {
+           double* partials, bool defer, bool r12) {
   const int nb = mix_blocks(g);
-  if (defer)
-    hipLaunchKernelGGL((k_mix_b<T, true>), dim3(nb), dim3(kBlk), 0, s, g, U, b, mass, st, partials, nb);
-  else
-    hipLaunchKernelGGL((k_mix_b<T, false>), dim3(nb), dim3(kBlk), 0, s, g, U, b, mass, st, partials, nb);
+  auto go = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, dim3(nb), dim3(kBlk), 0, s, g, U, b, mass, st, partials, nb);
+  };
+  if (defer) {
+    if (r12) go(k_mix_b<T, true, true>);
+    else go(k_mix_b<T, true, false>);
+  } else {
+    if (r12) go(k_mix_b<T, false, true>);
+    else go(k_mix_b<T, false, false>);
+  }
 }
 void mix_fin(hipStream_t s, const Geom& g, CGState* st, const double* partials) {
   hipLaunchKernelGGL(k_mix_fin, dim3(1), dim3(kBlk), 0, s, st, partials, mix_blocks(g));
@@ -395,13 +407,13 @@ template <class T> void mix_apply(hipStream_t s, const Geom& g, const double2* r
 }
 
 template void mix_a<float>(hipStream_t, const Geom&, const float2*, const MixBufs<float>&, float, const CGState*,
-                           CGState*, const double*);
+                           CGState*, const double*, bool);
 template void mix_a<Half>(hipStream_t, const Geom&, const __half2*, const MixBufs<Half>&, float, const CGState*,
-                          CGState*, const double*);
+                          CGState*, const double*, bool);
 template void mix_b<float>(hipStream_t, const Geom&, const float2*, const MixBufs<float>&, float, CGState*, double*,
-                           bool);
+                           bool, bool);
 template void mix_b<Half>(hipStream_t, const Geom&, const __half2*, const MixBufs<Half>&, float, CGState*, double*,
-                          bool);
+                          bool, bool);
 template void mix_upd<float>(hipStream_t, const Geom&, const MixBufs<float>&, CGState*);
 template void mix_upd<Half>(hipStream_t, const Geom&, const MixBufs<Half>&, CGState*);
 template void mix_acc<float>(hipStream_t, const Geom&, double2*, const MixBufs<float>&, CGState*);

@@ -19,7 +19,8 @@ namespace sfh {
 
 // UNR: directions processed per step of the mu loop (4: all links of the
 // site in flight at once; 1 or 2: fewer registers, more waves per SIMD).
-template <class T, bool DOT, int UNR>
+// R12: links read as two rows, the third rebuilt (SU(3) links only).
+template <class T, bool DOT, int UNR, bool R12 = false>
 __global__ __launch_bounds__(256) void k_hop(Geom g, const typename Prec<T>::L* __restrict__ U,
                                              const typename Prec<T>::S* __restrict__ in,
                                              const float* __restrict__ in_norm, typename Prec<T>::S* __restrict__ out,
@@ -69,9 +70,9 @@ __global__ __launch_bounds__(256) void k_hop(Geom g, const typename Prec<T>::L* 
       }
       mat3<R> Uf, Ub;
       vec3<R> vf, vb;
-      load_link<T>(U, V, mu, i, Uf);
+      load_link<T, R12>(U, V, mu, i, Uf);
       load_spinor<T>(in, in_norm, V, q + fw, vf);
-      load_link<T>(U, V, mu, q + bw, Ub);
+      load_link<T, R12>(U, V, mu, q + bw, Ub);
       load_spinor<T>(in, in_norm, V, q + bw, vb);
       acc_mv(acc, Uf, vf, sf);
       acc_mdv(acc, Ub, vb, -sb);
@@ -209,9 +210,10 @@ static inline int hop4_blocks(const Geom& g) { return (g.Vh + 31) / 32; }
 
 // Variant 0: one lane per site, all four directions unrolled; 1: four lanes
 // per site; 2, 3: one lane per site, the direction loop stepped one or two
-// directions at a time.
+// directions at a time.  r12 (low precision only, not the four-lane form):
+// SU(3) links read as two rows.
 template <class T>
-static void hop_launch(hipStream_t s, int variant, const Geom& g, const typename Prec<T>::L* U,
+static void hop_launch(hipStream_t s, int variant, bool r12, const Geom& g, const typename Prec<T>::L* U,
                        const typename Prec<T>::S* in, const float* in_norm, typename Prec<T>::S* out, float* out_norm,
                        typename Prec<T>::R a, typename Prec<T>::R hb, const typename Prec<T>::S* dw,
                        const float* dw_norm, CGState* st, double* partials, int gate) {
@@ -229,16 +231,22 @@ static void hop_launch(hipStream_t s, int variant, const Geom& g, const typename
       hipLaunchKernelGGL(kernel, dim3(nb), dim3(kHopBlock), 0, s, g, U, in, in_norm, out, out_norm, a, hb, dw, dw_norm,
                          st, partials, nb, gate);
     };
-    if (variant == 2) {
-      if (dw) go(k_hop<T, true, 1>);
-      else go(k_hop<T, false, 1>);
-    } else if (variant == 3) {
-      if (dw) go(k_hop<T, true, 2>);
-      else go(k_hop<T, false, 2>);
-    } else {
-      if (dw) go(k_hop<T, true, 4>);
-      else go(k_hop<T, false, 4>);
-    }
+    constexpr bool kLow = !std::is_same<T, double>::value;
+    auto pick = [&](auto r) {
+      constexpr bool R = decltype(r)::value;
+      if (variant == 2) {
+        if (dw) go(k_hop<T, true, 1, R>);
+        else go(k_hop<T, false, 1, R>);
+      } else if (variant == 3) {
+        if (dw) go(k_hop<T, true, 2, R>);
+        else go(k_hop<T, false, 2, R>);
+      } else {
+        if (dw) go(k_hop<T, true, 4, R>);
+        else go(k_hop<T, false, 4, R>);
+      }
+    };
+    if (kLow && r12) pick(std::integral_constant<bool, kLow>{});
+    else pick(std::false_type{});
   }
 }
 
@@ -259,21 +267,21 @@ int get_hop_variant(int prec, const Geom& g) { return pick_variant(prec, g); }
 
 void hop_f64(hipStream_t s, const Geom& g, const double2* U, const double2* in, double2* out, double a, double b,
              const double2* dot_with, CGState* st, double* partials, int gate) {
-  hop_launch<double>(s, pick_variant(0, g), g, U, in, nullptr, out, nullptr, a, 0.5 * b, dot_with, nullptr, st,
+  hop_launch<double>(s, pick_variant(0, g), false, g, U, in, nullptr, out, nullptr, a, 0.5 * b, dot_with, nullptr, st,
                      partials, gate);
 }
 
 void hop_f32(hipStream_t s, const Geom& g, const float2* U, const float2* in, float2* out, float a, float b,
-             const float2* dot_with, CGState* st, double* partials) {
-  hop_launch<float>(s, pick_variant(1, g), g, U, in, nullptr, out, nullptr, a, 0.5f * b, dot_with, nullptr, st,
+             const float2* dot_with, CGState* st, double* partials, bool r12) {
+  hop_launch<float>(s, pick_variant(1, g), r12, g, U, in, nullptr, out, nullptr, a, 0.5f * b, dot_with, nullptr, st,
                     partials, 0);
 }
 
 void hop_f16(hipStream_t s, const Geom& g, const __half2* U, const __half2* in, const float* in_norm, __half2* out,
              float* out_norm, float a, float b, const __half2* dot_with, const float* dot_norm, CGState* st,
-             double* partials) {
-  hop_launch<Half>(s, pick_variant(2, g), g, U, in, in_norm, out, out_norm, a, 0.5f * b, dot_with, dot_norm, st,
-                   partials, 0);
+             double* partials, bool r12) {
+  hop_launch<Half>(s, pick_variant(2, g), r12, g, U, in, in_norm, out, out_norm, a, 0.5f * b, dot_with, dot_norm,
+                   st, partials, 0);
 }
 
 // ------------------------------------------------------------------ layout converters
@@ -338,6 +346,38 @@ __global__ void k_links_to_f16(size_t n, const double2* __restrict__ U, __half2*
   u[i] = __floats2half2_rn((float)v.x, (float)v.y);
 }
 
+// One lane per link: |row2 - conj(row0 x row1)| in FP64, the block maximum
+// folded into *dev by an integer max on the float's bits (order-preserving
+// for non-negative floats; NaN compares above every finite value).
+__global__ __launch_bounds__(256) void k_su3_dev(Geom g, const double2* __restrict__ U, unsigned* dev) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int mu = blockIdx.y;
+  float d = 0.f;
+  if (i < g.V) {
+    cpx<double> m[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) m[e] = ldc(U + (size_t)(mu * 9 + e) * g.V + i);
+    const cpx<double> r[3] = {conj(m[1] * m[5] - m[2] * m[4]), conj(m[2] * m[3] - m[0] * m[5]),
+                              conj(m[0] * m[4] - m[1] * m[3])};
+    double x = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const cpx<double> e = m[6 + k] - r[k];
+      x = fmax(x, fmax(fabs(e.re), fabs(e.im)));
+    }
+    d = x == x ? (float)x : __int_as_float(0x7f800000);
+  }
+  __shared__ float w[4];
+  for (int o = 32; o > 0; o >>= 1) d = fmaxf(d, __shfl_xor(d, o));
+  if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = d;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float b = 0.f;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) b = fmaxf(b, w[k]);
+    atomicMax(dev, __float_as_uint(b));
+  }
+}
+
 static inline unsigned grid1(size_t n, int b) { return (unsigned)((n + b - 1) / b); }
 
 void links_aos_to_soa(hipStream_t s, const Geom& g, const double* aos, double2* soa) {
@@ -356,6 +396,10 @@ void links_to_f32(hipStream_t s, const Geom& g, const double2* U, float2* u32) {
   const size_t n = (size_t)36 * g.V;
   hipLaunchKernelGGL(k_links_to_f32, dim3(grid1(n, 256)), dim3(256), 0, s, n, U, u32);
 }
+void su3_deviation(hipStream_t s, const Geom& g, const double2* U, unsigned* dev) {
+  hipLaunchKernelGGL(k_su3_dev, dim3(grid1(g.V, 256), 4), dim3(256), 0, s, g, U, dev);
+}
+
 void links_to_f16(hipStream_t s, const Geom& g, const double2* U, __half2* u16) {
   const size_t n = (size_t)36 * g.V;
   hipLaunchKernelGGL(k_links_to_f16, dim3(grid1(n, 256)), dim3(256), 0, s, n, U, u16);

@@ -30,11 +30,18 @@ __device__ __forceinline__ cpx<float> ldc(const __half2* p) {
   return {v.x, v.y};
 }
 
-template <class T>
+// R12: read the first two rows only and rebuild the third as
+// conj(row0 x row1), exact for SU(3) links (two thirds of the link bytes).
+template <class T, bool R12 = false>
 __device__ __forceinline__ void load_link(const typename Prec<T>::L* __restrict__ U, int V, int mu, int i,
                                           mat3<typename Prec<T>::R>& m) {
 #pragma unroll
-  for (int e = 0; e < 9; ++e) m.e[e] = ldc(U + (size_t)(mu * 9 + e) * V + i);
+  for (int e = 0; e < (R12 ? 6 : 9); ++e) m.e[e] = ldc(U + (size_t)(mu * 9 + e) * V + i);
+  if constexpr (R12) {
+    m.e[6] = conj(m.e[1] * m.e[5] - m.e[2] * m.e[4]);
+    m.e[7] = conj(m.e[2] * m.e[3] - m.e[0] * m.e[5]);
+    m.e[8] = conj(m.e[0] * m.e[4] - m.e[1] * m.e[3]);
+  }
 }
 
 template <class T>

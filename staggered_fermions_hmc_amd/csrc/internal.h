@@ -63,6 +63,10 @@ struct sfh_gauge {
   double2* U = nullptr;     // 4*9*V complex, device layout
   sfh::LinksLP lp;
   bool lp32_valid = false, lp16_valid = false;
+  // the FP64 links are SU(3) (third row = conj(row0 x row1) to 1e-10), so the
+  // low-precision kernels may read two rows and rebuild the third; set with
+  // the low-precision copies
+  bool lp_su3 = false;
 };
 
 struct sfh_ctx {
@@ -130,6 +134,9 @@ void spinor_aos_to_soa(hipStream_t s, const Geom& g, const double* aos, double2*
 void spinor_soa_to_aos(hipStream_t s, const Geom& g, const double2* soa, double* aos);
 void links_to_f32(hipStream_t s, const Geom& g, const double2* U, float2* u32);
 void links_to_f16(hipStream_t s, const Geom& g, const double2* U, __half2* u16);
+// max over links of |row2 - conj(row0 x row1)| as the bits of a float, into
+// *dev (which must start at 0)
+void su3_deviation(hipStream_t s, const Geom& g, const double2* U, unsigned* dev);
 
 // ------------------------------------------------------------------ hop (hop.hip)
 // out = a*in + (b/2) * sum_mu eta [bup U psi(n+mu) - bdn U^dag psi(n-mu)]
@@ -138,11 +145,12 @@ void links_to_f16(hipStream_t s, const Geom& g, const double2* U, __half2* u16);
 // st->ru_pending (a reliable-update step).
 void hop_f64(hipStream_t s, const Geom& g, const double2* U, const double2* in, double2* out, double a, double b,
              const double2* dot_with, CGState* st, double* partials, int gate = 0);
+// r12: the links are SU(3); read two rows and rebuild the third.
 void hop_f32(hipStream_t s, const Geom& g, const float2* U, const float2* in, float2* out, float a, float b,
-             const float2* dot_with, CGState* st, double* partials);
+             const float2* dot_with, CGState* st, double* partials, bool r12);
 void hop_f16(hipStream_t s, const Geom& g, const __half2* U, const __half2* in, const float* in_norm,
              __half2* out, float* out_norm, float a, float b, const __half2* dot_with, const float* dot_norm,
-             CGState* st, double* partials);
+             CGState* st, double* partials, bool r12);
 
 // Hop kernel form: 0 one lane per site, 1 four lanes per site (one per
 // direction); -1 restores the size-based default.  prec: 0 f64, 1 f32, 2 f16.

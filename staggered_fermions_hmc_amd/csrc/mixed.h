@@ -22,9 +22,10 @@ template <class T> struct MixBufs {
 // recomputed at the neighbours, then t = D^dag p_k.  With st_in != st_out
 // (small grids, see mix_defers) it first finishes the previous B's dots from
 // `partials` and writes the advanced state to st_out.
+// r12: SU(3) links, read as two rows with the third rebuilt.
 template <class T>
 void mix_a(hipStream_t s, const Geom& g, const typename Prec<T>::L* U, const MixBufs<T>& b, float mass,
-           const CGState* st_in, CGState* st_out, const double* partials);
+           const CGState* st_in, CGState* st_out, const double* partials, bool r12);
 // Whether B leaves its dots as per-workgroup partials for the next A (and
 // mix_fin) to finish, instead of a grid reduction with its own finisher:
 // true when the grid is small enough for every workgroup to read them all.
@@ -40,7 +41,7 @@ constexpr int kMixFusedMaxV = 1 << 18;
 // Kernel B of iteration k: v = D t and the four dots of FinFused.
 template <class T>
 void mix_b(hipStream_t s, const Geom& g, const typename Prec<T>::L* U, const MixBufs<T>& b, float mass, CGState* st,
-           double* partials, bool defer);
+           double* partials, bool defer, bool r12);
 // Reliable update, first and last steps (gated on the solve state):
 // x64 += x + alpha p_k and x = 0; then r_{k} = r64 in sloppy storage.
 template <class T> void mix_acc(hipStream_t s, const Geom& g, double2* x64, const MixBufs<T>& b, CGState* st);

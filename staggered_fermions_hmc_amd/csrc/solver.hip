@@ -46,9 +46,27 @@ double dev_dot(sfh_ctx* c, const double2* a, const double2* b, size_t ncomplex) 
 }
 
 // Low-precision links are derived lazily from the FP64 links and invalidated
-// whenever those change.
+// whenever those change.  Building a copy also checks that the FP64 links
+// are SU(3) (third row = conj(row0 x row1) to 1e-10, one pass over the links
+// and one host sync); only then do the low-precision kernels read two rows
+// and rebuild the third.
+constexpr float kSu3Tol = 1e-10f;
 int ensure_lp(sfh_ctx* c, sfh_gauge* gauge, int precision) {
   const Geom& g = gauge->g;
+  if ((precision == SFH_PREC_F32 && !gauge->lp32_valid) || (precision == SFH_PREC_F16 && !gauge->lp16_valid)) {
+    if (!gauge->lp32_valid && !gauge->lp16_valid) {
+      unsigned* dev = static_cast<unsigned*>(scratch(c, "su3_dev", sizeof(unsigned)));
+      if (!dev) return SFH_E_NOMEM;
+      hipMemsetAsync(dev, 0, sizeof(unsigned), c->stream);
+      su3_deviation(c->stream, g, gauge->U, dev);
+      unsigned* h = reinterpret_cast<unsigned*>(c->h_scalar);
+      hipMemcpyAsync(h, dev, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream);
+      if (int e = hip_check(c, hipStreamSynchronize(c->stream), "SU(3) check")) return e;
+      float d;
+      std::memcpy(&d, h, sizeof d);
+      gauge->lp_su3 = d < kSu3Tol;
+    }
+  }
   if (precision == SFH_PREC_F32 && !gauge->lp32_valid) {
     if (!gauge->lp.u32 && hipMalloc(&gauge->lp.u32, 36 * (size_t)g.V * sizeof(float2)) != hipSuccess)
       return set_err(c, SFH_E_NOMEM, "FP32 link copy");
@@ -238,7 +256,7 @@ void mix_step_a(sfh_ctx* c, const sfh_gauge* gauge, bool fused, const MixBufs<T>
     KTimer kt(c, half ? "mix_a_f16" : "mix_a_f32");
     mix_a<T>(c->stream, g, reinterpret_cast<const typename Prec<T>::L*>(half ? (const void*)gauge->lp.u16
                                                                              : (const void*)gauge->lp.u32),
-             b, m, st_in, st_out, partials);
+             b, m, st_in, st_out, partials, gauge->lp_su3);
     return;
   }
   CGState* st = st_out;
@@ -249,10 +267,10 @@ void mix_step_a(sfh_ctx* c, const sfh_gauge* gauge, bool fused, const MixBufs<T>
   KTimer kt(c, half ? "hop_f16" : "hop_f32");
   if (half)
     hop_f16(c->stream, g, gauge->lp.u16, reinterpret_cast<const __half2*>(b.p[0]), b.pn[0],
-            reinterpret_cast<__half2*>(b.t), b.tn, m, -1.f, nullptr, nullptr, st, nullptr);
+            reinterpret_cast<__half2*>(b.t), b.tn, m, -1.f, nullptr, nullptr, st, nullptr, gauge->lp_su3);
   else
     hop_f32(c->stream, g, gauge->lp.u32, reinterpret_cast<const float2*>(b.p[0]), reinterpret_cast<float2*>(b.t), m,
-            -1.f, nullptr, st, nullptr);
+            -1.f, nullptr, st, nullptr, gauge->lp_su3);
 }
 
 }  // namespace
@@ -372,7 +390,7 @@ static int cg_mixed(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* prm, cons
   auto iter = [&](int j) {
     mix_step_a<T>(c, gauge, fused, b, m, sts[j & 1], sts[(j + 1) & 1], part);
     KTimer kt(c, HALF ? "mix_b_f16" : "mix_b_f32");
-    mix_b<T>(c->stream, g, U, b, m, sts[(j + 1) & 1], part, defer);
+    mix_b<T>(c->stream, g, U, b, m, sts[(j + 1) & 1], part, defer, gauge->lp_su3);
   };
   auto tail = [&]() {
     KTimer kt(c, "reliable_update");
@@ -390,7 +408,9 @@ static int cg_mixed(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* prm, cons
   const std::string key = graph_key(HALF ? (fused ? "mix16f" : "mix16s") : (fused ? "mix32f" : "mix32s"), g,
                                     {gauge->U, (const void*)U, x, r64, phi, t64, ax, b.r[0], b.r[1], b.p[0], b.p[1],
                                      b.v, b.t, b.rn[0], b.rn[1], b.pn[0], b.pn[1], b.vn, b.tn, b.x, st, sts[1], part},
-                                    prm->mass, get_hop_variant(0, g) * 4 + get_hop_variant(HALF ? 2 : 1, g));
+                                    prm->mass,
+                                    (get_hop_variant(0, g) * 4 + get_hop_variant(HALF ? 2 : 1, g)) * 2 +
+                                        (gauge->lp_su3 ? 1 : 0));
   CGState fin{};
   if (int e = run_chunks(c, [&]() { return launch_chunk(c, key, direct); }, kChunkMixed, s, fin)) return e;
   if (stats) {
@@ -500,7 +520,7 @@ static int time_mixed_t(sfh_ctx* c, sfh_gauge* gauge, int n, double* us_a, doubl
   const bool defer = fused && mix_defers(g);
   CGState* st_out = defer ? &c->d_state[3] : st;
   if (int e = timed([&]() { mix_step_a<T>(c, gauge, fused, b, m, st, st_out, part); }, us_a)) return e;
-  return timed([&]() { mix_b<T>(c->stream, g, U, b, m, st, part, defer); }, us_b);
+  return timed([&]() { mix_b<T>(c->stream, g, U, b, m, st, part, defer, gauge->lp_su3); }, us_b);
 }
 
 void set_mix_form(int precision, int form) {

@@ -137,3 +137,21 @@ def test_batch_launch_shapes_give_identical_results(precision):
     with pytest.raises(dev.SfhError):
         dev.tune_batch(ref_ctx, [(s["g"], s["mass"], s["rf"], s["xf"]) for s in systems], precision=precision,
                        shape=(300, 3))
+
+
+def test_batch_mixed_with_non_su3_links(ctx):
+    """Links scaled by 0.9 are not SU(3): the mixed hops must read all three
+    rows (a rebuilt third row would be 10% off) and still reach the true
+    residual, alone and next to SU(3) systems."""
+    systems = make_systems(ctx, SPECS[:2], seed0=300)
+    s = systems[1]
+    s["U"] = 0.9 * s["U"]
+    s["g"].upload(s["U"])
+    stats = run(ctx, systems, "f32")
+    k = 0
+    for s in systems:
+        for j, b in enumerate(s["b"]):
+            x = s["xf"][j].download()
+            r = b - oc.apply_ddag(s["dims"], s["mass"], s["U"], x)
+            assert np.sqrt(r @ r / (b @ b)) < 1e-10 and stats[k]["status"] == 0
+            k += 1

@@ -414,3 +414,22 @@ def test_hmc_trajectory_matches_oracle(ctx):
     assert abs(H[6] - Hr[6]) < 1e-7
     assert acc == accr
     assert np.abs(g.download() - Ur).max() < 1e-10
+
+
+@pytest.mark.parametrize("precision", ["f32", "f16"])
+def test_mixed_cg_non_su3_links(ctx, precision):
+    """The low-precision hops read two link rows and rebuild the third only
+    for SU(3) links; links scaled by 0.9 take the three-row kernels and the
+    mixed solve still reaches the true residual and the oracle's solution.
+    The same lattice's SU(3) links then take the two-row kernels again."""
+    dims = (8, 8, 8, 8)
+    U = dev.random_links(dims, 0.3, seed=51)
+    phi = dev.random_spinor(dims, seed=52)
+    g = dev.Gauge(ctx, dims, 0.9 * U)
+    for links in (0.9 * U, U):
+        g.upload(links)
+        x, st = g.cg(phi, 0.05, tol=1e-10, precision=precision)
+        xr, *_ = oc.cg(dims, 0.05, links, phi, tol=1e-12)
+        r = phi - oc.apply_ddag(dims, 0.05, links, x)
+        assert np.sqrt(r @ r / (phi @ phi)) < 1e-10, st
+        assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-8
