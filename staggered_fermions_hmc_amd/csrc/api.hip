@@ -310,12 +310,9 @@ int sfh_tune_hop(sfh_ctx* c, const sfh_gauge* g, int precision, int n, int* vari
   const size_t nc = 3 * (size_t)gg.V;
   void* a = scratch(c, "tune_a", nc * sizeof(double2));
   void* b = scratch(c, "tune_b", nc * sizeof(double2));
-  float* na = static_cast<float*>(scratch(c, "tune_na", gg.V * sizeof(float)));
-  float* nb = static_cast<float*>(scratch(c, "tune_nb", gg.V * sizeof(float)));
-  if (!a || !b || !na || !nb) return SFH_E_NOMEM;
+  if (!a || !b) return SFH_E_NOMEM;
   if (int e = ensure_lp(c, const_cast<sfh_gauge*>(g), precision)) return e;
   hipMemsetAsync(a, 0, nc * sizeof(double2), c->stream);
-  hipMemsetAsync(na, 0, gg.V * sizeof(float), c->stream);
   double best_us = 1e30;
   int best = 0;
   for (int v = 0; v < 4; ++v) {
@@ -327,8 +324,7 @@ int sfh_tune_hop(sfh_ctx* c, const sfh_gauge* g, int precision, int n, int* vari
         hop_f32(c->stream, gg, g->lp.u32, (const float2*)a, (float2*)b, 0.1f, 1.0f, nullptr, nullptr, nullptr,
                 g->lp_su3);
       else
-        hop_f16(c->stream, gg, g->lp.u16, (const __half2*)a, na, (__half2*)b, nb, 0.1f, 1.0f, nullptr, nullptr,
-                nullptr, nullptr, g->lp_su3);
+        hop_f16(c->stream, gg, g->lp.u16, (const H16*)a, (H16*)b, 0.1f, 1.0f, nullptr, nullptr, nullptr, g->lp_su3);
     };
     launch();
     hipEventRecord(c->ev0, c->stream);
@@ -356,9 +352,7 @@ int sfh_time_hop(sfh_ctx* c, const sfh_gauge* g, int precision, int with_dot, in
   const size_t nc = 3 * (size_t)gg.V;
   void* a = scratch(c, "tune_a", nc * sizeof(double2));
   void* b = scratch(c, "tune_b", nc * sizeof(double2));
-  float* na = static_cast<float*>(scratch(c, "tune_na", gg.V * sizeof(float)));
-  float* nb = static_cast<float*>(scratch(c, "tune_nb", gg.V * sizeof(float)));
-  if (!a || !b || !na || !nb) return SFH_E_NOMEM;
+  if (!a || !b) return SFH_E_NOMEM;
   if (int e = ensure_lp(c, const_cast<sfh_gauge*>(g), precision)) return e;
   // input of ones (a non-zero <p,v> keeps the timed solve state running)
   if (precision == 0) {
@@ -370,8 +364,7 @@ int sfh_time_hop(sfh_ctx* c, const sfh_gauge* g, int precision, int with_dot, in
     hipMemcpyAsync(a, ones.data(), ones.size() * sizeof(float), hipMemcpyHostToDevice, c->stream);
     hipStreamSynchronize(c->stream);
   } else {
-    hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(a), 0x3C003C00u, nc, c->stream);  // half2 (1, 1)
-    hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(na), 0x3F800000u, gg.V, c->stream);  // 1.0f
+    h16_ones(c->stream, static_cast<H16*>(a), gg.V);
   }
   // a live solve state so the reduction runs exactly as in the CG
   CGState* st = &c->d_state[1];
@@ -388,9 +381,8 @@ int sfh_time_hop(sfh_ctx* c, const sfh_gauge* g, int precision, int with_dot, in
       hop_f32(c->stream, gg, g->lp.u32, (const float2*)a, (float2*)b, 0.1f, 1.0f,
               with_dot ? (const float2*)a : nullptr, with_dot ? st : nullptr, part, g->lp_su3);
     else
-      hop_f16(c->stream, gg, g->lp.u16, (const __half2*)a, na, (__half2*)b, nb, 0.1f, 1.0f,
-              with_dot ? (const __half2*)a : nullptr, with_dot ? na : nullptr, with_dot ? st : nullptr, part,
-              g->lp_su3);
+      hop_f16(c->stream, gg, g->lp.u16, (const H16*)a, (H16*)b, 0.1f, 1.0f, with_dot ? (const H16*)a : nullptr,
+              with_dot ? st : nullptr, part, g->lp_su3);
   };
   launch();
   hipEventRecord(c->ev0, c->stream);

@@ -92,14 +92,16 @@ void cg_p_f32(hipStream_t s, const Geom& g, float2* p, const float2* r, CGState*
 }
 
 // ------------------------------------------------------------------ FP16 (per-site norm) CG updates
-__device__ __forceinline__ void st16(__half2* f, float* nrm, int V, int i, const float2 v[3]) {
+__device__ __forceinline__ void st16(H16* f, int i, const float2 v[3]) {
   float m = 0.f;
 #pragma unroll
   for (int c = 0; c < 3; ++c) m = fmaxf(m, fmaxf(fabsf(v[c].x), fabsf(v[c].y)));
   const float inv = m > 0.f ? 1.0f / m : 0.f;
-  nrm[i] = m;
+  H16 s;
 #pragma unroll
-  for (int c = 0; c < 3; ++c) f[(size_t)c * V + i] = __floats2half2_rn(v[c].x * inv, v[c].y * inv);
+  for (int c = 0; c < 3; ++c) s.c[c] = __floats2half2_rn(v[c].x * inv, v[c].y * inv);
+  s.n = m;
+  f[i] = s;
 }
 
 // ------------------------------------------------------------------ precision conversion / FP64 BLAS
@@ -111,7 +113,7 @@ void f64_to_f32(hipStream_t s, const double2* in, float2* out, size_t n) {
   hipLaunchKernelGGL(k_f64_to_f32, dim3(nblk(n)), dim3(kB), 0, s, in, out, n);
 }
 
-__global__ void k_f64_to_f16(int V, const double2* __restrict__ in, __half2* __restrict__ out, float* __restrict__ nrm) {
+__global__ void k_f64_to_f16(int V, const double2* __restrict__ in, H16* __restrict__ out) {
   const int i = blockIdx.x * kB + threadIdx.x;
   if (i >= V) return;
   float2 v[3];
@@ -120,11 +122,23 @@ __global__ void k_f64_to_f16(int V, const double2* __restrict__ in, __half2* __r
     const double2 t = in[(size_t)c * V + i];
     v[c] = make_float2((float)t.x, (float)t.y);
   }
-  st16(out, nrm, V, i, v);
+  st16(out, i, v);
 }
-void f64_to_f16(hipStream_t s, const Geom& g, const double2* in, __half2* out, float* norm) {
-  hipLaunchKernelGGL(k_f64_to_f16, dim3(nblk(g.V)), dim3(kB), 0, s, g.V, in, out, norm);
+void f64_to_f16(hipStream_t s, const Geom& g, const double2* in, H16* out) {
+  hipLaunchKernelGGL(k_f64_to_f16, dim3(nblk(g.V)), dim3(kB), 0, s, g.V, in, out);
 }
+
+// FP16 records of (1 + i) in every colour (halves 1.0, norm 1.0).
+__global__ void k_h16_ones(H16* __restrict__ f, int V) {
+  const int i = blockIdx.x * kB + threadIdx.x;
+  if (i >= V) return;
+  H16 s;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) s.c[c] = __floats2half2_rn(1.0f, 1.0f);
+  s.n = 1.0f;
+  f[i] = s;
+}
+void h16_ones(hipStream_t s, H16* f, int V) { hipLaunchKernelGGL(k_h16_ones, dim3(nblk(V)), dim3(kB), 0, s, f, V); }
 
 __global__ void k_acc_f32(double2* __restrict__ xh, float2* __restrict__ xl, size_t n) {
   const size_t i = (size_t)blockIdx.x * kB + threadIdx.x;

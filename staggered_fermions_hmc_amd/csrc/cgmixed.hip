@@ -66,10 +66,13 @@ __device__ __forceinline__ void advance(vec3<float>& r, const vec3<float>& v, ve
 template <class T>
 __device__ __forceinline__ void store_q(typename Prec<T>::S* f, float* nrm, int V, int i, const vec3<float>& v,
                                         const Q16& q) {
+  (void)nrm;
   if constexpr (is_half<T>()) {
+    H16 s;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) f[(size_t)c * V + i] = q.h[c];
-    nrm[i] = q.m;
+    for (int c = 0; c < 3; ++c) s.c[c] = q.h[c];
+    s.n = q.m;
+    f[i] = s;
   } else {
 #pragma unroll
     for (int c = 0; c < 3; ++c) f[(size_t)c * V + i] = make_float2(v.c[c].re, v.c[c].im);

@@ -277,11 +277,10 @@ void hop_f32(hipStream_t s, const Geom& g, const float2* U, const float2* in, fl
                     partials, 0);
 }
 
-void hop_f16(hipStream_t s, const Geom& g, const __half2* U, const __half2* in, const float* in_norm, __half2* out,
-             float* out_norm, float a, float b, const __half2* dot_with, const float* dot_norm, CGState* st,
-             double* partials, bool r12) {
-  hop_launch<Half>(s, pick_variant(2, g), r12, g, U, in, in_norm, out, out_norm, a, 0.5f * b, dot_with, dot_norm,
-                   st, partials, 0);
+void hop_f16(hipStream_t s, const Geom& g, const __half2* U, const H16* in, H16* out, float a, float b,
+             const H16* dot_with, CGState* st, double* partials, bool r12) {
+  hop_launch<Half>(s, pick_variant(2, g), r12, g, U, in, nullptr, out, nullptr, a, 0.5f * b, dot_with, nullptr, st,
+                   partials, 0);
 }
 
 // ------------------------------------------------------------------ layout converters
@@ -339,11 +338,23 @@ __global__ void k_links_to_f32(size_t n, const double2* __restrict__ U, float2* 
   u[i] = make_float2((float)v.x, (float)v.y);
 }
 
-__global__ void k_links_to_f16(size_t n, const double2* __restrict__ U, __half2* __restrict__ u) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const double2 v = U[i];
-  u[i] = __floats2half2_rn((float)v.x, (float)v.y);
+// One lane per (site, direction): the nine elements in the packed FP16
+// layout of Prec<Half> (records of elements 0-3, 4-5, 6-7 and 8).
+__global__ void k_links_to_f16(Geom g, const double2* __restrict__ U, __half2* __restrict__ u) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int mu = blockIdx.y;
+  if (i >= g.V) return;
+  unsigned w[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) {
+    const double2 v = U[(size_t)(mu * 9 + e) * g.V + i];
+    w[e] = __builtin_bit_cast(unsigned, __floats2half2_rn((float)v.x, (float)v.y));
+  }
+  char* base = reinterpret_cast<char*>(u) + (size_t)mu * 36 * g.V;
+  reinterpret_cast<uint4*>(base)[i] = make_uint4(w[0], w[1], w[2], w[3]);
+  reinterpret_cast<uint2*>(base + (size_t)16 * g.V)[i] = make_uint2(w[4], w[5]);
+  reinterpret_cast<uint2*>(base + (size_t)24 * g.V)[i] = make_uint2(w[6], w[7]);
+  reinterpret_cast<unsigned*>(base + (size_t)32 * g.V)[i] = w[8];
 }
 
 // One lane per link: |row2 - conj(row0 x row1)| in FP64, the block maximum
@@ -401,8 +412,7 @@ void su3_deviation(hipStream_t s, const Geom& g, const double2* U, unsigned* dev
 }
 
 void links_to_f16(hipStream_t s, const Geom& g, const double2* U, __half2* u16) {
-  const size_t n = (size_t)36 * g.V;
-  hipLaunchKernelGGL(k_links_to_f16, dim3(grid1(n, 256)), dim3(256), 0, s, n, U, u16);
+  hipLaunchKernelGGL(k_links_to_f16, dim3(grid1(g.V, 256), 4), dim3(256), 0, s, g, U, u16);
 }
 
 }  // namespace sfh

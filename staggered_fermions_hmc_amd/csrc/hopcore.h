@@ -12,10 +12,23 @@ namespace sfh {
 
 struct Half {};
 
+// FP16 spinor of one site: three complex halves scaled by the site's
+// max-norm, and that norm, in one 16-byte record (one load, one store).
+struct alignas(16) H16 {
+  __half2 c[3];
+  float n;
+};
+
 template <class T> struct Prec;
 template <> struct Prec<double> { using R = double; using L = double2; using S = double2; };
 template <> struct Prec<float> { using R = float; using L = float2; using S = float2; };
-template <> struct Prec<Half> { using R = float; using L = __half2; using S = __half2; };
+// FP16 links: per direction mu a block of 36 V bytes holding, for each site,
+// elements 0-3 (16-byte record), 4-5 (8 bytes), 6-7 (8 bytes) and 8 (4 bytes)
+// in four arrays, so a two-row read is two loads and a full read four.
+template <> struct Prec<Half> { using R = float; using L = __half2; using S = H16; };
+
+// Spinor storage elements per site: three complex, or one FP16 record.
+template <class T> constexpr int kSpinorElems = std::is_same<T, Half>::value ? 1 : 3;
 
 __device__ __forceinline__ cpx<double> ldc(const double2* p) {
   const double2 v = *p;
@@ -30,13 +43,37 @@ __device__ __forceinline__ cpx<float> ldc(const __half2* p) {
   return {v.x, v.y};
 }
 
+__device__ __forceinline__ cpx<float> h2c(unsigned w) {
+  const float2 v = __half22float2(__builtin_bit_cast(__half2, w));
+  return {v.x, v.y};
+}
+
 // R12: read the first two rows only and rebuild the third as
 // conj(row0 x row1), exact for SU(3) links (two thirds of the link bytes).
 template <class T, bool R12 = false>
 __device__ __forceinline__ void load_link(const typename Prec<T>::L* __restrict__ U, int V, int mu, int i,
                                           mat3<typename Prec<T>::R>& m) {
+  if constexpr (std::is_same<T, Half>::value) {
+    const char* base = reinterpret_cast<const char*>(U) + (size_t)mu * 36 * V;
+    const uint4 a = reinterpret_cast<const uint4*>(base)[i];
+    const uint2 b = reinterpret_cast<const uint2*>(base + (size_t)16 * V)[i];
+    m.e[0] = h2c(a.x);
+    m.e[1] = h2c(a.y);
+    m.e[2] = h2c(a.z);
+    m.e[3] = h2c(a.w);
+    m.e[4] = h2c(b.x);
+    m.e[5] = h2c(b.y);
+    if constexpr (!R12) {
+      const uint2 c = reinterpret_cast<const uint2*>(base + (size_t)24 * V)[i];
+      const unsigned d = reinterpret_cast<const unsigned*>(base + (size_t)32 * V)[i];
+      m.e[6] = h2c(c.x);
+      m.e[7] = h2c(c.y);
+      m.e[8] = h2c(d);
+    }
+  } else {
 #pragma unroll
-  for (int e = 0; e < (R12 ? 6 : 9); ++e) m.e[e] = ldc(U + (size_t)(mu * 9 + e) * V + i);
+    for (int e = 0; e < (R12 ? 6 : 9); ++e) m.e[e] = ldc(U + (size_t)(mu * 9 + e) * V + i);
+  }
   if constexpr (R12) {
     m.e[6] = conj(m.e[1] * m.e[5] - m.e[2] * m.e[4]);
     m.e[7] = conj(m.e[2] * m.e[3] - m.e[0] * m.e[5]);
@@ -44,29 +81,39 @@ __device__ __forceinline__ void load_link(const typename Prec<T>::L* __restrict_
   }
 }
 
+// nrm: unused (the FP16 record carries its norm); kept so one call serves
+// every precision.
 template <class T>
 __device__ __forceinline__ void load_spinor(const typename Prec<T>::S* __restrict__ f, const float* __restrict__ nrm,
                                             int V, int i, vec3<typename Prec<T>::R>& v) {
-#pragma unroll
-  for (int c = 0; c < 3; ++c) v.c[c] = ldc(f + (size_t)c * V + i);
+  (void)nrm;
   if constexpr (std::is_same<T, Half>::value) {
-    const float s = nrm[i];
+    const H16 s = f[i];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) v.c[c] = s * v.c[c];
+    for (int c = 0; c < 3; ++c) {
+      const float2 t = __half22float2(s.c[c]);
+      v.c[c] = {s.n * t.x, s.n * t.y};
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v.c[c] = ldc(f + (size_t)c * V + i);
   }
 }
 
 template <class T>
 __device__ __forceinline__ void store_spinor(typename Prec<T>::S* __restrict__ f, float* __restrict__ nrm, int V,
                                              int i, const vec3<typename Prec<T>::R>& v) {
+  (void)nrm;
   if constexpr (std::is_same<T, Half>::value) {
     float m = 0.f;
 #pragma unroll
     for (int c = 0; c < 3; ++c) m = fmaxf(m, fmaxf(fabsf(v.c[c].re), fabsf(v.c[c].im)));
     const float inv = m > 0.f ? 1.0f / m : 0.f;
-    nrm[i] = m;
+    H16 s;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) f[(size_t)c * V + i] = __floats2half2_rn(v.c[c].re * inv, v.c[c].im * inv);
+    for (int c = 0; c < 3; ++c) s.c[c] = __floats2half2_rn(v.c[c].re * inv, v.c[c].im * inv);
+    s.n = m;
+    f[i] = s;
   } else {
     using S = typename Prec<T>::S;
 #pragma unroll

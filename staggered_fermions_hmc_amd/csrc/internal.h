@@ -8,6 +8,7 @@
 
 #include "../../include/sfh.h"
 #include "common.h"
+#include "hopcore.h"
 
 namespace sfh {
 
@@ -48,7 +49,7 @@ SFH_HD unsigned* ctr_of(unsigned* arena, int kind) { return arena + kind * kCtrP
 // Low-precision link / spinor storage (FP16 uses a per-site FP32 norm).
 struct LinksLP {
   float2* u32 = nullptr;  // 4*9*V
-  __half2* u16 = nullptr; // 4*9*V
+  __half2* u16 = nullptr; // 4*9*V, packed per direction (see Prec<Half>)
 };
 
 }  // namespace sfh
@@ -148,9 +149,8 @@ void hop_f64(hipStream_t s, const Geom& g, const double2* U, const double2* in, 
 // r12: the links are SU(3); read two rows and rebuild the third.
 void hop_f32(hipStream_t s, const Geom& g, const float2* U, const float2* in, float2* out, float a, float b,
              const float2* dot_with, CGState* st, double* partials, bool r12);
-void hop_f16(hipStream_t s, const Geom& g, const __half2* U, const __half2* in, const float* in_norm,
-             __half2* out, float* out_norm, float a, float b, const __half2* dot_with, const float* dot_norm,
-             CGState* st, double* partials, bool r12);
+void hop_f16(hipStream_t s, const Geom& g, const __half2* U, const H16* in, H16* out, float a, float b,
+             const H16* dot_with, CGState* st, double* partials, bool r12);
 
 // Hop kernel form: 0 one lane per site, 1 four lanes per site (one per
 // direction); -1 restores the size-based default.  prec: 0 f64, 1 f32, 2 f16.
@@ -167,7 +167,8 @@ void cg_p_f64(hipStream_t s, const Geom& g, double2* p, const double2* r, CGStat
 void cg_p_f32(hipStream_t s, const Geom& g, float2* p, const float2* r, CGState* st);
 // mixed-precision support
 void f64_to_f32(hipStream_t s, const double2* in, float2* out, size_t n);
-void f64_to_f16(hipStream_t s, const Geom& g, const double2* in, __half2* out, float* norm);
+void f64_to_f16(hipStream_t s, const Geom& g, const double2* in, H16* out);
+void h16_ones(hipStream_t s, H16* f, int V);  // every colour (1 + i)
 void acc_f32_into_f64(hipStream_t s, double2* xh, float2* xl, size_t n);  // xh += xl; xl = 0
 void xpay_f64(hipStream_t s, double2* y, const double2* x, double a, size_t n);  // y = x + a*y
 void axpy_f64(hipStream_t s, double2* y, const double2* x, double a, size_t n);  // y += a*x

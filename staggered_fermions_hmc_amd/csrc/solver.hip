@@ -218,27 +218,21 @@ bool mix_fused(int prec, const Geom& g) {
 // Sloppy vectors of a mixed-precision solve on lattice g.  Fused form:
 // kernel A recomputes r_k and p_k at the neighbours, so r and p are double
 // buffers; split form: in-place update, then the plain hop.
+// (FP16 vectors are one record per site that carries its norm; the norm
+// pointers stay null.)
 template <class T> int mix_bufs(sfh_ctx* c, const Geom& g, bool fused, MixBufs<T>& b) {
   using S = typename Prec<T>::S;
-  constexpr bool half = std::is_same<T, Half>::value;
   const size_t n = 3 * (size_t)g.V;
   const char* names[6] = {"mx_r0", "mx_p0", "mx_v", "mx_t", "mx_r1", "mx_p1"};
   S* vecs[6] = {};
-  float* norms[6] = {};
   for (int k = 0; k < (fused ? 6 : 4); ++k) {
-    vecs[k] = buf<S>(c, names[k], n);
+    vecs[k] = buf<S>(c, names[k], (size_t)kSpinorElems<T> * g.V);
     if (!vecs[k]) return SFH_E_NOMEM;
-    if (half) {
-      norms[k] = buf<float>(c, (std::string(names[k]) + "n").c_str(), g.V);
-      if (!norms[k]) return SFH_E_NOMEM;
-    }
   }
   b.r[0] = vecs[0]; b.p[0] = vecs[1]; b.v = vecs[2]; b.t = vecs[3];
   b.r[1] = fused ? vecs[4] : vecs[0];
   b.p[1] = fused ? vecs[5] : vecs[1];
-  b.rn[0] = norms[0]; b.pn[0] = norms[1]; b.vn = norms[2]; b.tn = norms[3];
-  b.rn[1] = fused ? norms[4] : norms[0];
-  b.pn[1] = fused ? norms[5] : norms[1];
+  b.rn[0] = b.rn[1] = b.pn[0] = b.pn[1] = b.vn = b.tn = nullptr;
   b.x = buf<float2>(c, "cgl_x", n);
   return b.x ? SFH_OK : SFH_E_NOMEM;
 }
@@ -266,8 +260,8 @@ void mix_step_a(sfh_ctx* c, const sfh_gauge* gauge, bool fused, const MixBufs<T>
   }
   KTimer kt(c, half ? "hop_f16" : "hop_f32");
   if (half)
-    hop_f16(c->stream, g, gauge->lp.u16, reinterpret_cast<const __half2*>(b.p[0]), b.pn[0],
-            reinterpret_cast<__half2*>(b.t), b.tn, m, -1.f, nullptr, nullptr, st, nullptr, gauge->lp_su3);
+    hop_f16(c->stream, g, gauge->lp.u16, reinterpret_cast<const H16*>(b.p[0]), reinterpret_cast<H16*>(b.t), m, -1.f,
+            nullptr, st, nullptr, gauge->lp_su3);
   else
     hop_f32(c->stream, g, gauge->lp.u32, reinterpret_cast<const float2*>(b.p[0]), reinterpret_cast<float2*>(b.t), m,
             -1.f, nullptr, st, nullptr, gauge->lp_su3);
@@ -357,13 +351,12 @@ static int cg_mixed(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* prm, cons
   if (int e = mix_bufs<T>(c, g, fused, b)) return e;
   if (!r64 || !t64 || !ax || !b.x) return SFH_E_NOMEM;
   // iteration 0 reads r_{-1} = phi from buffer 1, p_{-1} = v_{-1} = 0, alpha = beta = 0
+  const size_t ns = (size_t)kSpinorElems<T> * g.V;
   hipMemsetAsync(b.x, 0, n * sizeof(float2), c->stream);
-  hipMemsetAsync(b.p[1], 0, n * sizeof(S), c->stream);
-  hipMemsetAsync(b.v, 0, n * sizeof(S), c->stream);
+  hipMemsetAsync(b.p[1], 0, ns * sizeof(S), c->stream);
+  hipMemsetAsync(b.v, 0, ns * sizeof(S), c->stream);
   if (HALF) {
-    hipMemsetAsync(b.pn[1], 0, g.V * sizeof(float), c->stream);
-    hipMemsetAsync(b.vn, 0, g.V * sizeof(float), c->stream);
-    f64_to_f16(c->stream, g, phi, reinterpret_cast<__half2*>(b.r[1]), b.rn[1]);
+    f64_to_f16(c->stream, g, phi, reinterpret_cast<H16*>(b.r[1]));
   } else {
     f64_to_f32(c->stream, phi, reinterpret_cast<float2*>(b.r[1]), n);
   }
@@ -482,16 +475,11 @@ static int time_mixed_t(sfh_ctx* c, sfh_gauge* gauge, int n, double* us_a, doubl
   if (int e = mix_bufs<T>(c, g, fused, b)) return e;
   for (int k = 0; k < 2; ++k) {
     for (void* v : {(void*)b.r[k], (void*)b.p[k]}) {
-      if (half) hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(v), 0x3C003C00u, n3, c->stream);  // (1, 1)
-      else hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(v), 0x3F800000u, 2 * n3, c->stream);   // 1.0f
-    }
-    if (half) {
-      hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(b.rn[k]), 0x3F800000u, g.V, c->stream);
-      hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(b.pn[k]), 0x3F800000u, g.V, c->stream);
+      if (half) h16_ones(c->stream, static_cast<H16*>(v), g.V);
+      else hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(v), 0x3F800000u, 2 * n3, c->stream);  // 1.0f
     }
   }
-  hipMemsetAsync(b.v, 0, n3 * sizeof(typename Prec<T>::S), c->stream);
-  if (half) hipMemsetAsync(b.vn, 0, g.V * sizeof(float), c->stream);
+  hipMemsetAsync(b.v, 0, (size_t)kSpinorElems<T> * g.V * sizeof(typename Prec<T>::S), c->stream);
   hipMemsetAsync(b.x, 0, n3 * sizeof(float2), c->stream);
   // alpha = beta = 0 keeps kernel A idempotent; target 0 and no budget keep B running
   CGState z{};
