@@ -26,6 +26,8 @@
 // that layout, runs its sweeps, and converts them back.
 //
 // About 87 SU(3) products (84 for the staples) per link.
+#include <cstring>
+
 #include "internal.h"
 #include "reduce.h"
 #include "staples.h"
@@ -192,7 +194,11 @@ __device__ __forceinline__ void subgroup(M& u, const M& A, double beta, Draws& r
 
 // Heatbath layout of the links: x3 = 4 q + r, site index
 // (((r L0 + x0) L1 + x1) L2 + x2) Q + q with Q = L3 / 4.
+// TWO: SU(3) links kept as two rows (a third less to read per staple link),
+// the third rebuilt on every load.
+template <bool TWO>
 struct HbIndex {
+  static constexpr bool kTwoRows = TWO;
   __device__ __forceinline__ static int of(const Geom& g, const Site4& x) {
     const int Q = g.L[3] >> 2;
     return ((((x.x3 & 3) * g.L[0] + x.x0) * g.L[1] + x.x1) * g.L[2] + x.x2) * Q + (x.x3 >> 2);
@@ -226,6 +232,7 @@ __device__ __forceinline__ Site4 class_site(const Geom& g, int mu, int col, int 
   return Site4{x[0], x[1], x[2], x[3]};
 }
 
+template <class IX>
 __global__ __launch_bounds__(kHbThreads) void k_heatbath(Geom g, double2* __restrict__ U, int mu, int col,
                                                          double beta, double c11, double c12, uint32_t k0,
                                                          uint32_t k1, uint32_t sweep, double* __restrict__ part,
@@ -238,12 +245,12 @@ __global__ __launch_bounds__(kHbThreads) void k_heatbath(Geom g, double2* __rest
   if (t < (g.V >> 2)) {
     int lex;
     const Site4 n = class_site(g, mu, col, t, lex);
-    const int i = HbIndex::of(g, n);
+    const int i = IX::of(g, n);
     // A1 waits in LDS (lane-interleaved, no bank conflicts) while the
     // rectangle walks run, so the walks keep two waves per SIMD
     M A;
     {
-      const M A1 = staples<HbIndex>(U, g, n, mu, 1.0, 0.0);
+      const M A1 = staples<IX>(U, g, n, mu, 1.0, 0.0);
 #pragma unroll
       for (int e = 0; e < 9; ++e) {
         sA1[2 * e][threadIdx.x] = A1.e[e].re;
@@ -251,8 +258,8 @@ __global__ __launch_bounds__(kHbThreads) void k_heatbath(Geom g, double2* __rest
       }
       A = mscale(c11, A1);
     }
-    if (c12 != 0.0) A = madd(A, mscale(c12, staples<HbIndex>(U, g, n, mu, 0.0, 1.0)));
-    M u = ldL(U, g, i, mu);
+    if (c12 != 0.0) A = madd(A, mscale(c12, staples<IX>(U, g, n, mu, 0.0, 1.0)));
+    M u = ldLx<IX>(U, g, i, mu);
     Draws rs{Philox{k0, k1}, 4u * (uint32_t)lex + (uint32_t)mu, sweep, 0u, 0, 0.0, 0};
     subgroup<0, 1>(u, A, beta, rs);
     subgroup<0, 2>(u, A, beta, rs);
@@ -261,7 +268,7 @@ __global__ __launch_bounds__(kHbThreads) void k_heatbath(Geom g, double2* __rest
       // the link keeps its value and the call fails
       atomicMax(status, (int)SFH_E_SAMPLER);
     } else {
-      stL(U, g, i, mu, u);
+      stLx<IX>(U, g, i, mu, u);
       M A1;
 #pragma unroll
       for (int e = 0; e < 9; ++e) A1.e[e] = {sA1[2 * e][threadIdx.x], sA1[2 * e + 1][threadIdx.x]};
@@ -277,15 +284,16 @@ __global__ __launch_bounds__(kHbThreads) void k_heatbath(Geom g, double2* __rest
 }
 
 // Links between the solver's even-odd layout and the heatbath layout, one
-// lane per site; to_hb selects the direction of the copy.
+// lane per (site, direction); to_hb selects the direction of the copy.
+template <class IX>
 __global__ __launch_bounds__(256) void k_hb_layout(Geom g, const double2* __restrict__ src,
                                                    double2* __restrict__ dst, int to_hb) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int mu = blockIdx.y;
   if (i >= g.V) return;
-  const int h = HbIndex::of(g, site_of(g, i));
-  const int from = to_hb ? i : h, to = to_hb ? h : i;
-#pragma unroll 4
-  for (int k = 0; k < 36; ++k) dst[(size_t)k * g.V + to] = src[(size_t)k * g.V + from];
+  const int h = IX::of(g, site_of(g, i));
+  if (to_hb) stLx<IX>(dst, g, h, mu, ldL(src, g, i, mu));
+  else stL(dst, g, i, mu, ldLx<IX>(src, g, h, mu));
 }
 
 // Sum of a sweep's n workgroup partials, in a fixed order.
@@ -311,17 +319,15 @@ bool heatbath_dims_ok(const Geom& g) {
   return true;
 }
 
-int heatbath(sfh_ctx* c, const Geom& g, double2* U, double beta, double c11, double c12, unsigned long long seed,
-             unsigned sweep0, int nsweeps, double* plaq, double* act) {
-  if (nsweeps == 0) return SFH_OK;
+template <class IX>
+static int heatbath_sweeps(sfh_ctx* c, const Geom& g, double2* U, double2* H, double* part, double* sums, double beta,
+                    double c11, double c12, unsigned long long seed, unsigned sweep0, int nsweeps, double* plaq,
+                    double* act) {
   const int nb = ((g.V >> 2) + kHbThreads - 1) / kHbThreads;
-  double* part = static_cast<double*>(scratch(c, "hb_part", 2 * 16 * (size_t)nb * sizeof(double)));
-  double* sums = static_cast<double*>(scratch(c, "hb_sums", 2 * (size_t)nsweeps * sizeof(double)));
-  double2* H = static_cast<double2*>(scratch(c, "hb_links", 36 * (size_t)g.V * sizeof(double2)));
-  if (!part || !sums || !H) return SFH_E_NOMEM;
+  const dim3 lgrid((g.V + 255) / 256, 4);
   {
     KTimer kt(c, "heatbath_layout");
-    hipLaunchKernelGGL(k_hb_layout, dim3((g.V + 255) / 256), dim3(256), 0, c->stream, g, U, H, 1);
+    hipLaunchKernelGGL(k_hb_layout<IX>, lgrid, dim3(256), 0, c->stream, g, U, H, 1);
   }
   const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
   int* d_status = &c->d_state[1].status;
@@ -331,7 +337,7 @@ int heatbath(sfh_ctx* c, const Geom& g, double2* U, double beta, double c11, dou
       KTimer kt(c, "heatbath");
       for (int mu = 0; mu < 4; ++mu)
         for (int col = 0; col < 4; ++col)
-          hipLaunchKernelGGL(k_heatbath, dim3(nb), dim3(kHbThreads), 0, c->stream, g, H, mu, col, beta, c11, c12,
+          hipLaunchKernelGGL(k_heatbath<IX>, dim3(nb), dim3(kHbThreads), 0, c->stream, g, H, mu, col, beta, c11, c12,
                              k0, k1, (uint32_t)(sweep0 + s), part + 2 * (size_t)nb * (mu * 4 + col), d_status,
                              nb);
     }
@@ -340,7 +346,7 @@ int heatbath(sfh_ctx* c, const Geom& g, double2* U, double beta, double c11, dou
   }
   {
     KTimer kt(c, "heatbath_layout");
-    hipLaunchKernelGGL(k_hb_layout, dim3((g.V + 255) / 256), dim3(256), 0, c->stream, g, H, U, 0);
+    hipLaunchKernelGGL(k_hb_layout<IX>, lgrid, dim3(256), 0, c->stream, g, H, U, 0);
   }
   if (int e = hip_check(c, hipGetLastError(), "heatbath layout")) return e;
   std::vector<double> h(2 * (size_t)nsweeps);
@@ -356,6 +362,30 @@ int heatbath(sfh_ctx* c, const Geom& g, double2* U, double beta, double c11, dou
     if (act) act[s] = (h[2 * s + 1] / 4.0) / g.V;
   }
   return SFH_OK;
+}
+
+
+int heatbath(sfh_ctx* c, const Geom& g, double2* U, double beta, double c11, double c12, unsigned long long seed,
+             unsigned sweep0, int nsweeps, double* plaq, double* act) {
+  if (nsweeps == 0) return SFH_OK;
+  const int nb = ((g.V >> 2) + kHbThreads - 1) / kHbThreads;
+  double* part = static_cast<double*>(scratch(c, "hb_part", 2 * 16 * (size_t)nb * sizeof(double)));
+  double* sums = static_cast<double*>(scratch(c, "hb_sums", 2 * (size_t)nsweeps * sizeof(double)));
+  double2* H = static_cast<double2*>(scratch(c, "hb_links", 36 * (size_t)g.V * sizeof(double2)));
+  unsigned* dev = static_cast<unsigned*>(scratch(c, "su3_dev", sizeof(unsigned)));
+  if (!part || !sums || !H || !dev) return SFH_E_NOMEM;
+  // SU(3) links are kept as two rows in the heatbath copy
+  hipMemsetAsync(dev, 0, sizeof(unsigned), c->stream);
+  su3_deviation(c->stream, g, U, dev);
+  unsigned* hd = reinterpret_cast<unsigned*>(c->h_scalar);
+  hipMemcpyAsync(hd, dev, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream);
+  if (int e = hip_check(c, hipStreamSynchronize(c->stream), "SU(3) check")) return e;
+  float d;
+  std::memcpy(&d, hd, sizeof d);
+  return d < kSu3Tol ? heatbath_sweeps<HbIndex<true>>(c, g, U, H, part, sums, beta, c11, c12, seed, sweep0, nsweeps,
+                                                      plaq, act)
+                     : heatbath_sweeps<HbIndex<false>>(c, g, U, H, part, sums, beta, c11, c12, seed, sweep0, nsweeps,
+                                                       plaq, act);
 }
 
 }  // namespace sfh

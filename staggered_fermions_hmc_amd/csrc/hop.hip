@@ -370,13 +370,16 @@ __global__ __launch_bounds__(256) void k_su3_dev(Geom g, const double2* __restri
     for (int e = 0; e < 9; ++e) m[e] = ldc(U + (size_t)(mu * 9 + e) * g.V + i);
     const cpx<double> r[3] = {conj(m[1] * m[5] - m[2] * m[4]), conj(m[2] * m[3] - m[0] * m[5]),
                               conj(m[0] * m[4] - m[1] * m[3])};
+    // fmax drops NaN operands, so non-finite entries are flagged apart
     double x = 0.0;
+    bool finite = true;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const cpx<double> e = m[6 + k] - r[k];
+      finite = finite && isfinite(e.re) && isfinite(e.im);
       x = fmax(x, fmax(fabs(e.re), fabs(e.im)));
     }
-    d = x == x ? (float)x : __int_as_float(0x7f800000);
+    d = finite ? (float)x : __int_as_float(0x7f800000);
   }
   __shared__ float w[4];
   for (int o = 32; o > 0; o >>= 1) d = fmaxf(d, __shfl_xor(d, o));

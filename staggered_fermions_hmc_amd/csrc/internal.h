@@ -136,8 +136,9 @@ void spinor_soa_to_aos(hipStream_t s, const Geom& g, const double2* soa, double*
 void links_to_f32(hipStream_t s, const Geom& g, const double2* U, float2* u32);
 void links_to_f16(hipStream_t s, const Geom& g, const double2* U, __half2* u16);
 // max over links of |row2 - conj(row0 x row1)| as the bits of a float, into
-// *dev (which must start at 0)
+// *dev (which must start at 0); links below kSu3Tol may be read as two rows
 void su3_deviation(hipStream_t s, const Geom& g, const double2* U, unsigned* dev);
+constexpr float kSu3Tol = 1e-10f;
 
 // ------------------------------------------------------------------ hop (hop.hip)
 // out = a*in + (b/2) * sum_mu eta [bup U psi(n+mu) - bdn U^dag psi(n-mu)]

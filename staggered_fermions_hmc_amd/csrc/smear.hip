@@ -26,6 +26,8 @@
 // (range_start), so x0- and x1-neighbour ranges are also processed close in
 // time.  FP64 throughout; level 2 (four staples of
 // level-1 links per output, 12 outputs per site) dominates.
+#include <cstring>
+
 #include "internal.h"
 #include "staples.h"
 #include "su3exp.h"
@@ -38,19 +40,50 @@ constexpr int kHexThreads = 2 * kRangeHalf;
 
 __device__ __forceinline__ int pair_slot(int a, int b) { return 3 * a + (b < a ? b : b - 1); }
 
-// Link of pair slot k at site i of a level array W[(k * 9 + e) * V + i].
+// TWO: the thin links are SU(3), and so are the level links (exp of an
+// anti-hermitian matrix times an SU(3) link): level links are stored as two
+// rows and every link load reads two rows and rebuilds the third.
+__device__ __forceinline__ void third_row(M& m) {
+  m.e[6] = conj(m.e[1] * m.e[5] - m.e[2] * m.e[4]);
+  m.e[7] = conj(m.e[2] * m.e[3] - m.e[0] * m.e[5]);
+  m.e[8] = conj(m.e[0] * m.e[4] - m.e[1] * m.e[3]);
+}
+
+// Link of pair slot k at site i of a level array W[(k * n + e) * V + i],
+// n = 6 (TWO) or 9 elements per link.
+template <bool TWO>
 __device__ __forceinline__ M ldW(const double2* __restrict__ W, const Geom& g, int i, int k) {
+  constexpr int n = TWO ? 6 : 9;
   M m;
 #pragma unroll
-  for (int e = 0; e < 9; ++e) {
-    const double2 v = W[(size_t)(k * 9 + e) * g.V + i];
+  for (int e = 0; e < n; ++e) {
+    const double2 v = W[(size_t)(k * n + e) * g.V + i];
     m.e[e] = {v.x, v.y};
   }
+  if constexpr (TWO) third_row(m);
   return m;
 }
+template <bool TWO>
 __device__ __forceinline__ void stW(double2* __restrict__ W, const Geom& g, int i, int k, const M& m) {
+  constexpr int n = TWO ? 6 : 9;
 #pragma unroll
-  for (int e = 0; e < 9; ++e) W[(size_t)(k * 9 + e) * g.V + i] = make_double2(m.e[e].re, m.e[e].im);
+  for (int e = 0; e < n; ++e) W[(size_t)(k * n + e) * g.V + i] = make_double2(m.e[e].re, m.e[e].im);
+}
+// Thin link (all nine elements stored; two rows read when TWO).
+template <bool TWO>
+__device__ __forceinline__ M ldT(const double2* __restrict__ U, const Geom& g, int i, int mu) {
+  if constexpr (TWO) {
+    M m;
+#pragma unroll
+    for (int e = 0; e < 6; ++e) {
+      const double2 v = U[(size_t)(mu * 9 + e) * g.V + i];
+      m.e[e] = {v.x, v.y};
+    }
+    third_row(m);
+    return m;
+  } else {
+    return ldL(U, g, i, mu);
+  }
 }
 
 // Up plus down staple of link (x, mu) in the (mu, sig) plane; L(i, d) is the
@@ -108,6 +141,7 @@ __device__ __forceinline__ int hex_lane(const Geom& g, int nk, int nb, int& k) {
   return range_site(g, lb / nk, threadIdx.x);
 }
 
+template <bool TWO>
 __global__ __launch_bounds__(kHexThreads) void k_hex1(Geom g, const double2* __restrict__ U,
                                                       double2* __restrict__ V1, double alpha, int* status, int nb) {
   int k;
@@ -115,13 +149,14 @@ __global__ __launch_bounds__(kHexThreads) void k_hex1(Geom g, const double2* __r
   if (i < 0) return;
   const int mu = k / 3, r = k % 3, sig = r < mu ? r : r + 1;
   const Site4 x = site_of(g, i);
-  const M G = staple_pair(g, x, mu, sig, [&](int j, int d) { return ldL(U, g, j, d); });
+  const M G = staple_pair(g, x, mu, sig, [&](int j, int d) { return ldT<TWO>(U, g, j, d); });
   M out;
-  const int st = stout(G, ldL(U, g, i, mu), alpha, out);
+  const int st = stout(G, ldT<TWO>(U, g, i, mu), alpha, out);
   if (st != SFH_OK) atomicMax(status, st);
-  else stW(V1, g, i, k, out);
+  else stW<TWO>(V1, g, i, k, out);
 }
 
+template <bool TWO>
 __global__ __launch_bounds__(kHexThreads) void k_hex2(Geom g, const double2* __restrict__ U,
                                                       const double2* __restrict__ V1, double2* __restrict__ V2,
                                                       double alpha, int* status, int nb) {
@@ -135,16 +170,17 @@ __global__ __launch_bounds__(kHexThreads) void k_hex2(Geom g, const double2* __r
   for (int sig = 0; sig < 4; ++sig) {
     if (sig == mu || sig == nu) continue;
     const int rho = 6 - mu - nu - sig;
-    G = madd(G, staple_pair(g, x, mu, sig, [&](int j, int d) { return ldW(V1, g, j, pair_slot(d, rho)); }));
+    G = madd(G, staple_pair(g, x, mu, sig, [&](int j, int d) { return ldW<TWO>(V1, g, j, pair_slot(d, rho)); }));
   }
   M out;
-  const int st = stout(G, ldL(U, g, i, mu), alpha, out);
+  const int st = stout(G, ldT<TWO>(U, g, i, mu), alpha, out);
   if (st != SFH_OK) atomicMax(status, st);
-  else stW(V2, g, i, k, out);
+  else stW<TWO>(V2, g, i, k, out);
 }
 
 // Level 3 reads only its own thin link besides level-2 links, so it updates
 // U in place.
+template <bool TWO>
 __global__ __launch_bounds__(kHexThreads) void k_hex3(Geom g, double2* __restrict__ U,
                                                       const double2* __restrict__ V2, double alpha, int* status,
                                                       int nb) {
@@ -157,38 +193,36 @@ __global__ __launch_bounds__(kHexThreads) void k_hex3(Geom g, double2* __restric
   for (int nu = 0; nu < 4; ++nu) {
     if (nu == mu) continue;
     G = madd(G, staple_pair(g, x, mu, nu,
-                            [&](int j, int d) { return ldW(V2, g, j, pair_slot(d, d == mu ? nu : mu)); }));
+                            [&](int j, int d) { return ldW<TWO>(V2, g, j, pair_slot(d, d == mu ? nu : mu)); }));
   }
   M out;
-  const int st = stout(G, ldL(U, g, i, mu), alpha, out);
+  const int st = stout(G, ldT<TWO>(U, g, i, mu), alpha, out);
   if (st != SFH_OK) atomicMax(status, st);
   else stL(U, g, i, mu, out);
 }
 
 }  // namespace
 
-int hex_smear(sfh_ctx* c, const Geom& g, double2* U, const double alpha[3], int nsteps) {
-  if (nsteps == 0) return SFH_OK;
-  double2* V1 = static_cast<double2*>(scratch(c, "hex_v1", 108 * (size_t)g.V * sizeof(double2)));
-  double2* V2 = static_cast<double2*>(scratch(c, "hex_v2", 108 * (size_t)g.V * sizeof(double2)));
-  if (!V1 || !V2) return SFH_E_NOMEM;
+template <bool TWO>
+static int hex_steps(sfh_ctx* c, const Geom& g, double2* U, double2* V1, double2* V2, const double alpha[3],
+                     int nsteps) {
   int* d_status = &c->d_state[1].status;
   hipMemsetAsync(d_status, 0, sizeof(int), c->stream);
   const int nb = (g.Vh + kRangeHalf - 1) / kRangeHalf;  // site ranges
   for (int n = 0; n < nsteps; ++n) {
     {
       KTimer kt(c, "hex1");
-      hipLaunchKernelGGL(k_hex1, dim3(12 * nb), dim3(kHexThreads), 0, c->stream, g, U, V1, alpha[2], d_status,
+      hipLaunchKernelGGL(k_hex1<TWO>, dim3(12 * nb), dim3(kHexThreads), 0, c->stream, g, U, V1, alpha[2], d_status,
                          12 * nb);
     }
     {
       KTimer kt(c, "hex2");
-      hipLaunchKernelGGL(k_hex2, dim3(12 * nb), dim3(kHexThreads), 0, c->stream, g, U, V1, V2, alpha[1], d_status,
-                         12 * nb);
+      hipLaunchKernelGGL(k_hex2<TWO>, dim3(12 * nb), dim3(kHexThreads), 0, c->stream, g, U, V1, V2, alpha[1],
+                         d_status, 12 * nb);
     }
     {
       KTimer kt(c, "hex3");
-      hipLaunchKernelGGL(k_hex3, dim3(4 * nb), dim3(kHexThreads), 0, c->stream, g, U, V2, alpha[0], d_status,
+      hipLaunchKernelGGL(k_hex3<TWO>, dim3(4 * nb), dim3(kHexThreads), 0, c->stream, g, U, V2, alpha[0], d_status,
                          4 * nb);
     }
     if (int e = hip_check(c, hipGetLastError(), "hex smearing launch")) return e;
@@ -198,6 +232,23 @@ int hex_smear(sfh_ctx* c, const Geom& g, double2* U, const double alpha[3], int 
   if (int e = hip_check(c, hipStreamSynchronize(c->stream), "hex smearing")) return e;
   if (*hs != SFH_OK) return set_err(c, *hs, "hex smearing: non-finite links");
   return SFH_OK;
+}
+
+int hex_smear(sfh_ctx* c, const Geom& g, double2* U, const double alpha[3], int nsteps) {
+  if (nsteps == 0) return SFH_OK;
+  double2* V1 = static_cast<double2*>(scratch(c, "hex_v1", 108 * (size_t)g.V * sizeof(double2)));
+  double2* V2 = static_cast<double2*>(scratch(c, "hex_v2", 108 * (size_t)g.V * sizeof(double2)));
+  unsigned* dev = static_cast<unsigned*>(scratch(c, "su3_dev", sizeof(unsigned)));
+  if (!V1 || !V2 || !dev) return SFH_E_NOMEM;
+  // SU(3) thin links (one check per call: smearing keeps them SU(3)): two rows
+  hipMemsetAsync(dev, 0, sizeof(unsigned), c->stream);
+  su3_deviation(c->stream, g, U, dev);
+  unsigned* hd = reinterpret_cast<unsigned*>(c->h_scalar);
+  hipMemcpyAsync(hd, dev, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream);
+  if (int e = hip_check(c, hipStreamSynchronize(c->stream), "SU(3) check")) return e;
+  float d;
+  std::memcpy(&d, hd, sizeof d);
+  return d < kSu3Tol ? hex_steps<true>(c, g, U, V1, V2, alpha, nsteps) : hex_steps<false>(c, g, U, V1, V2, alpha, nsteps);
 }
 
 }  // namespace sfh

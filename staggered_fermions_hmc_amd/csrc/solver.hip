@@ -50,7 +50,6 @@ double dev_dot(sfh_ctx* c, const double2* a, const double2* b, size_t ncomplex) 
 // are SU(3) (third row = conj(row0 x row1) to 1e-10, one pass over the links
 // and one host sync); only then do the low-precision kernels read two rows
 // and rebuild the third.
-constexpr float kSu3Tol = 1e-10f;
 int ensure_lp(sfh_ctx* c, sfh_gauge* gauge, int precision) {
   const Geom& g = gauge->g;
   if ((precision == SFH_PREC_F32 && !gauge->lp32_valid) || (precision == SFH_PREC_F16 && !gauge->lp16_valid)) {

@@ -30,8 +30,10 @@ struct Site4 {
   }
 };
 
-// Where a site's links sit in the links array: the solver's even-odd order.
+// Where a site's links sit in the links array: the solver's even-odd order,
+// all nine elements stored.
 struct EoIndex {
+  static constexpr bool kTwoRows = false;
   __device__ __forceinline__ static int of(const Geom& g, const Site4& x) { return x.index(g); }
 };
 
@@ -66,6 +68,36 @@ __device__ __forceinline__ M ldL(const double2* __restrict__ U, const Geom& g, i
 __device__ __forceinline__ void stL(double2* __restrict__ U, const Geom& g, int i, int mu, const M& m) {
 #pragma unroll
   for (int e = 0; e < 9; ++e) U[(size_t)(mu * 9 + e) * g.V + i] = make_double2(m.e[e].re, m.e[e].im);
+}
+
+// Links of a layout policy IX: all nine elements at (mu * 9 + e) V + i, or,
+// for SU(3) links kept as two rows (IX::kTwoRows), six at (mu * 6 + e) V + i
+// with the third row rebuilt as conj(row0 x row1).
+template <class IX>
+__device__ __forceinline__ M ldLx(const double2* __restrict__ U, const Geom& g, int i, int mu) {
+  if constexpr (IX::kTwoRows) {
+    M m;
+#pragma unroll
+    for (int e = 0; e < 6; ++e) {
+      const double2 v = U[(size_t)(mu * 6 + e) * g.V + i];
+      m.e[e] = {v.x, v.y};
+    }
+    m.e[6] = conj(m.e[1] * m.e[5] - m.e[2] * m.e[4]);
+    m.e[7] = conj(m.e[2] * m.e[3] - m.e[0] * m.e[5]);
+    m.e[8] = conj(m.e[0] * m.e[4] - m.e[1] * m.e[3]);
+    return m;
+  } else {
+    return ldL(U, g, i, mu);
+  }
+}
+template <class IX>
+__device__ __forceinline__ void stLx(double2* __restrict__ U, const Geom& g, int i, int mu, const M& m) {
+  if constexpr (IX::kTwoRows) {
+#pragma unroll
+    for (int e = 0; e < 6; ++e) U[(size_t)(mu * 6 + e) * g.V + i] = make_double2(m.e[e].re, m.e[e].im);
+  } else {
+    stL(U, g, i, mu, m);
+  }
 }
 
 // t <- t m and t <- t m^dag, one row at a time: only t, m and three entries
@@ -127,7 +159,7 @@ __device__ __forceinline__ M walk(const double2* __restrict__ U, const Geom& g, 
     const int d = path[k].d ? nu : mu;
     const int s = path[k].s;
     if (s < 0) x.move(g, d, -1);
-    const M m = ldL(U, g, IX::of(g, x), d);
+    const M m = ldLx<IX>(U, g, IX::of(g, x), d);
     if (s > 0) x.move(g, d, +1);
     if (k == 0) t = s > 0 ? m : mdag(m);
     else if (s > 0) rmul_n(t, m);

@@ -68,6 +68,19 @@ def test_heatbath_strong_coupling_sweep_matches_oracle(ctx, dims):
     assert np.allclose(pl, plr, rtol=1e-10, atol=0) and np.allclose(ac, acr, rtol=1e-10, atol=0)
 
 
+def test_heatbath_non_su3_links_match_oracle(ctx):
+    """SU(3) links are kept as two rows in the heatbath's copy; links scaled
+    by 0.9 are not SU(3) and keep all three rows, and still match the
+    oracle (which makes no assumption about the links)."""
+    dims = (4, 4, 4, 8)
+    U = 0.9 * dev.random_links(dims, 0.6, seed=33)
+    g = dev.Gauge(ctx, dims, U)
+    pl, ac = g.heatbath(*SYMANZIK, seed=5, sweep0=1, nsweeps=1)
+    Uref, plr, acr = oc.heatbath(dims, U, *SYMANZIK, seed=5, sweep0=1, nsweeps=1, order=1)
+    assert np.abs(g.download() - Uref).max() < 1e-12
+    assert np.allclose(pl, plr, rtol=1e-12, atol=0) and np.allclose(ac, acr, rtol=1e-12, atol=0)
+
+
 def test_heatbath_cold_start_matches_oracle(ctx):
     dims = (4, 4, 4, 8)
     U = _cold(dims)

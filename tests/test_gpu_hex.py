@@ -41,6 +41,18 @@ def test_hex_matches_oracle(ctx, dims, alpha, nsteps):
     assert np.abs(g.download() - ref).max() < 1e-12
 
 
+def test_hex_non_su3_links_match_oracle(ctx):
+    """SU(3) links make every level SU(3), and the kernels then store and read
+    two rows per link; links scaled by 0.9 are not SU(3), take the nine-element
+    path, and still match the oracle (which makes no such assumption)."""
+    dims = (6, 4, 4, 8)
+    U = 0.9 * dev.random_links(dims, 0.6, seed=43)
+    g = dev.Gauge(ctx, dims, U)
+    g.hex_smear(ALPHA, 2)
+    ref = oc.hex_smear(dims, U, alpha=ALPHA, nsteps=2)
+    assert np.abs(g.download() - ref).max() < 1e-12
+
+
 def test_hex_cold_links_fixed_and_zero_steps_noop(ctx):
     dims = (16, 16, 16, 16)
     U = np.zeros(int(np.prod(dims)) * 72)
