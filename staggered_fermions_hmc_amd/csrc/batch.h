@@ -65,7 +65,7 @@ template <class T> class Engine {
   // of solves is stable for many chunks, as in a fixed batch); otherwise
   // launch the chunk's kernels directly.  A chunk is `chunk` iterations of
   // every active solve, then (mixed precision) the reliable update of every
-  // solve that paused for one, then one copy of all solve states to the host.
+  // solve that has one due, then one copy of all solve states to the host.
   Engine(sfh_ctx* c, double tol, int max_iter, double delta, int chunk, bool use_graph = true);
   ~Engine();
   Engine(const Engine&) = delete;

@@ -14,10 +14,12 @@
 // * A chunk of iterations is captured into a HIP graph and replayed until the
 //   set of active slots changes.
 // * Mixed precision (FP32 inner): a slot whose iterated residual has dropped
-//   by the reliable-update factor pauses; the chunk's tail runs the FP64
-//   reliable update of every paused slot on the device (x accumulated in
-//   FP64, true residual with the FP64 links, restart or stop) while the
-//   others keep their place.  All states then reach the host in one copy.
+//   by the reliable-update factor has an update due and iterates on to the
+//   chunk's end (one that meets the target or its budget pauses); the
+//   chunk's tail runs the FP64 reliable update of every such slot on the
+//   device (x accumulated in FP64, true residual with the FP64 links,
+//   restart or stop) while the others keep their place.  All states then
+//   reach the host in one copy.
 // * Slots can be admitted between chunks (continuous batching): the solve
 //   service feeds the engine as requests arrive; each system owns a fixed
 //   pool of slot memory, so admission is by free memory.
@@ -221,11 +223,11 @@ __global__ __launch_bounds__(kBlas) void kb_p(const BSlot<T>* __restrict__ slots
   }
 }
 
-// Reliable update of every paused mixed solve, on the device (gated on each
-// slot's state, so solves that did not pause cost one state read):
+// Reliable update of every mixed solve that has one due, on the device (gated
+// on each slot's state, so the other solves cost one state read):
 //   kb_ru_acc   x64 += x + alpha p, x = 0
 //   kb_ru_hop   t64 = D^dag x64, then ax = D t64 (FP64 links, loaded once per
-//               block for all paused right-hand sides of the system)
+//               block for all such right-hand sides of the system)
 //   kb_ru_sub   r64 = b - ax, |r64|^2 -> FinRU (restart, converged or out of budget)
 //   kb_ru_apply r = r64 in FP32 (the next kb_upd restarts p from it)
 template <class T>
@@ -379,7 +381,7 @@ void Engine<T>::launch_chunk() {
   }
 }
 
-// Reliable updates of the paused mixed solves, then the state gather and its
+// Reliable updates of the mixed solves that have one due, then the state gather and its
 // copy to pinned host memory.
 template <class T>
 void Engine<T>::launch_tail() {

@@ -140,6 +140,18 @@ __global__ void k_h16_ones(H16* __restrict__ f, int V) {
 }
 void h16_ones(hipStream_t s, H16* f, int V) { hipLaunchKernelGGL(k_h16_ones, dim3(nblk(V)), dim3(kB), 0, s, f, V); }
 
+// The solve state copied to pinned host memory by one wave of dword stores:
+// a kernel on the stream costs less than a device-to-host copy command.
+__global__ __launch_bounds__(64) void k_state_out(const CGState* __restrict__ src, CGState* __restrict__ dst) {
+  static_assert(sizeof(CGState) % 4 == 0 && sizeof(CGState) / 4 <= 64, "one dword per lane");
+  const unsigned* a = reinterpret_cast<const unsigned*>(src);
+  unsigned* b = reinterpret_cast<unsigned*>(dst);
+  if (threadIdx.x < sizeof(CGState) / 4) b[threadIdx.x] = a[threadIdx.x];
+}
+void state_out(hipStream_t s, const CGState* d_state, CGState* h_state) {
+  hipLaunchKernelGGL(k_state_out, dim3(1), dim3(64), 0, s, d_state, h_state);
+}
+
 __global__ void k_acc_f32(double2* __restrict__ xh, float2* __restrict__ xl, size_t n) {
   const size_t i = (size_t)blockIdx.x * kB + threadIdx.x;
   if (i >= n) return;

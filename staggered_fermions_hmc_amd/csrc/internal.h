@@ -170,6 +170,8 @@ void cg_p_f32(hipStream_t s, const Geom& g, float2* p, const float2* r, CGState*
 void f64_to_f32(hipStream_t s, const double2* in, float2* out, size_t n);
 void f64_to_f16(hipStream_t s, const Geom& g, const double2* in, H16* out);
 void h16_ones(hipStream_t s, H16* f, int V);  // every colour (1 + i)
+// *h_state (pinned host memory) = *d_state, on the stream
+void state_out(hipStream_t s, const CGState* d_state, CGState* h_state);
 void acc_f32_into_f64(hipStream_t s, double2* xh, float2* xl, size_t n);  // xh += xl; xl = 0
 void xpay_f64(hipStream_t s, double2* y, const double2* x, double a, size_t n);  // y = x + a*y
 void axpy_f64(hipStream_t s, double2* y, const double2* x, double a, size_t n);  // y += a*x

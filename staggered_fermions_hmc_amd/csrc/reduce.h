@@ -153,10 +153,10 @@ struct FinPV {
 
 // Finisher of |r|^2 after x += alpha p, r -= alpha v: beta, convergence and
 // the iteration budget (Fermions.cpp:568, :594-600, :609-614).  In mixed
-// precision it also pauses the solve when a reliable update is due or the
-// iterated residual meets the target, for the host to run the FP64 update.
+// precision it also marks a reliable update due, and pauses the solve when
+// the iterated residual meets the target, for the FP64 update to decide.
 // Iteration bookkeeping once |r_{k+1}|^2 = rr is known: beta, the
-// convergence test, and in mixed precision the pause for a reliable update.
+// convergence test, and in mixed precision the reliable-update bookkeeping.
 __device__ __forceinline__ void finish_rr(CGState* st, double rr) {
   const double b = rr / st->rsq;
   st->iter += 1;
@@ -169,17 +169,20 @@ __device__ __forceinline__ void finish_rr(CGState* st, double rr) {
   }
   st->beta = b;
   if (st->mixed) {
-    // pause for an FP64 reliable update: it decides convergence and the
-    // budget on the true residual
+    // FP64 reliable update, run at the end of the chunk: due when the
+    // residual has fallen by delta since the last one (the iterations go on
+    // to the chunk's end), and the solve pauses for it when the iterated
+    // residual meets the target or the budget is spent (the update decides
+    // convergence and the budget on the true residual)
     const double r = sqrt(rr);
     int reason = 0;
     if (rr < st->target) reason = 2;
-    else if (r < st->delta * st->rmax) reason = 1;
     else if (st->iter >= st->max_iter) reason = 3;
+    else if (r < st->delta * st->rmax) reason = 1;
     if (reason) {
-      st->pause_reason = reason;
+      if (reason > st->pause_reason) st->pause_reason = reason;
       st->ru_pending = 1;
-      st->done = 1;
+      if (reason != 1) st->done = 1;
     }
     if (r > st->rmax) st->rmax = r;
     return;

@@ -83,11 +83,14 @@ int ensure_lp(sfh_ctx* c, sfh_gauge* gauge, int precision) {
 
 namespace {
 
-// Iterations per chunk: FP64 solves have no pauses; in mixed precision a
-// reliable update stops the iterations until the chunk's end, so shorter
-// chunks waste fewer early-exit iterations per update.
+// Iterations per chunk.  Mixed precision: a reliable update that falls due
+// runs at the chunk's end, so the chunk length bounds how long the sloppy
+// residual drifts past the update point: FP16 storage drifts fastest and
+// takes short chunks (32^4, m = 0.01: 227 iterations at 8, 252 at 16);
+// FP32 takes longer ones, which pay the per-chunk launches less often
+// (16^4: 16 instead of 8 is 3.06 -> 2.93 ms per solve at 129 iterations).
 constexpr int kChunk = 16;
-constexpr int kChunkMixed = 8;
+template <bool HALF> constexpr int kChunkMixed = HALF ? 8 : 16;
 
 struct Bufs64 {
   double2 *r, *p, *t, *v;
@@ -169,7 +172,7 @@ template <class F> int run_chunks(sfh_ctx* c, F issue_chunk, int per_chunk, cons
   auto issue = [&]() {
     const int slot = (int)(next & 1);
     if (int e = issue_chunk()) rc = e;
-    hipMemcpyAsync(ring[slot], c->d_state, sizeof(CGState), hipMemcpyDeviceToHost, c->stream);
+    state_out(c->stream, c->d_state, ring[slot]);
     hipEventRecord(ev[slot], c->stream);
     ++next;
   };
@@ -377,7 +380,7 @@ static int cg_mixed(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* prm, cons
   // small grids: iteration j of a chunk reads state copy j & 1 and writes
   // copy (j + 1) & 1; the chunk length is even, so the chunk ends in copy 0
   const bool defer = fused && mix_defers(g);
-  static_assert(kChunkMixed % 2 == 0, "state copies alternate per iteration");
+  static_assert(kChunkMixed<HALF> % 2 == 0, "state copies alternate per iteration");
   CGState* sts[2] = {st, defer ? c->d_state + 2 : st};
   auto iter = [&](int j) {
     mix_step_a<T>(c, gauge, fused, b, m, sts[j & 1], sts[(j + 1) & 1], part);
@@ -394,7 +397,7 @@ static int cg_mixed(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* prm, cons
     mix_apply<T>(c->stream, g, r64, b, st);
   };
   auto direct = [&]() {
-    for (int j = 0; j < kChunkMixed; ++j) iter(j);
+    for (int j = 0; j < kChunkMixed<HALF>; ++j) iter(j);
     tail();
   };
   const std::string key = graph_key(HALF ? (fused ? "mix16f" : "mix16s") : (fused ? "mix32f" : "mix32s"), g,
@@ -404,7 +407,7 @@ static int cg_mixed(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* prm, cons
                                     (get_hop_variant(0, g) * 4 + get_hop_variant(HALF ? 2 : 1, g)) * 2 +
                                         (gauge->lp_su3 ? 1 : 0));
   CGState fin{};
-  if (int e = run_chunks(c, [&]() { return launch_chunk(c, key, direct); }, kChunkMixed, s, fin)) return e;
+  if (int e = run_chunks(c, [&]() { return launch_chunk(c, key, direct); }, kChunkMixed<HALF>, s, fin)) return e;
   if (stats) {
     stats->iterations = fin.iter;
     stats->reliable_updates = fin.rupd;
