@@ -97,42 +97,107 @@ __device__ __forceinline__ Site site_of(const Geom& g, int nb) {
 
 // Sum of the four partials per workgroup of the previous kernel B, in one
 // fixed order for a 256-lane block: every workgroup that calls it gets the
-// same bits.
-// Split in two so the loads can be issued before the state is known.
-__device__ __forceinline__ void load_partials4(const double* __restrict__ partials, int nb, double (&d)[4]) {
+// same bits.  Split in two so the loads (a fixed count per lane, no loop, so
+// no wait between them) are issued before anything else the kernel loads,
+// and waited for only when summed.
+// PIT: partials per lane and value, ceil(nb / 256) rounded up to 1 or 4 (a
+// few loads per lane: the ones past nb are wasted and count against the 63
+// loads a wave can have in flight).
+constexpr int kPartIters = kDirectSum / kBlk;
+template <int PIT> struct Partials4 {
+  double v[PIT][4];
+};
+template <int PIT>
+__device__ __forceinline__ void load_partials4(const double* __restrict__ partials, int nb, Partials4<PIT>& pl) {
 #pragma unroll
-  for (int j = 0; j < 4; ++j) d[j] = 0.0;
-  for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+  for (int it = 0; it < PIT; ++it) {
+    const int k = threadIdx.x + it * kBlk;
+    const int kc = k < nb ? k : nb - 1;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) d[j] += partials[(size_t)j * nb + k];
+    for (int j = 0; j < 4; ++j) {
+      const double x = partials[(size_t)j * nb + kc];
+      pl.v[it][j] = k < nb ? x : 0.0;
+    }
   }
 }
-__device__ __forceinline__ void sum_partials4(const double* __restrict__ partials, int nb, double (&d)[4]) {
-  load_partials4(partials, nb, d);
-  block_sum_n<4>(d);
+template <int PIT> __device__ __forceinline__ void add_partials4(const Partials4<PIT>& pl, double (&d)[4]) {
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    d[j] = 0.0;
+#pragma unroll
+    for (int it = 0; it < PIT; ++it) d[j] += pl.v[it][j];
+  }
 }
 
-// DEFER (small grids): the previous B left its dots as plain per-workgroup
+// The solve state read by one load per lane (a dword each) and written to
+// LDS: one memory round trip, where a struct copy by one lane issues its
+// loads one after another.
+constexpr int kStateWords = (int)(sizeof(CGState) / 4);
+static_assert(sizeof(CGState) % 4 == 0 && kStateWords <= kBlk, "one dword per lane");
+__device__ __forceinline__ unsigned state_word(const CGState* st) {
+  const int t = (int)threadIdx.x < kStateWords ? (int)threadIdx.x : kStateWords - 1;
+  return reinterpret_cast<const unsigned*>(st)[t];
+}
+__device__ __forceinline__ void state_to_lds(CGState& sst, unsigned w) {
+  if ((int)threadIdx.x < kStateWords) reinterpret_cast<unsigned*>(&sst)[threadIdx.x] = w;
+}
+
+// PIT > 0 (small grids, DEFER): the previous B left its dots as plain per-workgroup
 // partials; every workgroup of A sums them and runs the finisher itself (no
 // grid-wide atomics on the iteration's critical path).  A reads the state
 // from st_in and writes the advanced state to st_out, a different copy, so
 // no workgroup reads what another writes; successive iterations alternate
 // the two copies.  Otherwise st_in == st_out and B has run the finisher.
-template <class T, bool DEFER, bool R12>
+// cur: the buffer this launch writes r_k and p_k into (the launch's position
+// in the chunk, so known before the state is read); workgroup 0 records it
+// in the state for the reliable update.
+// Loads that do not depend on the state (the own site's r, v, p and x) are
+// issued first and decoded after the state is known, so they are in flight
+// while the state and the previous B's dots are read and finished.  (Issuing
+// the eight links there as well was measured slower in the solve at 16^4:
+// 2.91 against 2.89 ms.)
+template <class T, int PIT, bool R12>
 __global__ __launch_bounds__(kBlk) void k_mix_a(Geom g, const typename Prec<T>::L* __restrict__ U, MixBufs<T> b,
                                                 float mass, const CGState* st_in, CGState* st_out,
-                                                const double* __restrict__ partials, int nb) {
-  int iter;
+                                                const double* __restrict__ partials, int nb, int cur) {
+  constexpr bool DEFER = PIT > 0;
+  // small grids: the previous B's partials and the state, loaded first
+  Partials4<DEFER ? PIT : 1> pl;
+  unsigned sw = 0;
+  if constexpr (DEFER) {
+    load_partials4(partials, nb, pl);
+    sw = state_word(st_in);
+  }
+  const Site s = site_of(g, nb);
+  const bool valid = s.cb < g.Vh;
+  const int prv = cur ^ 1;
+  const typename Prec<T>::S* __restrict__ rp = b.r[prv];
+  const typename Prec<T>::S* __restrict__ pp = b.p[prv];
+  const float* __restrict__ rpn = b.rn[prv];
+  const float* __restrict__ ppn = b.pn[prv];
+  const int V = g.V;
+  // lanes past the lattice load site (p, 0) and store nothing
+  const int cb = valid ? s.cb : 0;
+  const int i = valid ? s.i : s.p * g.Vh;
+  int n[4];
+  cb_coords(g, cb, s.p, n);
+  SpinorWords<T> rw, vw, pw;
+  rw.load(rp, V, i);
+  vw.load(b.v, V, i);
+  pw.load(pp, V, i);
+  float2 xx[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) xx[c] = b.x[(size_t)c * V + i];
   float al, be;
   if constexpr (DEFER) {
     __shared__ CGState sst;
-    // the partials are read together with the state (one round trip); they
+    // the partials were read together with the state (one round trip); they
     // are used only if the state says a finisher is pending
-    double d[4];
-    load_partials4(partials, nb, d);
-    if (threadIdx.x == 0) sst = *st_in;
+    state_to_lds(sst, sw);
     __syncthreads();
     if (sst.fin_pending) {
+      double d[4];
+      add_partials4(pl, d);
       block_sum_n<4>(d);
       if (threadIdx.x == 0) {
         finish_fused(&sst, d);
@@ -144,45 +209,34 @@ __global__ __launch_bounds__(kBlk) void k_mix_a(Geom g, const typename Prec<T>::
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       CGState o = sst;
       o.fin_pending = stop ? 0 : 1;  // the B that follows leaves partials
+      if (!stop) o.pbuf = cur;
       *st_out = o;
     }
     if (stop) return;  // uniform over the block
-    iter = sst.iter;
     al = (float)sst.alpha;
     be = (float)sst.beta;
   } else {
     if (st_in->done) return;
-    iter = st_in->iter;
     al = (float)st_in->alpha;
     be = (float)st_in->beta;
+    if (blockIdx.x == 0 && threadIdx.x == 0) st_out->pbuf = cur;
   }
-  const Site s = site_of(g, nb);
-  if (s.cb >= g.Vh) return;
-  const int cur = iter & 1, prv = cur ^ 1;
-  const typename Prec<T>::S* __restrict__ rp = b.r[prv];
-  const typename Prec<T>::S* __restrict__ pp = b.p[prv];
-  const float* __restrict__ rpn = b.rn[prv];
-  const float* __restrict__ ppn = b.pn[prv];
-  const int V = g.V;
-  int n[4];
-  cb_coords(g, s.cb, s.p, n);
+  if (!valid) return;
   Q16 qr, qp;
   // own site: advance, store r_k and p_k, accumulate x
   vec3<float> ro, vo, po;
-  load_spinor<T>(rp, rpn, V, s.i, ro);
-  load_spinor<T>(b.v, b.vn, V, s.i, vo);
-  load_spinor<T>(pp, ppn, V, s.i, po);
+  rw.decode(ro);
+  vw.decode(vo);
+  pw.decode(po);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    const size_t j = (size_t)c * V + s.i;
-    float2 xx = b.x[j];
-    xx.x = __fmaf_rn(al, po.c[c].re, xx.x);
-    xx.y = __fmaf_rn(al, po.c[c].im, xx.y);
-    b.x[j] = xx;
+    xx[c].x = __fmaf_rn(al, po.c[c].re, xx[c].x);
+    xx[c].y = __fmaf_rn(al, po.c[c].im, xx[c].y);
+    b.x[(size_t)c * V + i] = xx[c];
   }
   advance<T>(ro, vo, po, al, be, qr, qp);
-  store_q<T>(b.r[cur], b.rn[cur], V, s.i, ro, qr);
-  store_q<T>(b.p[cur], b.pn[cur], V, s.i, po, qp);
+  store_q<T>(b.r[cur], b.rn[cur], V, i, ro, qr);
+  store_q<T>(b.p[cur], b.pn[cur], V, i, po, qp);
   // t = m p_k - (1/2) sum_mu eta [bup U p_k(n+mu) - bdn U^dag p_k(n-mu)]
   vec3<float> acc;
 #pragma unroll
@@ -214,17 +268,16 @@ __global__ __launch_bounds__(kBlk) void k_mix_a(Geom g, const typename Prec<T>::
     res.c[c].re = mass * po.c[c].re - 0.5f * acc.c[c].re;
     res.c[c].im = mass * po.c[c].im - 0.5f * acc.c[c].im;
   }
-  store_spinor<T>(b.t, b.tn, V, s.i, res);
+  store_spinor<T>(b.t, b.tn, V, i, res);
 }
 
 template <class T, bool DEFER, bool R12>
 __global__ __launch_bounds__(kBlk) void k_mix_b(Geom g, const typename Prec<T>::L* __restrict__ U, MixBufs<T> b,
-                                                float mass, CGState* st, double* partials, int nb) {
+                                                float mass, CGState* st, double* partials, int nb, int cur) {
   if (st->done) return;
   const Site s = site_of(g, nb);
   double d[4] = {0.0, 0.0, 0.0, 0.0};
   if (s.cb < g.Vh) {
-    const int cur = st->iter & 1;
     const int V = g.V;
     int n[4];
     cb_coords(g, s.cb, s.p, n);
@@ -282,11 +335,14 @@ __global__ __launch_bounds__(kBlk) void k_mix_b(Geom g, const typename Prec<T>::
 // Finisher of the last B of a chunk (DEFER): one workgroup, in place.
 __global__ __launch_bounds__(kBlk) void k_mix_fin(CGState* st, const double* __restrict__ partials, int nb) {
   __shared__ CGState sst;
-  if (threadIdx.x == 0) sst = *st;
+  Partials4<kPartIters> pl;
+  load_partials4(partials, nb, pl);
+  state_to_lds(sst, state_word(st));
   __syncthreads();
   if (!sst.fin_pending) return;  // uniform
   double d[4];
-  sum_partials4(partials, nb, d);
+  add_partials4(pl, d);
+  block_sum_n<4>(d);
   if (threadIdx.x == 0) {
     finish_fused(&sst, d);
     sst.fin_pending = 0;
@@ -328,7 +384,7 @@ __global__ __launch_bounds__(kBlk) void k_mix_acc(int V, double2* __restrict__ x
   if (!st->ru_pending) return;
   const int i = blockIdx.x * kBlk + threadIdx.x;
   if (i >= V) return;
-  const int k = (st->iter & 1) ^ 1;  // buffers of p_k, the direction of the last iteration
+  const int k = st->pbuf;  // buffers of p_k, the direction of the last iteration
   const double al = st->alpha;
   vec3<float> pk;
   load_spinor<T>(b.p[k], b.pn[k], V, i, pk);
@@ -350,14 +406,19 @@ __global__ __launch_bounds__(kBlk) void k_mix_apply(int V, const double2* __rest
   if (!st->ru_apply) return;
   const int i = blockIdx.x * kBlk + threadIdx.x;
   if (i >= V) return;
-  const int k = (st->iter & 1) ^ 1;  // read as r_{k-1} by the next kernel A
+  // the next chunk's first kernel A reads r_{k-1} and p_{k-1} from buffer 1;
+  // p_k is in buffer pbuf (0 only when the solve paused at an odd position)
   vec3<float> r;
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const double2 t = r64[(size_t)c * V + i];
     r.c[c] = {(float)t.x, (float)t.y};
   }
-  store_spinor<T>(b.r[k], b.rn[k], V, i, r);
+  store_spinor<T>(b.r[1], b.rn[1], V, i, r);
+  if (st->pbuf == 0 && b.p[0] != b.p[1]) {
+#pragma unroll
+    for (int c = 0; c < kSpinorElems<T>; ++c) b.p[1][(size_t)c * V + i] = b.p[0][(size_t)c * V + i];
+  }
 }
 
 inline int mix_blocks(const Geom& g) { return (g.Vh + kBlk / 2 - 1) / (kBlk / 2); }
@@ -368,25 +429,28 @@ bool mix_defers(const Geom& g) { return mix_blocks(g) <= kDirectSum; }
 
 template <class T>
 void mix_a(hipStream_t s, const Geom& g, const typename Prec<T>::L* U, const MixBufs<T>& b, float mass,
-           const CGState* st_in, CGState* st_out, const double* partials, bool r12) {
+           const CGState* st_in, CGState* st_out, const double* partials, bool r12, int cur) {
   const int nb = mix_blocks(g);
   auto go = [&](auto kernel) {
-    hipLaunchKernelGGL(kernel, dim3(nb), dim3(kBlk), 0, s, g, U, b, mass, st_in, st_out, partials, nb);
+    hipLaunchKernelGGL(kernel, dim3(nb), dim3(kBlk), 0, s, g, U, b, mass, st_in, st_out, partials, nb, cur);
   };
-  if (st_in != st_out) {
-    if (r12) go(k_mix_a<T, true, true>);
-    else go(k_mix_a<T, true, false>);
+  if (st_in != st_out && nb <= kBlk) {
+    if (r12) go(k_mix_a<T, 1, true>);
+    else go(k_mix_a<T, 1, false>);
+  } else if (st_in != st_out) {
+    if (r12) go(k_mix_a<T, kPartIters, true>);
+    else go(k_mix_a<T, kPartIters, false>);
   } else {
-    if (r12) go(k_mix_a<T, false, true>);
-    else go(k_mix_a<T, false, false>);
+    if (r12) go(k_mix_a<T, 0, true>);
+    else go(k_mix_a<T, 0, false>);
   }
 }
 template <class T>
 void mix_b(hipStream_t s, const Geom& g, const typename Prec<T>::L* U, const MixBufs<T>& b, float mass, CGState* st,
-           double* partials, bool defer, bool r12) {
+           double* partials, bool defer, bool r12, int cur) {
   const int nb = mix_blocks(g);
   auto go = [&](auto kernel) {
-    hipLaunchKernelGGL(kernel, dim3(nb), dim3(kBlk), 0, s, g, U, b, mass, st, partials, nb);
+    hipLaunchKernelGGL(kernel, dim3(nb), dim3(kBlk), 0, s, g, U, b, mass, st, partials, nb, cur);
   };
   if (defer) {
     if (r12) go(k_mix_b<T, true, true>);
@@ -410,13 +474,13 @@ template <class T> void mix_apply(hipStream_t s, const Geom& g, const double2* r
 }
 
 template void mix_a<float>(hipStream_t, const Geom&, const float2*, const MixBufs<float>&, float, const CGState*,
-                           CGState*, const double*, bool);
+                           CGState*, const double*, bool, int);
 template void mix_a<Half>(hipStream_t, const Geom&, const __half2*, const MixBufs<Half>&, float, const CGState*,
-                          CGState*, const double*, bool);
+                          CGState*, const double*, bool, int);
 template void mix_b<float>(hipStream_t, const Geom&, const float2*, const MixBufs<float>&, float, CGState*, double*,
-                           bool, bool);
+                           bool, bool, int);
 template void mix_b<Half>(hipStream_t, const Geom&, const __half2*, const MixBufs<Half>&, float, CGState*, double*,
-                          bool, bool);
+                          bool, bool, int);
 template void mix_upd<float>(hipStream_t, const Geom&, const MixBufs<float>&, CGState*);
 template void mix_upd<Half>(hipStream_t, const Geom&, const MixBufs<Half>&, CGState*);
 template void mix_acc<float>(hipStream_t, const Geom&, double2*, const MixBufs<float>&, CGState*);

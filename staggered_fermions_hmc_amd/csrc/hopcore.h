@@ -121,6 +121,36 @@ __device__ __forceinline__ void store_spinor(typename Prec<T>::S* __restrict__ f
   }
 }
 
+// Storage words of one site's spinor, loaded now and decoded later: a kernel
+// can issue the loads before a wait on something else (a barrier, the solve
+// state) and convert after it, without the conversion pulling the wait for
+// the data forward.
+template <class T> struct SpinorWords {
+  cpx<typename Prec<T>::R> c[3];
+  __device__ __forceinline__ void load(const typename Prec<T>::S* __restrict__ f, int V, int i) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) c[k] = ldc(f + (size_t)k * V + i);
+  }
+  __device__ __forceinline__ void decode(vec3<typename Prec<T>::R>& v) const {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v.c[k] = c[k];
+  }
+};
+template <> struct SpinorWords<Half> {
+  H16 s;
+  __device__ __forceinline__ void load(const H16* __restrict__ f, int V, int i) {
+    (void)V;
+    s = f[i];
+  }
+  __device__ __forceinline__ void decode(vec3<float>& v) const {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const float2 t = __half22float2(s.c[k]);
+      v.c[k] = {s.n * t.x, s.n * t.y};
+    }
+  }
+};
+
 // acc += s * U v
 template <class R>
 __device__ __forceinline__ void acc_mv(vec3<R>& acc, const mat3<R>& U, const vec3<R>& v, R s) {

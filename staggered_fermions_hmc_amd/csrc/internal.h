@@ -31,6 +31,7 @@ struct CGState {
   int rupd;          // reliable updates done
   int converged;     // mixed precision: the FP64 true residual met the target
   int fin_pending;   // mixed iteration, small grids: the last B's dots still await their finisher
+  int pbuf;          // mixed iteration, fused form: buffer of the latest r_k and p_k
   unsigned* ctr;     // grid-reduction arrival counters (kCtrKinds x kCtrPerKind, device)
 };
 

@@ -7,9 +7,10 @@
 namespace sfh {
 
 // Sloppy vectors of one mixed-precision solve.  In the fused form r and p are
-// double-buffered: iteration k writes r_k and p_k into buffer k & 1 while
-// other workgroups still read r_{k-1} and p_{k-1} from buffer (k - 1) & 1.
-// In the split form both slots point to one array.  Norms are the per-site
+// double-buffered: the iteration at position j of a chunk (chunks have an
+// even length) writes r_k and p_k into buffer j & 1 while other workgroups
+// still read r_{k-1} and p_{k-1} from the other buffer; a chunk starts from
+// buffer 1.  In the split form both slots point to one array.  Norms are the per-site
 // max-norms of FP16 storage (unused in FP32).
 template <class T> struct MixBufs {
   typename Prec<T>::S *r[2], *p[2], *v, *t;
@@ -22,10 +23,11 @@ template <class T> struct MixBufs {
 // recomputed at the neighbours, then t = D^dag p_k.  With st_in != st_out
 // (small grids, see mix_defers) it first finishes the previous B's dots from
 // `partials` and writes the advanced state to st_out.
-// r12: SU(3) links, read as two rows with the third rebuilt.
+// r12: SU(3) links, read as two rows with the third rebuilt.  cur: the
+// buffer written (the iteration's position in the chunk, mod 2).
 template <class T>
 void mix_a(hipStream_t s, const Geom& g, const typename Prec<T>::L* U, const MixBufs<T>& b, float mass,
-           const CGState* st_in, CGState* st_out, const double* partials, bool r12);
+           const CGState* st_in, CGState* st_out, const double* partials, bool r12, int cur);
 // Whether B leaves its dots as per-workgroup partials for the next A (and
 // mix_fin) to finish, instead of a grid reduction with its own finisher:
 // true when the grid is small enough for every workgroup to read them all.
@@ -38,12 +40,15 @@ void mix_fin(hipStream_t s, const Geom& g, CGState* st, const double* partials);
 template <class T> void mix_upd(hipStream_t s, const Geom& g, const MixBufs<T>& b, CGState* st);
 // Fused form chosen below this many sites; the split form at and above.
 constexpr int kMixFusedMaxV = 1 << 18;
-// Kernel B of iteration k: v = D t and the four dots of FinFused.
+// Kernel B of iteration k: v = D t and the four dots of FinFused, with the
+// r_k and p_k of buffer cur (the same as the A before it).
 template <class T>
 void mix_b(hipStream_t s, const Geom& g, const typename Prec<T>::L* U, const MixBufs<T>& b, float mass, CGState* st,
-           double* partials, bool defer, bool r12);
+           double* partials, bool defer, bool r12, int cur);
 // Reliable update, first and last steps (gated on the solve state):
-// x64 += x + alpha p_k and x = 0; then r_{k} = r64 in sloppy storage.
+// x64 += x + alpha p_k (p_k from buffer st->pbuf) and x = 0; then r = r64 in
+// sloppy storage, into buffer 1 with p_k moved there, where the next chunk
+// starts.
 template <class T> void mix_acc(hipStream_t s, const Geom& g, double2* x64, const MixBufs<T>& b, CGState* st);
 template <class T> void mix_apply(hipStream_t s, const Geom& g, const double2* r64, const MixBufs<T>& b, CGState* st);
 

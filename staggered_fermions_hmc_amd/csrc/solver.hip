@@ -245,14 +245,14 @@ template <class T> int mix_bufs(sfh_ctx* c, const Geom& g, bool fused, MixBufs<T
 // only for the fused form on small grids, see mix_defers).
 template <class T>
 void mix_step_a(sfh_ctx* c, const sfh_gauge* gauge, bool fused, const MixBufs<T>& b, float m, const CGState* st_in,
-                CGState* st_out, const double* partials) {
+                CGState* st_out, const double* partials, int cur) {
   constexpr bool half = std::is_same<T, Half>::value;
   const Geom& g = gauge->g;
   if (fused) {
     KTimer kt(c, half ? "mix_a_f16" : "mix_a_f32");
     mix_a<T>(c->stream, g, reinterpret_cast<const typename Prec<T>::L*>(half ? (const void*)gauge->lp.u16
                                                                              : (const void*)gauge->lp.u32),
-             b, m, st_in, st_out, partials, gauge->lp_su3);
+             b, m, st_in, st_out, partials, gauge->lp_su3, cur);
     return;
   }
   CGState* st = st_out;
@@ -370,6 +370,7 @@ static int cg_mixed(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* prm, cons
   s.delta = prm->reliable_delta > 0 ? prm->reliable_delta : 0.1;
   s.rmax = std::sqrt(phi_sq);
   s.rsq_true = phi_sq;
+  s.pbuf = 1;
   if (int e = upload_state(c, s)) return e;
   CGState* st = c->d_state;
   double* part = c->d_partials;
@@ -383,9 +384,9 @@ static int cg_mixed(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* prm, cons
   static_assert(kChunkMixed<HALF> % 2 == 0, "state copies alternate per iteration");
   CGState* sts[2] = {st, defer ? c->d_state + 2 : st};
   auto iter = [&](int j) {
-    mix_step_a<T>(c, gauge, fused, b, m, sts[j & 1], sts[(j + 1) & 1], part);
+    mix_step_a<T>(c, gauge, fused, b, m, sts[j & 1], sts[(j + 1) & 1], part, j & 1);
     KTimer kt(c, HALF ? "mix_b_f16" : "mix_b_f32");
-    mix_b<T>(c->stream, g, U, b, m, sts[(j + 1) & 1], part, defer, gauge->lp_su3);
+    mix_b<T>(c->stream, g, U, b, m, sts[(j + 1) & 1], part, defer, gauge->lp_su3, j & 1);
   };
   auto tail = [&]() {
     KTimer kt(c, "reliable_update");
@@ -509,8 +510,8 @@ static int time_mixed_t(sfh_ctx* c, sfh_gauge* gauge, int n, double* us_a, doubl
   // sees the same state; B changes alpha and beta or only leaves partials
   const bool defer = fused && mix_defers(g);
   CGState* st_out = defer ? &c->d_state[3] : st;
-  if (int e = timed([&]() { mix_step_a<T>(c, gauge, fused, b, m, st, st_out, part); }, us_a)) return e;
-  return timed([&]() { mix_b<T>(c->stream, g, U, b, m, st, part, defer, gauge->lp_su3); }, us_b);
+  if (int e = timed([&]() { mix_step_a<T>(c, gauge, fused, b, m, st, st_out, part, 0); }, us_a)) return e;
+  return timed([&]() { mix_b<T>(c->stream, g, U, b, m, st, part, defer, gauge->lp_su3, 0); }, us_b);
 }
 
 void set_mix_form(int precision, int form) {
