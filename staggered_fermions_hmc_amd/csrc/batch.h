@@ -12,6 +12,11 @@ namespace sfh {
 
 constexpr int kMaxRhs = 8;  // right-hand sides in flight per system
 
+// Iterations per chunk of the mixed engine, for batched solves and the solve
+// service alike (a result is the same alone or in any mix only under one
+// chunk length): a reliable update that falls due runs at the chunk's end.
+constexpr int kBatchChunk = 16;
+
 // Launch shape of the batched hop, chosen by the autotuner.  hop_threads
 // (128, 256 or 512 lanes, one site each, half per parity) applies to the
 // first hop of an iteration, t = D^dag p, which has no reduction; rhs_tile

@@ -788,7 +788,7 @@ int tune_batch(sfh_ctx* c, int nsys, const BatchSys* sys, const sfh_cg_params* p
     return SFH_OK;
   };
   if (p->precision == SFH_PREC_F32) {
-    Engine<float> eng(c, p->tol, p->max_iter, p->reliable_delta, 8);
+    Engine<float> eng(c, p->tol, p->max_iter, p->reliable_delta, kBatchChunk);
     return tune(eng);
   }
   if (p->precision == SFH_PREC_F64) {
@@ -837,7 +837,7 @@ int cg_batch(sfh_ctx* c, int nsys, const BatchSys* sys, const sfh_cg_params* p, 
     return err;
   };
   if (p->precision == SFH_PREC_F32) {
-    Engine<float> eng(c, p->tol, p->max_iter, p->reliable_delta, 8);
+    Engine<float> eng(c, p->tol, p->max_iter, p->reliable_delta, kBatchChunk);
     return run(eng);
   }
   if (p->precision == SFH_PREC_F64) {

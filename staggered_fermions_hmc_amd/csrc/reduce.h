@@ -258,7 +258,12 @@ struct FinRU {
       st->done = 1;
       return;
     }
-    st->beta = rr / st->rsq_prev;
+    // the true residual replaces the iterated one; the direction is kept
+    // (beta = |r_true|^2 / |r_k|^2) unless the two residuals have drifted
+    // apart (|r_true| > 10 |r_iterated|, e.g. when no update fell due before
+    // the iterated residual met the target): the old direction then no
+    // longer fits the residual and the iteration starts again from p = r_true
+    st->beta = rr > 100.0 * st->rsq ? 0.0 : rr / st->rsq_prev;
     st->alpha = 0.0;  // the two-kernel iteration then restarts from r_true
     st->rsq = rr;
     st->rmax = sqrt(rr);

@@ -416,6 +416,62 @@ def test_hmc_trajectory_matches_oracle(ctx):
     assert np.abs(g.download() - Ur).max() < 1e-10
 
 
+@pytest.mark.parametrize("prec", ["f32", "f16"])
+def test_mixed_restart_after_convergence_pause(ctx, prec):
+    """Near the end of a solve the iterated residual can meet the target
+    while the true one has not: the solve pauses, and the FP64 update
+    restarts it.  In the fused form the pause can fall at either buffer
+    parity (the restart moves p_k into the buffer the next chunk starts
+    from); the split form keeps one buffer.  Both run the same recurrence, so
+    over several right-hand sides the iteration counts agree and every solve
+    reaches the true residual."""
+    dims, m = (8, 8, 8, 12), 0.03
+    U = dev.random_links(dims, 0.3, seed=71)
+    g = dev.Gauge(ctx, dims, U)
+    try:
+        for seed in range(72, 80):
+            phi = dev.random_spinor(dims, seed=seed)
+            out = {}
+            for form in (0, 1):
+                dev.set_cg_form(prec, form)
+                out[form] = g.cg(phi, m, tol=1e-10, precision=prec)
+            (x0, s0), (x1, s1) = out[0], out[1]
+            assert abs(s0["iterations"] - s1["iterations"]) <= 2, (seed, s0, s1)
+            assert abs(s0["reliable_updates"] - s1["reliable_updates"]) <= 1, (seed, s0, s1)
+            for x in (x0, x1):
+                r = phi - oc.apply_ddag(dims, m, U, x)
+                assert np.sqrt(r @ r / (phi @ phi)) < 1e-10
+    finally:
+        dev.set_cg_form(prec, -1)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f16"])
+def test_mixed_without_due_updates_restarts_fresh(ctx, prec):
+    """A reliable-update factor too small to ever fall due leaves the sloppy
+    residual to drift until the iterated residual meets the target; the FP64
+    update then finds the true residual far above it and restarts the
+    iteration from p = r_true (keeping the old direction, scaled by
+    |r_true|^2 / |r_k|^2, would stall).  Both forms converge alike."""
+    dims, m = (8, 8, 8, 12), 0.03
+    U = dev.random_links(dims, 0.3, seed=81)
+    g = dev.Gauge(ctx, dims, U)
+    phi = dev.random_spinor(dims, seed=82)
+    _, s64 = g.cg(phi, m, tol=1e-10, precision="f64")
+    try:
+        out = {}
+        for form in (0, 1):
+            dev.set_cg_form(prec, form)
+            out[form] = g.cg(phi, m, tol=1e-10, precision=prec, reliable_delta=1e-30, max_iter=20000)
+    finally:
+        dev.set_cg_form(prec, -1)
+    (x0, s0), (x1, s1) = out[0], out[1]
+    assert abs(s0["iterations"] - s1["iterations"]) <= 2, (s0, s1)
+    assert s0["reliable_updates"] >= 2 and s0["iterations"] < 6 * s64["iterations"], (s0, s64)
+    for x in (x0, x1):
+        r = phi - oc.apply_ddag(dims, m, U, x)
+        assert np.sqrt(r @ r / (phi @ phi)) < 1e-10
+
+
 @pytest.mark.parametrize("precision", ["f32", "f16"])
 def test_mixed_cg_non_su3_links(ctx, precision):
     """The low-precision hops read two link rows and rebuild the third only
