@@ -145,6 +145,7 @@ double2* dbuf(sfh_ctx* c, const char* key, size_t n) {
 void invalidate_lp(sfh_gauge* g) {
   g->lp32_valid = false;
   g->lp16_valid = false;
+  g->su3_known = false;
 }
 
 // One force evaluation and momentum kick P -= coef F on the device; X (may be
@@ -162,8 +163,9 @@ int md_kick(sfh_ctx* c, sfh_gauge* gauge, const sfh_md_params* p, const double2*
     if (int e = cg_solve(c, gauge, &cp, phi, X, nullptr)) return e;
     Y = dbuf(c, "md_Y", 3 * (size_t)g.V);
     if (!Y) return SFH_E_NOMEM;
+    if (int e = ensure_su3(c, gauge)) return e;
     KTimer kt(c, "hop_f64");
-    hop_f64(c->stream, g, gauge->U, X, Y, p->mass, -1.0, nullptr, nullptr, nullptr);
+    hop_f64(c->stream, g, gauge->U, X, Y, p->mass, -1.0, nullptr, nullptr, nullptr, 0, gauge->f64_su3);
   }
   return force_field(c, g, gauge->U, X, Y, p->beta, p->c11, p->c12, p->which, nullptr, P, coef);
 }
@@ -319,7 +321,8 @@ int sfh_tune_hop(sfh_ctx* c, const sfh_gauge* g, int precision, int n, int* vari
     set_hop_variant(precision, v);
     auto launch = [&]() {
       if (precision == 0)
-        hop_f64(c->stream, gg, g->U, (const double2*)a, (double2*)b, 0.1, 1.0, nullptr, nullptr, nullptr);
+        hop_f64(c->stream, gg, g->U, (const double2*)a, (double2*)b, 0.1, 1.0, nullptr, nullptr, nullptr, 0,
+                g->f64_su3);
       else if (precision == 1)
         hop_f32(c->stream, gg, g->lp.u32, (const float2*)a, (float2*)b, 0.1f, 1.0f, nullptr, nullptr, nullptr,
                 g->lp_su3);
@@ -376,7 +379,7 @@ int sfh_time_hop(sfh_ctx* c, const sfh_gauge* g, int precision, int with_dot, in
   auto launch = [&]() {
     if (precision == 0)
       hop_f64(c->stream, gg, g->U, (const double2*)a, (double2*)b, 0.1, 1.0, with_dot ? (const double2*)a : nullptr,
-              with_dot ? st : nullptr, part);
+              with_dot ? st : nullptr, part, 0, g->f64_su3);
     else if (precision == 1)
       hop_f32(c->stream, gg, g->lp.u32, (const float2*)a, (float2*)b, 0.1f, 1.0f,
               with_dot ? (const float2*)a : nullptr, with_dot ? st : nullptr, part, g->lp_su3);
@@ -580,9 +583,10 @@ int sfh_apply_d(sfh_ctx* c, const sfh_gauge* g, double mass, int dagger, const d
   double2 *a = dbuf(c, "io_a", n), *b = dbuf(c, "io_b", n);
   if (!a || !b) return SFH_E_NOMEM;
   if (int e = upload_spinor(c, g->g, in, a)) return e;
+  if (int e = ensure_su3(c, const_cast<sfh_gauge*>(g))) return e;
   {
     KTimer kt(c, "hop_f64");
-    hop_f64(c->stream, g->g, g->U, a, b, mass, dagger ? -1.0 : 1.0, nullptr, nullptr, nullptr);
+    hop_f64(c->stream, g->g, g->U, a, b, mass, dagger ? -1.0 : 1.0, nullptr, nullptr, nullptr, 0, g->f64_su3);
   }
   return download_spinor(c, g->g, b, out);
 }
@@ -603,9 +607,11 @@ int sfh_apply_d_field(sfh_ctx* c, const sfh_gauge* g, double mass, int dagger, c
   if (!c || !g || !in || !out || in == out) return SFH_E_INVALID;
   Guard gd(c);
   if (in->g.V != g->g.V || out->g.V != g->g.V) return set_err(c, SFH_E_INVALID, "field / gauge size mismatch");
+  if (int e = ensure_su3(c, const_cast<sfh_gauge*>(g))) return e;
   {
     KTimer kt(c, "hop_f64");
-    hop_f64(c->stream, g->g, g->U, in->d, out->d, mass, dagger ? -1.0 : 1.0, nullptr, nullptr, nullptr);
+    hop_f64(c->stream, g->g, g->U, in->d, out->d, mass, dagger ? -1.0 : 1.0, nullptr, nullptr, nullptr, 0,
+            g->f64_su3);
   }
   return hip_check(c, hipStreamSynchronize(c->stream), "apply_d_field");
 }
@@ -829,7 +835,8 @@ int sfh_hmc_trajectory(sfh_ctx* c, sfh_gauge* g, const sfh_md_params* p, const d
   // Fermions.cpp:812-828
   const double sg_i = gauge_action(c, gg, g->U, p->beta, p->c11, p->c12);
   if (int e = upload_spinor(c, gg, chi, dchi)) return e;
-  hop_f64(c->stream, gg, g->U, dchi, dphi, p->mass, 1.0, nullptr, nullptr, nullptr);
+  if (int e = ensure_su3(c, g)) return e;
+  hop_f64(c->stream, gg, g->U, dchi, dphi, p->mass, 1.0, nullptr, nullptr, nullptr, 0, g->f64_su3);
   if (int e = upload_links(c, gg, P0, dP)) return e;
   const double tp_i = tr_p2(c, gg, dP);
   sfh_cg_params cp{};

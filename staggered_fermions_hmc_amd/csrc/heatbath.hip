@@ -268,6 +268,12 @@ __global__ __launch_bounds__(kHbThreads) void k_heatbath(Geom g, double2* __rest
       // the link keeps its value and the call fails
       atomicMax(status, (int)SFH_E_SAMPLER);
     } else {
+      if constexpr (IX::kTwoRows) {
+        orthonormalize_rows01(u);
+        u.e[6] = conj(u.e[1] * u.e[5] - u.e[2] * u.e[4]);
+        u.e[7] = conj(u.e[2] * u.e[3] - u.e[0] * u.e[5]);
+        u.e[8] = conj(u.e[0] * u.e[4] - u.e[1] * u.e[3]);
+      }
       stLx<IX>(U, g, i, mu, u);
       M A1;
 #pragma unroll
@@ -382,7 +388,7 @@ int heatbath(sfh_ctx* c, const Geom& g, double2* U, double beta, double c11, dou
   if (int e = hip_check(c, hipStreamSynchronize(c->stream), "SU(3) check")) return e;
   float d;
   std::memcpy(&d, hd, sizeof d);
-  return d < kSu3Tol ? heatbath_sweeps<HbIndex<true>>(c, g, U, H, part, sums, beta, c11, c12, seed, sweep0, nsweeps,
+  return d < kSu3TolF64 ? heatbath_sweeps<HbIndex<true>>(c, g, U, H, part, sums, beta, c11, c12, seed, sweep0, nsweeps,
                                                       plaq, act)
                      : heatbath_sweeps<HbIndex<false>>(c, g, U, H, part, sums, beta, c11, c12, seed, sweep0, nsweeps,
                                                        plaq, act);

@@ -231,7 +231,6 @@ static void hop_launch(hipStream_t s, int variant, bool r12, const Geom& g, cons
       hipLaunchKernelGGL(kernel, dim3(nb), dim3(kHopBlock), 0, s, g, U, in, in_norm, out, out_norm, a, hb, dw, dw_norm,
                          st, partials, nb, gate);
     };
-    constexpr bool kLow = !std::is_same<T, double>::value;
     auto pick = [&](auto r) {
       constexpr bool R = decltype(r)::value;
       if (variant == 2) {
@@ -245,7 +244,7 @@ static void hop_launch(hipStream_t s, int variant, bool r12, const Geom& g, cons
         else go(k_hop<T, false, 4, R>);
       }
     };
-    if (kLow && r12) pick(std::integral_constant<bool, kLow>{});
+    if (r12) pick(std::true_type{});
     else pick(std::false_type{});
   }
 }
@@ -266,8 +265,8 @@ void set_hop_variant(int prec, int variant) {
 int get_hop_variant(int prec, const Geom& g) { return pick_variant(prec, g); }
 
 void hop_f64(hipStream_t s, const Geom& g, const double2* U, const double2* in, double2* out, double a, double b,
-             const double2* dot_with, CGState* st, double* partials, int gate) {
-  hop_launch<double>(s, pick_variant(0, g), false, g, U, in, nullptr, out, nullptr, a, 0.5 * b, dot_with, nullptr, st,
+             const double2* dot_with, CGState* st, double* partials, int gate, bool r12) {
+  hop_launch<double>(s, pick_variant(0, g), r12, g, U, in, nullptr, out, nullptr, a, 0.5 * b, dot_with, nullptr, st,
                      partials, gate);
 }
 

@@ -65,10 +65,13 @@ struct sfh_gauge {
   double2* U = nullptr;     // 4*9*V complex, device layout
   sfh::LinksLP lp;
   bool lp32_valid = false, lp16_valid = false;
-  // the FP64 links are SU(3) (third row = conj(row0 x row1) to 1e-10), so the
-  // low-precision kernels may read two rows and rebuild the third; set with
-  // the low-precision copies
+  // whether the FP64 links are SU(3) (third row = conj(row0 x row1)), so
+  // kernels may read two rows and rebuild the third: to kSu3Tol for the
+  // low-precision copies (lp_su3), to kSu3TolF64 for the FP64 kernels
+  // (f64_su3); measured once after the links change (su3_known)
+  bool su3_known = false;
   bool lp_su3 = false;
+  bool f64_su3 = false;
 };
 
 struct sfh_ctx {
@@ -137,18 +140,21 @@ void spinor_soa_to_aos(hipStream_t s, const Geom& g, const double2* soa, double*
 void links_to_f32(hipStream_t s, const Geom& g, const double2* U, float2* u32);
 void links_to_f16(hipStream_t s, const Geom& g, const double2* U, __half2* u16);
 // max over links of |row2 - conj(row0 x row1)| as the bits of a float, into
-// *dev (which must start at 0); links below kSu3Tol may be read as two rows
+// *dev (which must start at 0; non-finite entries give +inf).  Below kSu3Tol
+// the FP32 and FP16 kernels read two rows and rebuild the third; below
+// kSu3TolF64 (the rounding of a few FP64 operations) the FP64 kernels do.
 void su3_deviation(hipStream_t s, const Geom& g, const double2* U, unsigned* dev);
 constexpr float kSu3Tol = 1e-10f;
+constexpr float kSu3TolF64 = 1e-14f;
 
 // ------------------------------------------------------------------ hop (hop.hip)
 // out = a*in + (b/2) * sum_mu eta [bup U psi(n+mu) - bdn U^dag psi(n-mu)]
 // If dot_with != nullptr, also reduces Re<dot_with, out> into st (pv, alpha).
 // gate 0: skip when st->done (a CG step); gate 1: run only while
 // st->ru_pending (a reliable-update step).
-void hop_f64(hipStream_t s, const Geom& g, const double2* U, const double2* in, double2* out, double a, double b,
-             const double2* dot_with, CGState* st, double* partials, int gate = 0);
 // r12: the links are SU(3); read two rows and rebuild the third.
+void hop_f64(hipStream_t s, const Geom& g, const double2* U, const double2* in, double2* out, double a, double b,
+             const double2* dot_with, CGState* st, double* partials, int gate, bool r12);
 void hop_f32(hipStream_t s, const Geom& g, const float2* U, const float2* in, float2* out, float a, float b,
              const float2* dot_with, CGState* st, double* partials, bool r12);
 void hop_f16(hipStream_t s, const Geom& g, const __half2* U, const H16* in, H16* out, float a, float b,
@@ -214,6 +220,9 @@ int map_status(sfh_ctx* c, const CGState& s, int max_iter);
 double dev_dot(sfh_ctx* c, const double2* a, const double2* b, size_t ncomplex);
 // Brings the FP32 / FP16 link copy of g up to date with its FP64 links.
 int ensure_lp(sfh_ctx* c, sfh_gauge* g, int precision);
+// Measures whether the links are SU(3) if they changed since the last time
+// (one pass over the links and one host sync): sets lp_su3 and f64_su3.
+int ensure_su3(sfh_ctx* c, sfh_gauge* g);
 
 // ------------------------------------------------------------------ batched solves (batch.hip)
 struct BatchSys {

@@ -248,7 +248,7 @@ int hex_smear(sfh_ctx* c, const Geom& g, double2* U, const double alpha[3], int 
   if (int e = hip_check(c, hipStreamSynchronize(c->stream), "SU(3) check")) return e;
   float d;
   std::memcpy(&d, hd, sizeof d);
-  return d < kSu3Tol ? hex_steps<true>(c, g, U, V1, V2, alpha, nsteps) : hex_steps<false>(c, g, U, V1, V2, alpha, nsteps);
+  return d < kSu3TolF64 ? hex_steps<true>(c, g, U, V1, V2, alpha, nsteps) : hex_steps<false>(c, g, U, V1, V2, alpha, nsteps);
 }
 
 }  // namespace sfh

@@ -45,27 +45,29 @@ double dev_dot(sfh_ctx* c, const double2* a, const double2* b, size_t ncomplex) 
   return *c->h_scalar;
 }
 
+int ensure_su3(sfh_ctx* c, sfh_gauge* gauge) {
+  if (gauge->su3_known) return SFH_OK;
+  unsigned* dev = static_cast<unsigned*>(scratch(c, "su3_dev", sizeof(unsigned)));
+  if (!dev) return SFH_E_NOMEM;
+  hipMemsetAsync(dev, 0, sizeof(unsigned), c->stream);
+  su3_deviation(c->stream, gauge->g, gauge->U, dev);
+  unsigned* h = reinterpret_cast<unsigned*>(c->h_scalar);
+  hipMemcpyAsync(h, dev, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream);
+  if (int e = hip_check(c, hipStreamSynchronize(c->stream), "SU(3) check")) return e;
+  float d;
+  std::memcpy(&d, h, sizeof d);
+  gauge->lp_su3 = d < kSu3Tol;
+  gauge->f64_su3 = d < kSu3TolF64;
+  gauge->su3_known = true;
+  return SFH_OK;
+}
+
 // Low-precision links are derived lazily from the FP64 links and invalidated
-// whenever those change.  Building a copy also checks that the FP64 links
-// are SU(3) (third row = conj(row0 x row1) to 1e-10, one pass over the links
-// and one host sync); only then do the low-precision kernels read two rows
-// and rebuild the third.
+// whenever those change; the SU(3) check (ensure_su3) decides whether the
+// low-precision kernels read two rows and rebuild the third.
 int ensure_lp(sfh_ctx* c, sfh_gauge* gauge, int precision) {
   const Geom& g = gauge->g;
-  if ((precision == SFH_PREC_F32 && !gauge->lp32_valid) || (precision == SFH_PREC_F16 && !gauge->lp16_valid)) {
-    if (!gauge->lp32_valid && !gauge->lp16_valid) {
-      unsigned* dev = static_cast<unsigned*>(scratch(c, "su3_dev", sizeof(unsigned)));
-      if (!dev) return SFH_E_NOMEM;
-      hipMemsetAsync(dev, 0, sizeof(unsigned), c->stream);
-      su3_deviation(c->stream, g, gauge->U, dev);
-      unsigned* h = reinterpret_cast<unsigned*>(c->h_scalar);
-      hipMemcpyAsync(h, dev, sizeof(unsigned), hipMemcpyDeviceToHost, c->stream);
-      if (int e = hip_check(c, hipStreamSynchronize(c->stream), "SU(3) check")) return e;
-      float d;
-      std::memcpy(&d, h, sizeof d);
-      gauge->lp_su3 = d < kSu3Tol;
-    }
-  }
+  if (int e = ensure_su3(c, gauge)) return e;
   if (precision == SFH_PREC_F32 && !gauge->lp32_valid) {
     if (!gauge->lp.u32 && hipMalloc(&gauge->lp.u32, 36 * (size_t)g.V * sizeof(float2)) != hipSuccess)
       return set_err(c, SFH_E_NOMEM, "FP32 link copy");
@@ -275,9 +277,10 @@ int apply_ddag_dev(sfh_ctx* c, const sfh_gauge* gauge, double mass, const double
   const Geom& g = gauge->g;
   double2* t = buf<double2>(c, "ddag_tmp", 3 * (size_t)g.V);
   if (!t) return SFH_E_NOMEM;
+  if (int e = ensure_su3(c, const_cast<sfh_gauge*>(gauge))) return e;
   KTimer kt(c, "hop_f64");
-  hop_f64(c->stream, g, gauge->U, in, t, mass, -1.0, nullptr, nullptr, nullptr);
-  hop_f64(c->stream, g, gauge->U, t, out, mass, +1.0, nullptr, nullptr, nullptr);
+  hop_f64(c->stream, g, gauge->U, in, t, mass, -1.0, nullptr, nullptr, nullptr, 0, gauge->f64_su3);
+  hop_f64(c->stream, g, gauge->U, t, out, mass, +1.0, nullptr, nullptr, nullptr, 0, gauge->f64_su3);
   return hip_check(c, hipGetLastError(), "DDdag");
 }
 
@@ -285,6 +288,7 @@ static int cg_f64(sfh_ctx* c, const sfh_gauge* gauge, const sfh_cg_params* prm, 
                   double phi_sq, sfh_cg_stats* stats) {
   const Geom& g = gauge->g;
   const size_t n = 3 * (size_t)g.V;
+  if (int e = ensure_su3(c, const_cast<sfh_gauge*>(gauge))) return e;
   Bufs64 b{buf<double2>(c, "cg_r", n), buf<double2>(c, "cg_p", n), buf<double2>(c, "cg_t", n),
            buf<double2>(c, "cg_v", n)};
   if (!b.r || !b.p || !b.t || !b.v) return SFH_E_NOMEM;
@@ -304,11 +308,11 @@ static int cg_f64(sfh_ctx* c, const sfh_gauge* gauge, const sfh_cg_params* prm, 
     auto iter = [&]() {
       {
         KTimer kt(c, "hop_f64");
-        hop_f64(c->stream, g, gauge->U, b.p, b.t, prm->mass, -1.0, nullptr, st, nullptr);
+        hop_f64(c->stream, g, gauge->U, b.p, b.t, prm->mass, -1.0, nullptr, st, nullptr, 0, gauge->f64_su3);
       }
       {
         KTimer kt(c, "hop_dot_f64");
-        hop_f64(c->stream, g, gauge->U, b.t, b.v, prm->mass, +1.0, b.p, st, part);
+        hop_f64(c->stream, g, gauge->U, b.t, b.v, prm->mass, +1.0, b.p, st, part, 0, gauge->f64_su3);
       }
       KTimer kt(c, "cg_blas");
       cg_xr_f64(c->stream, g, x, b.r, b.p, b.v, st, part + kPartialSlot);
@@ -318,7 +322,7 @@ static int cg_f64(sfh_ctx* c, const sfh_gauge* gauge, const sfh_cg_params* prm, 
       for (int j = 0; j < kChunk; ++j) iter();
     };
     const std::string key = graph_key("f64", g, {gauge->U, x, b.r, b.p, b.t, b.v, st, part}, prm->mass,
-                                      get_hop_variant(0, g));
+                                      get_hop_variant(0, g) * 2 + (gauge->f64_su3 ? 1 : 0));
     if (int e = run_chunks(c, [&]() { return launch_chunk(c, key, direct); }, kChunk, s, fin)) return e;
   }
   if (stats) {
@@ -392,8 +396,8 @@ static int cg_mixed(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* prm, cons
     KTimer kt(c, "reliable_update");
     if (defer) mix_fin(c->stream, g, st, part);
     mix_acc<T>(c->stream, g, x, b, st);
-    hop_f64(c->stream, g, gauge->U, x, t64, prm->mass, -1.0, nullptr, st, nullptr, 1);
-    hop_f64(c->stream, g, gauge->U, t64, ax, prm->mass, +1.0, nullptr, st, nullptr, 1);
+    hop_f64(c->stream, g, gauge->U, x, t64, prm->mass, -1.0, nullptr, st, nullptr, 1, gauge->f64_su3);
+    hop_f64(c->stream, g, gauge->U, t64, ax, prm->mass, +1.0, nullptr, st, nullptr, 1, gauge->f64_su3);
     ru_sub(c->stream, g, r64, phi, ax, st, part_ru);
     mix_apply<T>(c->stream, g, r64, b, st);
   };
@@ -405,8 +409,8 @@ static int cg_mixed(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* prm, cons
                                     {gauge->U, (const void*)U, x, r64, phi, t64, ax, b.r[0], b.r[1], b.p[0], b.p[1],
                                      b.v, b.t, b.rn[0], b.rn[1], b.pn[0], b.pn[1], b.vn, b.tn, b.x, st, sts[1], part},
                                     prm->mass,
-                                    (get_hop_variant(0, g) * 4 + get_hop_variant(HALF ? 2 : 1, g)) * 2 +
-                                        (gauge->lp_su3 ? 1 : 0));
+                                    ((get_hop_variant(0, g) * 4 + get_hop_variant(HALF ? 2 : 1, g)) * 2 +
+                                     (gauge->lp_su3 ? 1 : 0)) * 2 + (gauge->f64_su3 ? 1 : 0));
   CGState fin{};
   if (int e = run_chunks(c, [&]() { return launch_chunk(c, key, direct); }, kChunkMixed<HALF>, s, fin)) return e;
   if (stats) {

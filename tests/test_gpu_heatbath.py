@@ -162,6 +162,8 @@ def test_heatbath_strong_coupling_plaquette(ctx):
     u = F.u_exact(1.0)
     for x in (np.array(meas), np.array(rep)):
         assert abs(x.mean() - u) < 4 * x.std() / np.sqrt(len(x)) + 1e-4, (x.mean(), u)
+    # 120 sweeps of links stored as two rows stay unitary to rounding
+    assert max(_unitarity(g.download())) < 1e-12
 
 
 @pytest.mark.parametrize("dims", [(16, 16, 16, 16), (32, 32, 32, 32)])
