@@ -562,8 +562,8 @@ def main():
                          "and one FP32 link conversion; for hex, two 32^4 steps and one FP32 link conversion")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is a "
-                                   "design only, see DESIGN.md)"}))
+        print(json.dumps({"error": "runs on one GPU only; --gpus must be 1 (the lattice would shard by "
+                                   "time slices with halo exchange, which is not implemented)"}))
         sys.exit(2)
 
     from staggered_fermions_hmc_amd import device as dev
