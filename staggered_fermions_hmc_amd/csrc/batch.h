@@ -133,7 +133,11 @@ template <class T> class Engine {
   bool use_graph_;
   void launch_chunk();
   std::vector<SysH> sys_;
-  bool r12_ = true;  // every system's links are SU(3): the mixed hops read two rows
+  // every system's links are SU(3): the iteration's hops read two rows (to
+  // kSu3Tol for the FP32 copies, kSu3TolF64 for FP64 links), and so do the
+  // FP64 hops of the reliable updates (r12_64_)
+  bool r12_ = true;
+  bool r12_64_ = true;
   std::vector<SlotH> slots_;
   std::vector<int> done_;
   std::vector<void*> owned_;

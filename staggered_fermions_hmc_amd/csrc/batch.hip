@@ -252,7 +252,7 @@ __global__ __launch_bounds__(kBlas) void kb_ru_acc(const BSlot<T>* __restrict__ 
   }
 }
 
-template <class T, int PHASE>
+template <class T, int PHASE, bool R12>
 __global__ __launch_bounds__(256) void kb_ru_hop(const BSys<T>* __restrict__ sys, const BSlot<T>* __restrict__ slots,
                                                  const int4* __restrict__ bmap) {
   const int4 m = bmap[blockIdx.x];
@@ -283,7 +283,7 @@ __global__ __launch_bounds__(256) void kb_ru_hop(const BSys<T>* __restrict__ sys
     const double2* in = PHASE == 0 ? L.x64 : L.t64;
     double2* out = PHASE == 0 ? L.t64 : L.ax;
     vec3<double> acc, self;
-    site_hop<double>(S.U64, in, V, i, q, nb, acc);
+    site_hop<double, R12>(S.U64, in, V, i, q, nb, acc);
     load_spinor<double>(in, nullptr, V, i, self);
 #pragma unroll
     for (int c = 0; c < 3; ++c)
@@ -363,10 +363,10 @@ void Engine<T>::launch_chunk() {
   // FP64: the reference recurrence, hop, hop + <p,v>, x/r update, p update
   for (int it = 0; it < chunk_; ++it) {
     if constexpr (kMixed) hipLaunchKernelGGL((kb_upd<T>), dim3(nbb_), dim3(kBlas), 0, c_->stream, dslot_, dmb_);
-    if (kMixed && r12_) {
-      hipLaunchKernelGGL((kb_hop<T, false, kMixed>), dim3(nbh0_), dim3(cfg_.hop_threads), 0, c_->stream, dsys_,
+    if (r12_) {
+      hipLaunchKernelGGL((kb_hop<T, false, true>), dim3(nbh0_), dim3(cfg_.hop_threads), 0, c_->stream, dsys_,
                          dslot_, dmh0_, (R)(-0.5));
-      hipLaunchKernelGGL((kb_hop<T, true, kMixed>), dim3(nbh_), dim3(256), 0, c_->stream, dsys_, dslot_, dmh_,
+      hipLaunchKernelGGL((kb_hop<T, true, true>), dim3(nbh_), dim3(256), 0, c_->stream, dsys_, dslot_, dmh_,
                          (R)(0.5));
     } else {
       hipLaunchKernelGGL((kb_hop<T, false, false>), dim3(nbh0_), dim3(cfg_.hop_threads), 0, c_->stream, dsys_, dslot_,
@@ -387,8 +387,13 @@ template <class T>
 void Engine<T>::launch_tail() {
   if constexpr (kMixed) {
     hipLaunchKernelGGL((kb_ru_acc<T>), dim3(nbb_), dim3(kBlas), 0, c_->stream, dslot_, dmb_);
-    hipLaunchKernelGGL((kb_ru_hop<T, 0>), dim3(nbh_), dim3(256), 0, c_->stream, dsys_, dslot_, dmh_);
-    hipLaunchKernelGGL((kb_ru_hop<T, 1>), dim3(nbh_), dim3(256), 0, c_->stream, dsys_, dslot_, dmh_);
+    if (r12_64_) {
+      hipLaunchKernelGGL((kb_ru_hop<T, 0, true>), dim3(nbh_), dim3(256), 0, c_->stream, dsys_, dslot_, dmh_);
+      hipLaunchKernelGGL((kb_ru_hop<T, 1, true>), dim3(nbh_), dim3(256), 0, c_->stream, dsys_, dslot_, dmh_);
+    } else {
+      hipLaunchKernelGGL((kb_ru_hop<T, 0, false>), dim3(nbh_), dim3(256), 0, c_->stream, dsys_, dslot_, dmh_);
+      hipLaunchKernelGGL((kb_ru_hop<T, 1, false>), dim3(nbh_), dim3(256), 0, c_->stream, dsys_, dslot_, dmh_);
+    }
     hipLaunchKernelGGL((kb_ru_sub<T>), dim3(nbb_), dim3(kBlas), 0, c_->stream, dslot_, dmb_);
     hipLaunchKernelGGL((kb_ru_apply<T>), dim3(nbb_), dim3(kBlas), 0, c_->stream, dslot_, dmb_);
   }
@@ -412,12 +417,15 @@ int Engine<T>::add_system(const sfh_gauge* gauge, double mass, int pool) {
   SysH s;
   s.gauge = gauge;
   s.mass = mass;
+  if (int e = ensure_su3(c_, gk)) return -e;
+  r12_64_ = r12_64_ && gk->f64_su3;
   if (kMixed) {
     if (int e = ensure_lp(c_, gk, SFH_PREC_F32)) return -e;
     s.U = reinterpret_cast<const typename Prec<T>::L*>(gk->lp.u32);
     r12_ = r12_ && gk->lp_su3;
   } else {
     s.U = reinterpret_cast<const typename Prec<T>::L*>(gk->U);
+    r12_ = r12_64_;
   }
   const int V = gauge->g.V;
   const size_t sb = slot_bytes(V);
@@ -643,8 +651,8 @@ int Engine<T>::time_hop(const BatchConfig& cfg, int n, double* us) {
   if (!dm) return SFH_E_NOMEM;
   hipMemcpyAsync(dm, m.data(), m.size() * sizeof(int4), hipMemcpyHostToDevice, c_->stream);
   auto launch = [&]() {
-    if (kMixed && r12_)
-      hipLaunchKernelGGL((kb_hop<T, false, kMixed>), dim3((int)m.size()), dim3(cfg.hop_threads), 0, c_->stream, dsys_,
+    if (r12_)
+      hipLaunchKernelGGL((kb_hop<T, false, true>), dim3((int)m.size()), dim3(cfg.hop_threads), 0, c_->stream, dsys_,
                          dslot_, dm, (R)(-0.5));
     else
       hipLaunchKernelGGL((kb_hop<T, false, false>), dim3((int)m.size()), dim3(cfg.hop_threads), 0, c_->stream, dsys_,
