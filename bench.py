@@ -546,8 +546,8 @@ def bench_hex(args, dev):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="16^4",
                     choices=sorted(WORKLOADS) + ["batch", "serve", "precision", "traj32", "heatbath", "hex"])
     ap.add_argument("--rates", default="0.5,1,2", help="serve: arrival rates as multiples of the measured "

@@ -351,19 +351,18 @@ __global__ __launch_bounds__(kBlk) void k_mix_fin(CGState* st, const double* __r
 }
 
 // Split form, own-site part of kernel A: r_k, p_k and x updated in place
-// (r and p single-buffered: both parity slots point to one array).  The hop
+// (r and p single-buffered: both slots point to one array).  The hop
 // t = D^dag p_k then runs as a separate launch.
 template <class T>
 __global__ __launch_bounds__(kBlk) void k_mix_upd(int V, MixBufs<T> b, const CGState* st) {
   if (st->done) return;
   const int i = blockIdx.x * kBlk + threadIdx.x;
   if (i >= V) return;
-  const int cur = st->iter & 1, prv = cur ^ 1;
   const float al = (float)st->alpha, be = (float)st->beta;
   vec3<float> ro, vo, po;
-  load_spinor<T>(b.r[prv], b.rn[prv], V, i, ro);
+  load_spinor<T>(b.r[0], b.rn[0], V, i, ro);
   load_spinor<T>(b.v, b.vn, V, i, vo);
-  load_spinor<T>(b.p[prv], b.pn[prv], V, i, po);
+  load_spinor<T>(b.p[0], b.pn[0], V, i, po);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const size_t j = (size_t)c * V + i;
@@ -374,8 +373,8 @@ __global__ __launch_bounds__(kBlk) void k_mix_upd(int V, MixBufs<T> b, const CGS
   }
   Q16 qr, qp;
   advance<T>(ro, vo, po, al, be, qr, qp);
-  store_q<T>(b.r[cur], b.rn[cur], V, i, ro, qr);
-  store_q<T>(b.p[cur], b.pn[cur], V, i, po, qp);
+  store_q<T>(b.r[0], b.rn[0], V, i, ro, qr);
+  store_q<T>(b.p[0], b.pn[0], V, i, po, qp);
 }
 
 template <class T>
