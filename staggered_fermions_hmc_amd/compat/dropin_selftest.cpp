@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/sfh.h"
@@ -240,6 +241,44 @@ static int run() {
         }
     }
     report("4HEX keeps links unitary", err < 1e-12, err);
+  }
+
+  // several host threads on different lattices at once, as the reference's
+  // Main.cpp runs its configurations under `omp parallel for` (Main.cpp:174):
+  // each thread gets its own context and stream, and every result equals the
+  // same calls made one after another on one thread, bit for bit
+  {
+    constexpr int kThreads = 4;
+    struct Out {
+      std::vector<double> X, F;
+      double S = 0;
+    };
+    auto work = [&](int t, Out& o) {
+      const Link_array U = hot(0.3, 500 + t);
+      std::vector<double> phi((size_t)V * 6);
+      for (size_t k = 0; k < phi.size(); ++k) phi[k] = std::sin(0.37 * (double)k + t);
+      o.X = solve_X_field(phi, U);
+      o.F = compute_full_force_field(o.X, solve_Y_field(o.X, U), U, beta);
+      o.S = compute_action(U, beta);
+    };
+    std::vector<Out> serial(kThreads), conc(kThreads);
+    for (int t = 0; t < kThreads; ++t) work(t, serial[t]);
+    std::vector<std::thread> pool;
+    std::vector<std::string> errors(kThreads);
+    for (int t = 0; t < kThreads; ++t)
+      pool.emplace_back([&, t]() {
+        try {
+          work(t, conc[t]);
+        } catch (const std::exception& e) {
+          errors[t] = e.what();
+        }
+      });
+    for (auto& th : pool) th.join();
+    bool same = true;
+    for (int t = 0; t < kThreads; ++t)
+      same = same && errors[t].empty() && conc[t].X == serial[t].X && conc[t].F == serial[t].F &&
+             conc[t].S == serial[t].S;
+    report("4 threads at once = the same calls serially", same, same ? 0.0 : 1.0);
   }
 
   std::printf("%s: %d failure(s)\n", failures ? "FAIL" : "PASS", failures);
