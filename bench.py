@@ -702,7 +702,7 @@ def main():
         "timing": "HIP events on the library stream around 50 back-to-back launches of each kernel as the solver "
                   "runs it at this lattice",
         "bytes_note": "algorithmic bytes count the full 18-real links (72 link reals per site); the links here are "
-                      "SU(3), so the low-precision kernels read two rows per link and rebuild the third, and their "
+                      "SU(3), so every hop (FP64 included) reads two rows per link and rebuilds the third, and the "
                       "measured traffic (traffic) is below the algorithmic bytes",
     }
 
