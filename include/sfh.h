@@ -63,8 +63,10 @@ typedef enum {
 /* Arithmetic of the CG inner iterations.  F64 follows the reference's
  * recurrence (Fermions.cpp:553-617).  F32 and F16 run the inner iterations in
  * low precision with FP64 reliable updates (true residual recomputed in FP64
- * whenever the iterated residual drops by reliable_delta), FP64 global
- * reductions, and stop on the FP64 true residual.  F16 stores spinors as
+ * once the iterated residual has dropped by reliable_delta, at the end of the
+ * chunk of iterations in which that happens: up to 15 FP32 or 7 FP16
+ * iterations later), FP64 global reductions, and stop on the FP64 true
+ * residual.  F16 stores spinors as
  * 16-bit floats with a per-site FP32 max-norm and links as 16-bit floats,
  * accumulating in FP32. */
 typedef enum { SFH_PREC_F64 = 0, SFH_PREC_F32 = 1, SFH_PREC_F16 = 2 } sfh_precision;
@@ -83,8 +85,9 @@ typedef struct {
   double tol;            /* relative residual, Parameters.h:23                */
   int max_iter;          /* Parameters.h:24                                   */
   int precision;         /* sfh_precision                                     */
-  double reliable_delta; /* mixed precision: reliable update when |r| drops by
-                            this factor since the last one (0 -> 0.1)         */
+  double reliable_delta; /* mixed precision: reliable update due when |r|
+                            drops by this factor since the last one, run at
+                            the end of that chunk (0 -> 0.1)                  */
   int compute_true_residual; /* F64: also report the FP64 true residual       */
 } sfh_cg_params;
 
