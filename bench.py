@@ -563,7 +563,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "runs on one GPU only; --gpus must be 1 (the lattice would shard by "
-                                   "time slices with halo exchange, which is not implemented)"}))
+                                   "4-D domain decomposition with halo exchange, which is not implemented)"}))
         sys.exit(2)
 
     from staggered_fermions_hmc_amd import device as dev
