@@ -38,6 +38,10 @@ extern "C" {
  * EXP_QCUBED / EXP_NEG_TRACE         <- std::invalid_argument in
  *                                      numerically_stable_matrix_exponential
  *                                      (Fermions.cpp:848-857)
+ * IO                                 <- the false return of
+ *                                      save_all_lattice_configs /
+ *                                      load_all_lattice_configs
+ *                                      (Lattice.cpp:810-847)
  * SAMPLER                            <- no counterpart: a heatbath rejection
  *                                      loop (SU3_Sampling.cpp:17-29, :53-68,
  *                                      :175-191) that found no sample in 2^20
@@ -57,7 +61,8 @@ typedef enum {
   SFH_E_NOT_HERMITIAN = 9,
   SFH_E_EXP_QCUBED = 10,
   SFH_E_EXP_NEG_TRACE = 11,
-  SFH_E_SAMPLER = 12
+  SFH_E_SAMPLER = 12,
+  SFH_E_IO = 13
 } sfh_status;
 
 /* Arithmetic of the CG inner iterations.  F64 follows the reference's
@@ -88,7 +93,12 @@ typedef struct {
   double reliable_delta; /* mixed precision: reliable update due when |r|
                             drops by this factor since the last one, run at
                             the end of that chunk (0 -> 0.1)                  */
-  int compute_true_residual; /* F64: also report the FP64 true residual       */
+  int compute_true_residual; /* F64: 1 also report the FP64 true residual; 2
+                                also stop on it (once the iterated residual
+                                meets tol, r = phi - D D^dag x is formed and
+                                the recurrence restarts from it, p = r, while
+                                it misses tol).  Mixed precision always stops
+                                on the FP64 true residual.                    */
 } sfh_cg_params;
 
 typedef struct {
@@ -252,19 +262,36 @@ int sfh_tune_batch(sfh_ctx* ctx, int nsys, const sfh_batch_system* systems, cons
 /* ---------------------------------------------------------------- solve service
  * Continuous-batching CG service for concurrent HMC chains (mixed precision:
  * FP32 inner, FP64 reliable updates, stop on the FP64 true residual).
- * A chain registers its device-resident links once with a pool of up to 8
- * solves in flight; the pool's memory is reserved at registration and a
- * chain that would exceed mem_budget (0 = unlimited) is refused with
- * SFH_E_NOMEM.  Submitted requests queue FIFO and are admitted as their
- * chain's slots free up; a worker thread advances every solve in flight
- * together, a chunk of iterations at a time, so requests that arrive while
- * others run join at the next chunk boundary.  Each result equals the
- * solve alone (bit-identical).  stats.seconds is the latency from submit to
- * completion.  The links of a registered chain must not change while it is
- * registered.  No counterpart in the reference. */
+ * A chain registers its device-resident links once, with a cap of up to 8
+ * solves in flight; a chain one solve of which exceeds mem_budget
+ * (0 = unlimited) is refused with SFH_E_NOMEM.  Submitted requests queue
+ * FIFO per chain and are admitted by memory as they arrive: into a free
+ * slot of the chain, else into a new slot while the chain is below its cap,
+ * the service's slot memory stays within mem_budget and free HBM
+ * (hipMemGetInfo at that moment) keeps a headroom of max(256 MiB, 1/64 of
+ * the device); otherwise the request waits for a slot to free up (a
+ * request of a chain with no slot, while nothing runs, completes with
+ * SFH_E_NOMEM).  A worker thread advances every solve in flight together, a
+ * chunk of iterations at a time, so requests that arrive while others run
+ * join at the next chunk boundary.  Each result equals the solve alone
+ * (bit-identical).  stats.seconds is the latency from submit to completion.
+ * The links of a chain must not change while it has requests queued or in
+ * flight; after they change, sfh_service_update_chain refreshes the chain.
+ * No counterpart in the reference. */
 typedef struct sfh_service sfh_service;
 int sfh_service_create(int device, size_t mem_budget, double tol, int max_iter, sfh_service** out);
 int sfh_service_add_chain(sfh_service* s, const sfh_gauge* g, double mass, int max_inflight, int* chain);
+/* New links for a chain with no request queued or in flight (g: the same
+ * lattice shape, e.g. the chain's own gauge after a link update) and its
+ * mass: the SU(3) check and the FP32 link copy are redone.  SFH_E_INVALID if
+ * requests are pending. */
+int sfh_service_update_chain(sfh_service* s, int chain, const sfh_gauge* g, double mass);
+/* Drops a chain with no request queued or in flight; its solve slots stay
+ * allocated for other chains' requests.  The id is not reused. */
+int sfh_service_remove_chain(sfh_service* s, int chain);
+/* Device memory held by solve slots, and how many times a new slot was
+ * refused for memory (the request then waited). */
+int sfh_service_memory(sfh_service* s, size_t* slot_bytes, long* refusals);
 int sfh_service_submit(sfh_service* s, int chain, const sfh_field* rhs, sfh_field* x, long long* ticket);
 /* Waits for one ticket (returns its status; SFH_E_HIP on timeout). */
 int sfh_service_wait(sfh_service* s, long long ticket, double timeout_s, sfh_cg_stats* stats);
@@ -290,6 +317,30 @@ int sfh_avg_plaq(sfh_ctx* ctx, const sfh_gauge* g, double* out);
  * poly_loop_at_spat_coord (Lattice.cpp:849-884); the correlators of
  * Lattice.cpp:888-948 are cheap host sums over this grid. */
 int sfh_polyakov_grid(sfh_ctx* ctx, const sfh_gauge* g, double* out);
+/* Polyakov-loop correlators of the device-resident links:
+ *   C(r^2) = (1/N) sum_m sum_{d in D(r^2)} P(m) conj(P(m + d)),
+ * P the grid of sfh_polyakov_grid (computed on the device, never copied to
+ * the host), D(r^2) the integer spatial displacements with d.d = r^2 and
+ * |d_k| <= min(floor(sqrt(r^2)), L_k / 2), d_k = +L_k / 2 left out for even
+ * L_k, N = (L0 L1 L2) |D(r^2)|, and C = 0 when D(r^2) is empty.
+ * out[2k], out[2k+1] = Re, Im C(r2_first + k) for k < count.  Replaces the
+ * measurement step precompute_polyakov_grid + correlator_over_fixed_distance_fast
+ * (Lattice.cpp:868-948) as Main.cpp:219-228 calls it (r2_first = 1,
+ * count = maxdistance). */
+int sfh_polyakov_correlators(sfh_ctx* ctx, const sfh_gauge* g, int r2_first, int count, double* out);
+/* The same correlators of a host grid (complex (re, im) pairs at
+ * [(x*L1 + y)*L2 + z], spatial = {L0, L1, L2}): replaces
+ * correlator_over_fixed_distance_fast (Lattice.cpp:888-948) on the grid
+ * that precompute_polyakov_grid returns. */
+int sfh_polyakov_correlators_grid(sfh_ctx* ctx, const int spatial[3], const double* grid, int r2_first, int count,
+                                  double* out);
+/* Host link arrays (n of them, doubles_each doubles each) written back to
+ * back as raw little-endian doubles, and read back: the byte format of
+ * save_all_lattice_configs / load_all_lattice_configs (Lattice.cpp:810-847),
+ * the same as sfh_gauge_save / sfh_gauge_load.  SFH_E_IO where the reference
+ * returns false (the file cannot be opened, or is shorter than n arrays). */
+int sfh_links_save(const double* const* arrays, int n, size_t doubles_each, const char* path);
+int sfh_links_load(double* const* arrays, int n, size_t doubles_each, const char* path);
 /* nsweeps Cabibbo-Marinari heatbath sweeps of the Symanzik gauge action
  * (beta, c11, c12) on the device-resident links.  Replaces heatbath_update
  * (Lattice.cpp:725-757) with single_link_heatbath (:696-723), the subgroup

@@ -25,6 +25,7 @@
 // rectangle; finite-difference derivatives of the actions against the
 // forces (dS/de = 2 ReTr(F H), ai_check.cpp:31-34); eigendecomposition for
 // the exponential; reversibility and dH ~ eps^2 for the integrator.
+#include <array>
 #include <cmath>
 #include <complex>
 #include <cstdint>
@@ -634,6 +635,53 @@ int oc_polyakov_grid(const int* dims, const double* U, double* out) {
     out[2 * k] = tr.real();
     out[2 * k + 1] = tr.imag();
   }
+  return OK;
+}
+
+// correlator_over_fixed_distance_fast, Lattice.cpp:888-948: the
+// displacements (dx, dy, dz) in ascending loop order with
+// |d_k| <= min(floor(sqrt(r^2)), L_k / 2), d_k = L_k / 2 skipped for even L_k
+// and d.d = r^2 (:892-913, one Spatial_Size generalised to L0, L1, L2); then
+// the sum of P(m) conj(P(m + d)) over the grid in (x, y, z) order with the
+// displacements innermost, periodic wrap, divided by the number of terms
+// (:915-947).  Serial, in the reference's order.
+int oc_polyakov_correlator(const int* spatial, const double* grid, int r_squared, double* out) {
+  const int L0 = spatial[0], L1 = spatial[1], L2 = spatial[2];
+  const int L[3] = {L0, L1, L2};
+  int md[3];
+  for (int k = 0; k < 3; ++k) md[k] = std::min(static_cast<int>(std::sqrt(r_squared)), L[k] / 2);
+  std::vector<std::array<int, 3>> disp;
+  for (int dx = -md[0]; dx <= md[0]; dx++) {
+    if (dx == L0 / 2 && L0 % 2 == 0) continue;
+    for (int dy = -md[1]; dy <= md[1]; dy++) {
+      if (dy == L1 / 2 && L1 % 2 == 0) continue;
+      for (int dz = -md[2]; dz <= md[2]; dz++) {
+        if (dz == L2 / 2 && L2 % 2 == 0) continue;
+        if (dx * dx + dy * dy + dz * dz == r_squared) disp.push_back({dx, dy, dz});
+      }
+    }
+  }
+  out[0] = out[1] = 0.0;
+  if (disp.empty()) return OK;
+  const std::complex<double>* P = reinterpret_cast<const std::complex<double>*>(grid);
+  std::complex<double> acc = 0.0;
+  long N = 0;
+  for (int x = 0; x < L0; x++)
+    for (int y = 0; y < L1; y++)
+      for (int z = 0; z < L2; z++) {
+        const std::complex<double> pm = P[(x * L1 + y) * L2 + z];
+        for (const auto& d : disp) {
+          int nx = x + d[0], ny = y + d[1], nz = z + d[2];
+          if (nx < 0) nx += L0; else if (nx >= L0) nx -= L0;
+          if (ny < 0) ny += L1; else if (ny >= L1) ny -= L1;
+          if (nz < 0) nz += L2; else if (nz >= L2) nz -= L2;
+          acc += pm * std::conj(P[(nx * L1 + ny) * L2 + nz]);
+          N++;
+        }
+      }
+  acc /= static_cast<double>(N);
+  out[0] = acc.real();
+  out[1] = acc.imag();
   return OK;
 }
 

@@ -50,6 +50,7 @@ def lib():
             "oc_hmc_trajectory": (_I, [_P, _D, _D, _I, _D, _D, _D, _D, _I, _P, _P, _D, _P, _P, _P]),
             "oc_set_threads": (_I, [_I]),
             "oc_polyakov_grid": (_I, [_P, _P, _P]),
+            "oc_polyakov_correlator": (_I, [_P, _P, _I, _P]),
             "oc_philox": (_I, [_P, _P, _P]),
             "oc_heatbath_x0": (_I, [_D, _D, ctypes.c_ulonglong, ctypes.c_uint, _I, _P]),
             "oc_heatbath_subgroup": (_I, [_P, _P, _D, _I, ctypes.c_ulonglong, ctypes.c_uint, ctypes.c_uint, _P,
@@ -210,6 +211,17 @@ def polyakov_grid(dims, U):
     out = np.empty(2 * int(d[0] * d[1] * d[2]))
     lib().oc_polyakov_grid(_p(d), _p(U), _p(out))
     return out.view(np.complex128)
+
+
+def polyakov_correlator(spatial, grid, r_squared):
+    """correlator_over_fixed_distance_fast (Lattice.cpp:888-948) of a complex
+    grid [(x*L1 + y)*L2 + z] for one r^2, in the reference's summation order."""
+    sp = np.ascontiguousarray(np.asarray(spatial, dtype=np.int32))
+    g = np.ascontiguousarray(np.asarray(grid, dtype=np.complex128)).view(np.float64)
+    assert sp.shape == (3,) and g.size == 2 * int(np.prod(sp))
+    out = np.empty(2)
+    lib().oc_polyakov_correlator(_p(sp), _p(g), int(r_squared), _p(out))
+    return complex(out[0], out[1])
 
 
 def philox(key, ctr):

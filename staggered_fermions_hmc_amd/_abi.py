@@ -93,6 +93,9 @@ SIGNATURES = {
                             ctypes.POINTER(_I), ctypes.POINTER(_D)]),
     "sfh_service_create": (_I, [_I, _SZ, _D, _I, ctypes.POINTER(_P)]),
     "sfh_service_add_chain": (_I, [_P, _P, _D, _I, ctypes.POINTER(_I)]),
+    "sfh_service_update_chain": (_I, [_P, _I, _P, _D]),
+    "sfh_service_remove_chain": (_I, [_P, _I]),
+    "sfh_service_memory": (_I, [_P, ctypes.POINTER(_SZ), ctypes.POINTER(ctypes.c_long)]),
     "sfh_service_submit": (_I, [_P, _I, _P, _P, ctypes.POINTER(ctypes.c_longlong)]),
     "sfh_service_wait": (_I, [_P, ctypes.c_longlong, _D, ctypes.POINTER(CGStats)]),
     "sfh_service_poll": (_I, [_P, _D, _I, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(CGStats),
@@ -104,6 +107,10 @@ SIGNATURES = {
     "sfh_avg_plaq": (_I, [_P, _P, ctypes.POINTER(_D)]),
     "sfh_tr_p2": (_I, [_P, _P, _P, ctypes.POINTER(_D)]),
     "sfh_polyakov_grid": (_I, [_P, _P, _P]),
+    "sfh_polyakov_correlators": (_I, [_P, _P, _I, _I, _P]),
+    "sfh_polyakov_correlators_grid": (_I, [_P, _P, _P, _I, _I, _P]),
+    "sfh_links_save": (_I, [_P, _I, ctypes.c_size_t, ctypes.c_char_p]),
+    "sfh_links_load": (_I, [_P, _I, ctypes.c_size_t, ctypes.c_char_p]),
     "sfh_heatbath": (_I, [_P, _P, _D, _D, _D, ctypes.c_ulonglong, ctypes.c_uint, _I, _P, _P]),
     "sfh_hex_smear": (_I, [_P, _P, _P, _I]),
     "sfh_link_update": (_I, [_P, _P, _D, _P, _I]),

@@ -15,12 +15,17 @@
 // from inside an OpenMP loop over coupling values, Main.cpp:174).
 #pragma once
 
+#include <array>
+#include <complex>
+#include <cstddef>
+#include <string>
 #include <utility>
 #include <vector>
 
 #include "sfh_compat.h"
 
 using Link_array = std::vector<double>;
+using complex = std::complex<double>;
 
 std::vector<double> generate_phi_field(const Link_array& arr);
 std::vector<double> generate_conjugate_field_configuration();
@@ -73,3 +78,28 @@ std::pair<double, double> heatbath_update(Link_array& arr, double beta);
 void Apply_HEX_smearing(Link_array& U_array);
 void Apply_4HEX_smearing(Link_array& U_array);
 std::vector<Link_array> Apply_4HEX_smearing_with_history(const Link_array& U_thin);
+
+// Measurement step of Main.cpp:219-228 (Lattice.h:84-85): the Polyakov grid
+// Tr prod_t U_3 at every spatial site [(x*L + y)*L + z] (Lattice.cpp:868-884)
+// and the correlator over the displacements with d.d = r_squared
+// (Lattice.cpp:888-948), both computed on the device.
+std::vector<complex> precompute_polyakov_grid(const Link_array& arr);
+complex correlator_over_fixed_distance_fast(const std::vector<complex>& P_grid, int r_squared);
+
+// Checkpoints in the reference's byte format (Lattice.h:43-44,
+// Lattice.cpp:810-847): the CONFIG arrays back to back as raw doubles; false
+// (and the reference's message on std::cerr) when the file cannot be opened
+// or written / read completely.  CONFIG is deduced from the array.
+namespace sfh_compat {
+bool save_links(const Link_array* arrays, size_t n, const std::string& filename);
+bool load_links(Link_array* arrays, size_t n, const std::string& filename);
+}  // namespace sfh_compat
+
+template <size_t N>
+bool save_all_lattice_configs(const std::array<Link_array, N>& links_at_coupling, const std::string& filename) {
+  return sfh_compat::save_links(links_at_coupling.data(), N, filename);
+}
+template <size_t N>
+bool load_all_lattice_configs(std::array<Link_array, N>& links_at_coupling, const std::string& filename) {
+  return sfh_compat::load_links(links_at_coupling.data(), N, filename);
+}

@@ -2,6 +2,8 @@
 // field-level calls the reference's own checks use (ai_check.cpp:172-333,
 // Fermions.cpp:908-955, :1061-1108).  Prints PASS/FAIL per check and exits
 // non-zero if any fails.  Needs a GPU.
+#include <algorithm>
+#include <array>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -281,12 +283,91 @@ static int run() {
     report("4 threads at once = the same calls serially", same, same ? 0.0 : 1.0);
   }
 
+  // measurement step of Main.cpp:219-228 on cold links: P = 3, C(r^2) = 9
+  {
+    const Link_array U = cold();
+    const std::vector<complex> P = precompute_polyakov_grid(U);
+    double dp = 0.0, dc = 0.0;
+    for (const complex& p : P) dp = std::max(dp, std::abs(p - 3.0));
+    const int maxdistance = cfg.Spatial_Size / 2 - 1;  // Parameters.h:15
+    for (int r2 = 1; r2 <= maxdistance; ++r2) dc = std::max(dc, std::abs(correlator_over_fixed_distance_fast(P, r2) - 9.0));
+    report("cold Polyakov grid = 3, correlators = 9", P.size() == 64 && dp < 1e-14 && dc < 1e-13, std::max(dp, dc));
+    // a hot grid: C(1) summed over the six axis neighbours by hand
+    const Link_array H = hot(0.6, 77);
+    const std::vector<complex> Q = precompute_polyakov_grid(H);
+    const int L = cfg.Spatial_Size;
+    complex acc = 0.0;
+    for (int x = 0; x < L; ++x)
+      for (int y = 0; y < L; ++y)
+        for (int z = 0; z < L; ++z) {
+          const complex pm = Q[(x * L + y) * L + z];
+          const int nb[6][3] = {{x - 1, y, z}, {x + 1, y, z}, {x, y - 1, z}, {x, y + 1, z}, {x, y, z - 1}, {x, y, z + 1}};
+          for (const auto& n : nb)
+            acc += pm * std::conj(Q[(((n[0] + L) % L) * L + (n[1] + L) % L) * L + (n[2] + L) % L]);
+        }
+    acc /= 6.0 * L * L * L;
+    const double e = std::abs(correlator_over_fixed_distance_fast(Q, 1) - acc);
+    report("Polyakov correlator r^2 = 1 = mean over 6 axis neighbours", e < 1e-12 * std::max(1.0, std::abs(acc)), e);
+  }
+
+  // checkpoints (Lattice.cpp:810-847): round trip, missing and short files
+  {
+    std::array<Link_array, 2> saved{hot(0.4, 81), hot(0.4, 82)}, loaded{cold(), cold()};
+    const std::string path = "/tmp/sfh_dropin_selftest_ckpt.bin";
+    const bool ok_save = save_all_lattice_configs(saved, path);
+    const bool ok_load = load_all_lattice_configs(loaded, path);
+    report("checkpoint round trip is byte-exact", ok_save && ok_load && loaded == saved, ok_load ? 0.0 : 1.0);
+    std::array<Link_array, 3> three{cold(), cold(), cold()};
+    const bool short_fails = !load_all_lattice_configs(three, path);
+    const bool missing_fails = !load_all_lattice_configs(loaded, "/tmp/sfh_dropin_selftest_missing/none.bin");
+    report("short and missing checkpoints return false", short_fails && missing_fails, 0.0);
+    std::remove(path.c_str());
+  }
+
+  // resident links: a call after a link-changing call, or on an unchanged
+  // array, gives the same result as after a fresh upload
+  {
+    Link_array U = hot(0.3, 91);
+    const double a0 = compute_action(U, beta);
+    const double p0 = compute_avg_plaq(U);
+    U[5] += 1e-3;  // the caller changes one entry: the library must see it
+    const double p1 = compute_avg_plaq(U);
+    U[5] -= 1e-3;
+    const double p2 = compute_avg_plaq(U);
+    report("changed link array is re-uploaded", p1 != p0 && p2 == p0 && compute_action(U, beta) == a0,
+           std::abs(p1 - p0));
+  }
+
   std::printf("%s: %d failure(s)\n", failures ? "FAIL" : "PASS", failures);
   return failures ? 1 : 0;
 }
 
-int main() {
+// --fields SEED: host-side field generation only (no GPU): prints checksums
+// of two momentum fields drawn on the main thread and one drawn on a second
+// thread, for the reproducibility test (same seed, two processes).
+static int fields(unsigned long long seed) {
+  sfh_compat::Config cfg;
+  cfg.seed = seed;
+  cfg.verbose = false;
+  sfh_compat::configure(cfg);
+  auto sum = [](const std::vector<double>& v) {
+    double a = 0.0, b = 0.0;
+    for (size_t i = 0; i < v.size(); ++i) {
+      a += v[i];
+      b += v[i] * (double)(i % 97 + 1);
+    }
+    std::printf("%a %a %zu\n", a, b, v.size());
+  };
+  sum(generate_conjugate_field_configuration());
+  sum(generate_conjugate_field_configuration());
+  std::thread t([&] { sum(generate_conjugate_field_configuration()); });
+  t.join();
+  return 0;
+}
+
+int main(int argc, char** argv) {
   try {
+    if (argc == 3 && std::string(argv[1]) == "--fields") return fields(std::stoull(argv[2]));
     return run();
   } catch (const std::exception& e) {
     std::printf("FAIL: unexpected exception: %s\n", e.what());

@@ -27,5 +27,14 @@ struct Config {
 void configure(const Config& c);
 const Config& config();
 int volume();
+// Random streams: every field generator, heatbath sweep and Metropolis
+// uniform of a host thread draws from streams keyed by (Config::seed, the
+// thread's stream index, a per-thread counter), so a run with the same seed
+// gives the same fields in every process.  The stream index is the OpenMP
+// thread number when called from an OpenMP region (one stream per OpenMP
+// thread, as the reference's per-thread generators, rng.cpp:8-16), else the
+// order in which threads first draw; set_thread_stream(k) fixes it to k for
+// the calling thread and restarts its counters.
+void set_thread_stream(int k);
 
 }  // namespace sfh_compat

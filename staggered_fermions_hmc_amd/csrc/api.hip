@@ -24,10 +24,30 @@ int hip_check(sfh_ctx* c, hipError_t e, const char* what) {
   return set_err(c, e == hipErrorOutOfMemory ? SFH_E_NOMEM : SFH_E_HIP, m);
 }
 
+void drop_graphs(sfh_ctx* c) {
+  if (c->graphs.empty()) return;
+  hipStreamSynchronize(c->stream);
+  for (auto& kv : c->graphs) {
+    hipGraphExecDestroy(kv.second.second);
+    hipGraphDestroy(kv.second.first);
+  }
+  c->graphs.clear();
+}
+
+void reset_aux_state(sfh_ctx* c) {
+  CGState z{};
+  z.ctr = c->d_ctr + kCtrKinds * kCtrPerKind;
+  CGState* h = c->h_state;  // pinned staging; the copy completes before return
+  *h = z;
+  hipMemcpyAsync(&c->d_state[1], h, sizeof(CGState), hipMemcpyHostToDevice, c->stream);
+  hipStreamSynchronize(c->stream);
+}
+
 void* scratch(sfh_ctx* c, const std::string& key, size_t bytes) {
   auto it = c->scratch.find(key);
   if (it != c->scratch.end() && it->second.second >= bytes) return it->second.first;
   if (it != c->scratch.end()) {
+    drop_graphs(c);
     hipStreamSynchronize(c->stream);
     hipFree(it->second.first);
     c->scratch.erase(it);
@@ -209,6 +229,7 @@ const char* sfh_status_string(int s) {
     case SFH_E_EXP_QCUBED: return "exp: Im Tr Q^3 non-zero";
     case SFH_E_EXP_NEG_TRACE: return "exp: Tr Q^2 negative";
     case SFH_E_SAMPLER: return "heatbath sampler found no sample in 2^20 tries";
+    case SFH_E_IO: return "file could not be opened, read or written completely";
     default: return "unknown status";
   }
 }
@@ -228,6 +249,8 @@ int sfh_create(int device, sfh_ctx** out) {
   ok = ok && hipHostMalloc(&c->h_state, 3 * sizeof(CGState), hipHostMallocDefault) == hipSuccess;
   ok = ok && hipHostMalloc(&c->h_scalar, 8 * sizeof(double), hipHostMallocDefault) == hipSuccess;
   ok = ok && hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess;
+  ok = ok && hipEventCreateWithFlags(&c->chunk_ev[0], hipEventDisableTiming) == hipSuccess &&
+       hipEventCreateWithFlags(&c->chunk_ev[1], hipEventDisableTiming) == hipSuccess;
   ok = ok && hipMalloc(&c->d_ctr, 2 * kCtrBytes) == hipSuccess;
   if (ok) {
     CGState z[4] = {};
@@ -265,6 +288,8 @@ int sfh_destroy(sfh_ctx* c) {
   if (c->h_scalar) hipHostFree(c->h_scalar);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
+  for (hipEvent_t e : c->chunk_ev)
+    if (e) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return SFH_OK;
@@ -371,10 +396,12 @@ int sfh_time_hop(sfh_ctx* c, const sfh_gauge* g, int precision, int with_dot, in
   }
   // a live solve state so the reduction runs exactly as in the CG
   CGState* st = &c->d_state[1];
-  CGState z{};
-  z.rsq = 1.0;
-  z.ctr = c->d_ctr + kCtrKinds * kCtrPerKind;
-  hipMemcpyAsync(st, &z, sizeof(CGState), hipMemcpyHostToDevice, c->stream);
+  CGState* z = c->h_state;  // pinned staging
+  *z = CGState{};
+  z->rsq = 1.0;
+  z->ctr = c->d_ctr + kCtrKinds * kCtrPerKind;
+  hipMemcpyAsync(st, z, sizeof(CGState), hipMemcpyHostToDevice, c->stream);
+  if (int e = hip_check(c, hipStreamSynchronize(c->stream), "hop timing state")) return e;
   double* part = c->d_partials + 4 * (size_t)kPartialSlot;
   auto launch = [&]() {
     if (precision == 0)
@@ -395,6 +422,7 @@ int sfh_time_hop(sfh_ctx* c, const sfh_gauge* g, int precision, int with_dot, in
   float ms = 0.f;
   hipEventElapsedTime(&ms, c->ev0, c->ev1);
   *us = 1e3 * ms / n;
+  reset_aux_state(c);
   return SFH_OK;
 }
 
@@ -452,6 +480,7 @@ int sfh_gauge_create(sfh_ctx* c, const int dims[4], sfh_gauge** out) {
 int sfh_gauge_destroy(sfh_ctx* c, sfh_gauge* g) {
   if (!c || !g) return SFH_E_INVALID;
   Guard gd(c);
+  drop_graphs(c);
   hipStreamSynchronize(c->stream);
   hipFree(g->U);
   if (g->lp.u32) hipFree(g->lp.u32);
@@ -556,6 +585,7 @@ int sfh_field_create(sfh_ctx* c, const sfh_gauge* g, sfh_field** out) {
 int sfh_field_destroy(sfh_ctx* c, sfh_field* f) {
   if (!c || !f) return SFH_E_INVALID;
   Guard gd(c);
+  drop_graphs(c);
   hipStreamSynchronize(c->stream);
   hipFree(f->d);
   delete f;
@@ -750,6 +780,47 @@ int sfh_polyakov_grid(sfh_ctx* c, const sfh_gauge* g, double* out) {
   if (!c || !g || !out) return SFH_E_INVALID;
   Guard gd(c);
   return polyakov_grid(c, g->g, g->U, out);
+}
+
+int sfh_polyakov_correlators(sfh_ctx* c, const sfh_gauge* g, int r2_first, int count, double* out) {
+  if (!c || !g || !out || r2_first < 1 || count < 1 || count > 65535) return SFH_E_INVALID;
+  Guard gd(c);
+  return polyakov_correlators(c, g->g, g->U, r2_first, count, out);
+}
+
+int sfh_polyakov_correlators_grid(sfh_ctx* c, const int spatial[3], const double* grid, int r2_first, int count,
+                                  double* out) {
+  if (!c || !spatial || !grid || !out || r2_first < 1 || count < 1 || count > 65535) return SFH_E_INVALID;
+  long ns = 1;
+  for (int k = 0; k < 3; ++k) {
+    if (spatial[k] < 1) return SFH_E_INVALID;
+    ns *= spatial[k];
+  }
+  if (ns > (1L << 24)) return SFH_E_INVALID;
+  Guard gd(c);
+  return polyakov_correlators_grid(c, spatial, grid, r2_first, count, out);
+}
+
+int sfh_links_save(const double* const* arrays, int n, size_t doubles_each, const char* path) {
+  if (!arrays || n < 1 || !path) return SFH_E_INVALID;
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return SFH_E_IO;
+  int rc = SFH_OK;
+  for (int k = 0; k < n && rc == SFH_OK; ++k)
+    if (!arrays[k] || std::fwrite(arrays[k], sizeof(double), doubles_each, f) != doubles_each) rc = SFH_E_IO;
+  if (std::fclose(f) != 0) rc = SFH_E_IO;
+  return rc;
+}
+
+int sfh_links_load(double* const* arrays, int n, size_t doubles_each, const char* path) {
+  if (!arrays || n < 1 || !path) return SFH_E_INVALID;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return SFH_E_IO;
+  int rc = SFH_OK;
+  for (int k = 0; k < n && rc == SFH_OK; ++k)
+    if (!arrays[k] || std::fread(arrays[k], sizeof(double), doubles_each, f) != doubles_each) rc = SFH_E_IO;
+  std::fclose(f);
+  return rc;
 }
 
 int sfh_heatbath(sfh_ctx* c, sfh_gauge* g, double beta, double c11, double c12, unsigned long long seed,

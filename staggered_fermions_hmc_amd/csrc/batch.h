@@ -62,6 +62,7 @@ template <class T> class Engine {
   struct Done {
     long long ticket;
     int slot;
+    int sys;
     sfh_cg_stats stats;
     CGState state;
   };
@@ -76,10 +77,24 @@ template <class T> class Engine {
   Engine(const Engine&) = delete;
   Engine& operator=(const Engine&) = delete;
 
-  // Registers a system (device links, mass) with a pool of `pool` slots of
-  // solve memory; returns its id, or minus a status.
-  int add_system(const sfh_gauge* gauge, double mass, int pool);
+  // Registers a system (device links, mass) that may hold up to `cap`
+  // solves at once, with `prealloc` slots of solve memory allocated now (the
+  // rest on demand by grow()); returns its id, or minus a status.
+  int add_system(const sfh_gauge* gauge, double mass, int cap, int prealloc);
+  int add_system(const sfh_gauge* gauge, double mass, int pool) { return add_system(gauge, mass, pool, pool); }
+  // One more slot for system sys (a spare slot of a removed system if one is
+  // large enough, else a new allocation): its id, or minus a status.
+  int grow(int sys);
+  // New links (same lattice) or mass for a system with no solve in flight:
+  // the SU(3) check and low-precision copy are redone.
+  int update_system(int sys, const sfh_gauge* gauge, double mass);
+  // Drops a system with no solve in flight; its slots become spares.
+  int remove_system(int sys);
   int free_slots(int sys) const;
+  int slots_of(int sys) const;
+  int cap_of(int sys) const;
+  // A spare slot large enough for a lattice of V sites exists.
+  bool has_spare(int V) const;
   // Starts a solve of (D D^dag) x = b on system sys in a free slot: returns
   // the slot or -1 when the pool is full.  b and x are device spinors.
   int admit(int sys, const double2* b, double2* x, long long ticket);
@@ -102,12 +117,19 @@ template <class T> class Engine {
 
  private:
   struct SysH {
-    const sfh_gauge* gauge;
-    double mass;
-    const typename Prec<T>::L* U;
+    const sfh_gauge* gauge = nullptr;  // not dereferenced once removed
+    Geom g{};
+    double mass = 0.0;
+    const typename Prec<T>::L* U = nullptr;
+    const double2* U64 = nullptr;
+    int cap = 0;
+    bool removed = false;
+    bool su3_lp = false, su3_64 = false;  // links pass the two-row checks
     std::vector<int> pool;
   };
   struct SlotH {
+    void* mem = nullptr;  // this slot's device memory (owned_)
+    size_t capacity = 0;
     int sys = 0;
     bool used = false, finished = false;
     int status = SFH_OK, rupd = 0;
@@ -122,6 +144,8 @@ template <class T> class Engine {
     BSlot<T> dev{};
   };
   int rebuild();
+  int bind_system(SysH& s, const sfh_gauge* gauge, double mass);
+  int layout_slot(SlotH& h, int V);
   void launch_tail();
   void hop_map(int threads, int tile, std::vector<int4>& out) const;
 
@@ -139,6 +163,7 @@ template <class T> class Engine {
   bool r12_ = true;
   bool r12_64_ = true;
   std::vector<SlotH> slots_;
+  std::vector<int> spare_;  // slots of removed systems, kept for reuse
   std::vector<int> done_;
   std::vector<void*> owned_;
   size_t bytes_ = 0;

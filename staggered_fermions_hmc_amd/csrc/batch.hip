@@ -411,70 +411,165 @@ Engine<T>::~Engine() {
 }
 
 template <class T>
-int Engine<T>::add_system(const sfh_gauge* gauge, double mass, int pool) {
-  if (pool < 1 || pool > kMaxRhs) return set_err(c_, SFH_E_INVALID, "slot pool must be 1..8");
+int Engine<T>::bind_system(SysH& s, const sfh_gauge* gauge, double mass) {
   sfh_gauge* gk = const_cast<sfh_gauge*>(gauge);
-  SysH s;
+  if (int e = ensure_su3(c_, gk)) return e;
   s.gauge = gauge;
+  s.g = gauge->g;
+  s.U64 = gauge->U;
   s.mass = mass;
-  if (int e = ensure_su3(c_, gk)) return -e;
-  r12_64_ = r12_64_ && gk->f64_su3;
+  s.su3_64 = gk->f64_su3;
   if (kMixed) {
-    if (int e = ensure_lp(c_, gk, SFH_PREC_F32)) return -e;
+    if (int e = ensure_lp(c_, gk, SFH_PREC_F32)) return e;
     s.U = reinterpret_cast<const typename Prec<T>::L*>(gk->lp.u32);
-    r12_ = r12_ && gk->lp_su3;
+    s.su3_lp = gk->lp_su3;
   } else {
     s.U = reinterpret_cast<const typename Prec<T>::L*>(gk->U);
-    r12_ = r12_64_;
+    s.su3_lp = gk->f64_su3;
   }
-  const int V = gauge->g.V;
-  const size_t sb = slot_bytes(V);
-  char* base = nullptr;
-  if (hipMalloc(&base, sb * pool) != hipSuccess) return -set_err(c_, SFH_E_NOMEM, "slot pool allocation");
-  owned_.push_back(base);
-  // reduction counters must start at zero
-  if (hipMemsetAsync(base, 0, sb * pool, c_->stream) != hipSuccess || hipStreamSynchronize(c_->stream) != hipSuccess)
-    return -set_err(c_, SFH_E_HIP, "slot pool clear");
-  bytes_ += sb * pool;
+  return hip_check(c_, hipStreamSynchronize(c_->stream), "system links");
+}
+
+template <class T>
+int Engine<T>::add_system(const sfh_gauge* gauge, double mass, int cap, int prealloc) {
+  if (cap < 1 || cap > kMaxRhs || prealloc < 0 || prealloc > cap)
+    return -set_err(c_, SFH_E_INVALID, "slot pool must be 1..8");
+  SysH s;
+  s.cap = cap;
+  if (int e = bind_system(s, gauge, mass)) return -e;
+  sys_.push_back(s);
+  const int id = (int)sys_.size() - 1;
+  for (int k = 0; k < prealloc; ++k) {
+    const int slot = grow(id);
+    if (slot < 0) return slot;
+  }
+  dirty_ = true;
+  return id;
+}
+
+template <class T>
+int Engine<T>::update_system(int sys, const sfh_gauge* gauge, double mass) {
+  if (sys < 0 || sys >= (int)sys_.size() || sys_[sys].removed) return set_err(c_, SFH_E_INVALID, "no such system");
+  for (int k : sys_[sys].pool)
+    if (slots_[k].used) return set_err(c_, SFH_E_INVALID, "system has solves in flight");
+  for (int k = 0; k < 4; ++k)
+    if (gauge->g.L[k] != sys_[sys].g.L[k]) return set_err(c_, SFH_E_INVALID, "system lattice changed shape");
+  // the links may have changed in place: their SU(3) check and low-precision
+  // copy are measured again
+  sfh_gauge* gk = const_cast<sfh_gauge*>(gauge);
+  gk->su3_known = false;
+  gk->lp32_valid = false;
+  gk->lp16_valid = false;
+  if (int e = bind_system(sys_[sys], gauge, mass)) return e;
+  dirty_ = true;
+  return SFH_OK;
+}
+
+template <class T>
+int Engine<T>::remove_system(int sys) {
+  if (sys < 0 || sys >= (int)sys_.size() || sys_[sys].removed) return set_err(c_, SFH_E_INVALID, "no such system");
+  for (int k : sys_[sys].pool)
+    if (slots_[k].used) return set_err(c_, SFH_E_INVALID, "system has solves in flight");
+  for (int k : sys_[sys].pool) spare_.push_back(k);
+  sys_[sys].pool.clear();
+  sys_[sys].removed = true;
+  dirty_ = true;
+  return SFH_OK;
+}
+
+// Carves one slot's arrays out of its memory for lattice volume V; the
+// arrays are exactly those counted by slot_bytes(V).  The reduction counters
+// must start at zero.
+template <class T>
+int Engine<T>::layout_slot(SlotH& h, int V) {
+  char* p = static_cast<char*>(h.mem);
+  auto take = [&](size_t bytes) {
+    char* q = p;
+    p += align256(bytes);
+    return q;
+  };
+  BSlot<T>& L = h.dev;
+  L = BSlot<T>{};
+  L.V = V;
   const size_t n = 3 * (size_t)V;
-  for (int k = 0; k < pool; ++k) {
-    char* p = base + sb * k;
-    auto take = [&](size_t bytes) {
-      char* q = p;
-      p += align256(bytes);
-      return q;
-    };
-    SlotH h;
-    h.sys = (int)sys_.size();
-    BSlot<T>& L = h.dev;
-    L.V = V;
-    // FP64 solves iterate directly in the caller's x (set at admission); the
-    // mixed ones keep an FP32 x and an FP64 residual of their own.  The
-    // arrays taken here are exactly those counted by slot_bytes().
-    if (kMixed) L.x = reinterpret_cast<S*>(take(n * sizeof(S)));
-    L.r = reinterpret_cast<S*>(take(n * sizeof(S)));
-    L.p = reinterpret_cast<S*>(take(n * sizeof(S)));
-    L.t = reinterpret_cast<S*>(take(n * sizeof(S)));
-    L.v = reinterpret_cast<S*>(take(n * sizeof(S)));
-    if (kMixed) {
-      L.r64 = reinterpret_cast<double2*>(take(n * sizeof(double2)));
-      L.t64 = reinterpret_cast<double2*>(take(n * sizeof(double2)));
-      L.ax = reinterpret_cast<double2*>(take(n * sizeof(double2)));
-      h.r64 = L.r64;
+  // FP64 solves iterate directly in the caller's x (set at admission); the
+  // mixed ones keep an FP32 x and an FP64 residual of their own
+  if (kMixed) L.x = reinterpret_cast<S*>(take(n * sizeof(S)));
+  L.r = reinterpret_cast<S*>(take(n * sizeof(S)));
+  L.p = reinterpret_cast<S*>(take(n * sizeof(S)));
+  L.t = reinterpret_cast<S*>(take(n * sizeof(S)));
+  L.v = reinterpret_cast<S*>(take(n * sizeof(S)));
+  h.r64 = nullptr;
+  if (kMixed) {
+    L.r64 = reinterpret_cast<double2*>(take(n * sizeof(double2)));
+    L.t64 = reinterpret_cast<double2*>(take(n * sizeof(double2)));
+    L.ax = reinterpret_cast<double2*>(take(n * sizeof(double2)));
+    h.r64 = L.r64;
+  }
+  L.st = reinterpret_cast<CGState*>(take(sizeof(CGState)));
+  h.ctr = reinterpret_cast<unsigned*>(take(kCtrBytes));
+  L.part = reinterpret_cast<double*>(p);
+  const int nbh = (V / 2 + 127) / 128, nbb = (V + kBlas - 1) / kBlas;
+  const size_t used = (size_t)(p - static_cast<char*>(h.mem)) +
+                      align256(4 * (std::max(nbh, nbb) + kRedGroups) * sizeof(double));
+  if (used > h.capacity) return set_err(c_, SFH_E_INVALID, "slot layout exceeds its allocation");
+  if (hipMemsetAsync(h.ctr, 0, kCtrBytes, c_->stream) != hipSuccess ||
+      hipStreamSynchronize(c_->stream) != hipSuccess)
+    return set_err(c_, SFH_E_HIP, "slot counters clear");
+  return SFH_OK;
+}
+
+template <class T>
+int Engine<T>::grow(int sys) {
+  SysH& s = sys_[sys];
+  if ((int)s.pool.size() >= s.cap) return -set_err(c_, SFH_E_INVALID, "slot pool is at its cap");
+  const int V = s.g.V;
+  const size_t sb = slot_bytes(V);
+  // a spare slot (freed by a removed system) large enough for V, else new memory
+  int id = -1;
+  for (size_t j = 0; j < spare_.size(); ++j)
+    if (slots_[spare_[j]].capacity >= sb) {
+      id = spare_[j];
+      spare_.erase(spare_.begin() + (long)j);
+      break;
     }
-    L.st = reinterpret_cast<CGState*>(take(sizeof(CGState)));
-    h.ctr = reinterpret_cast<unsigned*>(take(kCtrBytes));
-    L.part = reinterpret_cast<double*>(p);
-    const int nbh = (V / 2 + 127) / 128, nbb = (V + kBlas - 1) / kBlas;
-    const size_t used =
-        (size_t)(p - (base + sb * k)) + align256(4 * (std::max(nbh, nbb) + kRedGroups) * sizeof(double));
-    if (used > sb) return -set_err(c_, SFH_E_INVALID, "slot layout exceeds its allocation");
-    s.pool.push_back((int)slots_.size());
+  if (id < 0) {
+    void* mem = nullptr;
+    if (hipMalloc(&mem, sb) != hipSuccess) return -set_err(c_, SFH_E_NOMEM, "slot allocation");
+    owned_.push_back(mem);
+    bytes_ += sb;
+    SlotH h;
+    h.mem = mem;
+    h.capacity = sb;
+    id = (int)slots_.size();
     slots_.push_back(h);
   }
-  sys_.push_back(s);
+  SlotH& h = slots_[id];
+  h.sys = sys;
+  h.used = false;
+  h.finished = false;
+  if (int e = layout_slot(h, V)) return -e;
+  s.pool.push_back(id);
   dirty_ = true;
-  return (int)sys_.size() - 1;
+  return id;
+}
+
+template <class T>
+int Engine<T>::slots_of(int sys) const {
+  return (int)sys_[sys].pool.size();
+}
+
+template <class T>
+int Engine<T>::cap_of(int sys) const {
+  return sys_[sys].cap;
+}
+
+template <class T>
+bool Engine<T>::has_spare(int V) const {
+  const size_t sb = slot_bytes(V);
+  for (int k : spare_)
+    if (slots_[k].capacity >= sb) return true;
+  return false;
 }
 
 template <class T>
@@ -568,11 +663,17 @@ int Engine<T>::rebuild() {
   for (size_t k = 0; k < slots_.size(); ++k) hl[k] = slots_[k].dev;
   mapped_.clear();
   hsys_.clear();
+  r12_ = r12_64_ = true;
+  for (const SysH& sh : sys_) {
+    if (sh.removed) continue;
+    r12_ = r12_ && sh.su3_lp;
+    r12_64_ = r12_64_ && sh.su3_64;
+  }
   for (size_t s = 0; s < sys_.size(); ++s) {
     BSys<T> d{};
-    d.g = sys_[s].gauge->g;
+    d.g = sys_[s].g;
     d.U = sys_[s].U;
-    d.U64 = sys_[s].gauge->U;
+    d.U64 = sys_[s].U64;
     d.mass = (R)sys_[s].mass;
     d.mass64 = sys_[s].mass;
     d.nb_hop = (d.g.Vh + 127) / 128;
@@ -725,6 +826,7 @@ std::vector<typename Engine<T>::Done> Engine<T>::take_done() {
     Done d;
     d.ticket = h.ticket;
     d.slot = k;
+    d.sys = h.sys;
     sfh_cg_stats& o = d.stats;
     std::memset(&o, 0, sizeof o);
     o.iterations = h.last.iter;

@@ -4,6 +4,9 @@
 // All FP64, one thread per link.  The staple kernels are FP64-VALU-bound
 // (about 85 SU(3) products per link); the fermion force and link update are
 // HBM-bound.
+#include <cmath>
+#include <vector>
+
 #include "internal.h"
 #include "reduce.h"
 #include "staples.h"
@@ -177,13 +180,140 @@ __global__ void k_polyakov(Geom g, const double2* __restrict__ U, double2* __res
   out[s] = make_double2(tr.re, tr.im);
 }
 
-int polyakov_grid(sfh_ctx* c, const Geom& g, const double2* U, double* out) {
+static double2* polyakov_dev(sfh_ctx* c, const Geom& g, const double2* U) {
   const int ns = g.L[0] * g.L[1] * g.L[2];
   double2* d = static_cast<double2*>(scratch(c, "polyakov", ns * sizeof(double2)));
+  if (d) hipLaunchKernelGGL(k_polyakov, dim3((ns + 127) / 128), dim3(128), 0, c->stream, g, U, d);
+  return d;
+}
+
+int polyakov_grid(sfh_ctx* c, const Geom& g, const double2* U, double* out) {
+  const int ns = g.L[0] * g.L[1] * g.L[2];
+  double2* d = polyakov_dev(c, g, U);
   if (!d) return SFH_E_NOMEM;
-  hipLaunchKernelGGL(k_polyakov, dim3((ns + 127) / 128), dim3(128), 0, c->stream, g, U, d);
   hipMemcpyAsync(out, d, ns * sizeof(double2), hipMemcpyDeviceToHost, c->stream);
   return hip_check(c, hipStreamSynchronize(c->stream), "polyakov");
+}
+
+// ------------------------------------------------------------------ Polyakov-loop correlators
+// C(r^2) = (1/N) sum_m sum_{d in D(r^2)} P(m) conj(P(m + d)), N = |grid| |D(r^2)|,
+// with D(r^2) the integer displacements of correlator_over_fixed_distance_fast
+// (Lattice.cpp:888-948): |d_k| <= min(floor(sqrt(r^2)), L_k / 2), d_k = +L_k / 2
+// left out (it is the same neighbour as -L_k / 2), d . d = r^2; C = 0 when D
+// is empty.  One lane per grid point m and one block row per r^2; each lane
+// sums its displacements in table order, blocks reduce in a fixed order, and a
+// second kernel sums the block partials of each r^2 in block order, so the
+// result does not depend on timing.
+constexpr int kCorrB = 256;
+
+__global__ __launch_bounds__(kCorrB) void k_poly_corr(int L0, int L1, int L2, const double2* __restrict__ P,
+                                                     const int4* __restrict__ disp, const int* __restrict__ off,
+                                                     double* __restrict__ part) {
+  const int r = blockIdx.y;
+  const int ns = L0 * L1 * L2;
+  const int m = blockIdx.x * kCorrB + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (m < ns) {
+    const int x = m / (L1 * L2), y = (m / L2) % L1, z = m % L2;
+    const double2 pm = P[m];
+    for (int k = off[r]; k < off[r + 1]; ++k) {
+      const int4 d = disp[k];
+      int nx = x + d.x, ny = y + d.y, nz = z + d.z;
+      nx += nx < 0 ? L0 : (nx >= L0 ? -L0 : 0);
+      ny += ny < 0 ? L1 : (ny >= L1 ? -L1 : 0);
+      nz += nz < 0 ? L2 : (nz >= L2 ? -L2 : 0);
+      const double2 pn = P[(nx * L1 + ny) * L2 + nz];
+      v[0] += pm.x * pn.x + pm.y * pn.y;  // Re P(m) conj(P(n))
+      v[1] += pm.y * pn.x - pm.x * pn.y;  // Im
+    }
+  }
+  block_sum_n<2>(v);
+  if (threadIdx.x == 0) {
+    part[2 * ((size_t)r * gridDim.x + blockIdx.x)] = v[0];
+    part[2 * ((size_t)r * gridDim.x + blockIdx.x) + 1] = v[1];
+  }
+}
+
+__global__ __launch_bounds__(kCorrB) void k_poly_corr_fin(const double* __restrict__ part, int nblk,
+                                                         const int* __restrict__ off, int ns,
+                                                         double* __restrict__ out) {
+  const int r = blockIdx.x;
+  double v[2] = {0.0, 0.0};
+  for (int b = threadIdx.x; b < nblk; b += kCorrB) {
+    v[0] += part[2 * ((size_t)r * nblk + b)];
+    v[1] += part[2 * ((size_t)r * nblk + b) + 1];
+  }
+  block_sum_n<2>(v);
+  if (threadIdx.x == 0) {
+    const double n = (double)ns * (double)(off[r + 1] - off[r]);
+    out[2 * r] = n > 0 ? v[0] / n : 0.0;
+    out[2 * r + 1] = n > 0 ? v[1] / n : 0.0;
+  }
+}
+
+// Displacement tables of r^2 = r2_first .. r2_first + count - 1 in the
+// reference's loop order (dx, then dy, then dz ascending).
+static void corr_tables(const int L[3], int r2_first, int count, std::vector<int4>& disp, std::vector<int>& off) {
+  disp.clear();
+  off.assign(1, 0);
+  for (int r2 = r2_first; r2 < r2_first + count; ++r2) {
+    int md[3];
+    for (int k = 0; k < 3; ++k) {
+      int s = (int)std::sqrt((double)r2);
+      while ((s + 1) * (s + 1) <= r2) ++s;
+      while (s * s > r2) --s;
+      md[k] = s < L[k] / 2 ? s : L[k] / 2;
+    }
+    for (int dx = -md[0]; dx <= md[0]; ++dx) {
+      if (L[0] % 2 == 0 && dx == L[0] / 2) continue;
+      for (int dy = -md[1]; dy <= md[1]; ++dy) {
+        if (L[1] % 2 == 0 && dy == L[1] / 2) continue;
+        for (int dz = -md[2]; dz <= md[2]; ++dz) {
+          if (L[2] % 2 == 0 && dz == L[2] / 2) continue;
+          if (dx * dx + dy * dy + dz * dz == r2) disp.push_back(make_int4(dx, dy, dz, 0));
+        }
+      }
+    }
+    off.push_back((int)disp.size());
+  }
+}
+
+// Correlators of the device grid P (L[0] x L[1] x L[2] complex) into out[2 k],
+// out[2 k + 1] = C(r2_first + k), k < count (host memory).
+static int poly_corr_dev(sfh_ctx* c, const int L[3], const double2* P, int r2_first, int count, double* out) {
+  std::vector<int4> disp;
+  std::vector<int> off;
+  corr_tables(L, r2_first, count, disp, off);
+  const int ns = L[0] * L[1] * L[2];
+  const int nblk = (ns + kCorrB - 1) / kCorrB;
+  int4* d_disp = static_cast<int4*>(scratch(c, "pc_disp", (disp.size() + 1) * sizeof(int4)));
+  int* d_off = static_cast<int*>(scratch(c, "pc_off", off.size() * sizeof(int)));
+  double* d_part = static_cast<double*>(scratch(c, "pc_part", 2 * (size_t)count * nblk * sizeof(double)));
+  double* d_out = static_cast<double*>(scratch(c, "pc_out", 2 * (size_t)count * sizeof(double)));
+  if (!d_disp || !d_off || !d_part || !d_out) return SFH_E_NOMEM;
+  if (!disp.empty())
+    hipMemcpyAsync(d_disp, disp.data(), disp.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream);
+  hipMemcpyAsync(d_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, c->stream);
+  hipLaunchKernelGGL(k_poly_corr, dim3(nblk, count), dim3(kCorrB), 0, c->stream, L[0], L[1], L[2], P, d_disp, d_off,
+                     d_part);
+  hipLaunchKernelGGL(k_poly_corr_fin, dim3(count), dim3(kCorrB), 0, c->stream, d_part, nblk, d_off, ns, d_out);
+  hipMemcpyAsync(out, d_out, 2 * (size_t)count * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  // the host tables stay alive until the stream has drained
+  return hip_check(c, hipStreamSynchronize(c->stream), "polyakov correlators");
+}
+
+int polyakov_correlators(sfh_ctx* c, const Geom& g, const double2* U, int r2_first, int count, double* out) {
+  double2* P = polyakov_dev(c, g, U);
+  if (!P) return SFH_E_NOMEM;
+  return poly_corr_dev(c, g.L, P, r2_first, count, out);
+}
+
+int polyakov_correlators_grid(sfh_ctx* c, const int L[3], const double* grid, int r2_first, int count, double* out) {
+  const int ns = L[0] * L[1] * L[2];
+  double2* P = static_cast<double2*>(scratch(c, "polyakov", ns * sizeof(double2)));
+  if (!P) return SFH_E_NOMEM;
+  hipMemcpyAsync(P, grid, ns * sizeof(double2), hipMemcpyHostToDevice, c->stream);
+  return poly_corr_dev(c, L, P, r2_first, count, out);
 }
 
 // ------------------------------------------------------------------ exponential and link update

@@ -103,6 +103,7 @@ struct sfh_ctx {
   struct Pending { std::string name; hipEvent_t a, b; };
   std::vector<Pending> pending;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;  // whole-solve timing
+  hipEvent_t chunk_ev[2] = {nullptr, nullptr};  // CG chunks in flight (no timing)
   // captured CG chunks, keyed by the buffers and parameters they replay
   std::map<std::string, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
 };
@@ -119,6 +120,12 @@ constexpr int kPartialSlot = (1 << 18) + 4 * kRedGroups;
 int set_err(sfh_ctx* c, int code, const std::string& msg);
 int hip_check(sfh_ctx* c, hipError_t e, const char* what);
 void* scratch(sfh_ctx* c, const std::string& key, size_t bytes);
+// Destroys every cached CG chunk graph (after the stream drains): called
+// whenever device memory that a graph may reference is freed, so no cached
+// graph ever holds a dangling pointer.
+void drop_graphs(sfh_ctx* c);
+// Returns d_state[1] (reductions and kernel timings) to an idle state.
+void reset_aux_state(sfh_ctx* c);
 
 // Profiling: brackets the launches of its scope with a pair of events on the
 // context stream when profiling is on (no synchronisation); the durations are
@@ -196,6 +203,10 @@ double tr_p2(sfh_ctx* c, const Geom& g, const double2* P);
 int link_update(sfh_ctx* c, const Geom& g, double2* U, const double2* P, double eps, int check);
 int exp_iq_one(sfh_ctx* c, const double* Q18, double* out18);
 int polyakov_grid(sfh_ctx* c, const Geom& g, const double2* U, double* out);
+// Polyakov-loop correlators C(r^2) for r^2 = r2_first .. r2_first + count - 1
+// into out (host, count complex), from the device links or a host grid.
+int polyakov_correlators(sfh_ctx* c, const Geom& g, const double2* U, int r2_first, int count, double* out);
+int polyakov_correlators_grid(sfh_ctx* c, const int L[3], const double* grid, int r2_first, int count, double* out);
 
 // ------------------------------------------------------------------ heatbath (heatbath.hip)
 bool heatbath_dims_ok(const Geom& g);

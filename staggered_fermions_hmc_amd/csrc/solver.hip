@@ -169,21 +169,22 @@ std::string graph_key(const char* kind, const Geom& g, std::initializer_list<con
 // state through out.
 template <class F> int run_chunks(sfh_ctx* c, F issue_chunk, int per_chunk, const CGState& init, CGState& out) {
   CGState* ring[2] = {c->h_state + 1, c->h_state + 2};
-  hipEvent_t ev[2];
-  hipEventCreateWithFlags(&ev[0], hipEventDisableTiming);
-  hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
+  hipEvent_t* ev = c->chunk_ev;  // created with the context, reused by every solve
   long next = 0, head = 0;  // next chunk to issue, oldest chunk in flight
   int rc = SFH_OK;
   auto issue = [&]() {
     const int slot = (int)(next & 1);
     if (int e = issue_chunk()) rc = e;
     state_out(c->stream, c->d_state, ring[slot]);
-    hipEventRecord(ev[slot], c->stream);
+    if (!rc) rc = hip_check(c, hipEventRecord(ev[slot], c->stream), "CG chunk event");
     ++next;
   };
   const double lr0 = std::log(init.rsq > 0 ? init.rsq : 1.0);
   bool hold = false;
-  long polls = 0;
+  // every chunk advances the iteration count, runs a reliable update or
+  // stops the solve; a state that shows none of these for several polls
+  // means the kernels are not running at all
+  int last_iter = init.iter, last_rupd = init.rupd, stalled = 0;
   for (;;) {
     while (!rc && (next == head || (!hold && next - head < 2))) issue();
     if (rc) break;
@@ -193,11 +194,15 @@ template <class F> int run_chunks(sfh_ctx* c, F issue_chunk, int per_chunk, cons
     const CGState s = *ring[head & 1];
     ++head;
     if (s.done) break;
-    // every iteration advances the device counter or stops the solve, so the
-    // budget is exceeded only if kernels are not running at all
-    if (++polls * per_chunk > (long)s.max_iter + 4L * per_chunk + 64) {
-      rc = set_err(c, SFH_E_HIP, "CG state did not advance");
-      break;
+    if (s.iter == last_iter && s.rupd == last_rupd) {
+      if (++stalled >= 4) {
+        rc = set_err(c, SFH_E_HIP, "CG state did not advance");
+        break;
+      }
+    } else {
+      stalled = 0;
+      last_iter = s.iter;
+      last_rupd = s.rupd;
     }
     // iterations still needed at the mean convergence rate so far
     hold = false;
@@ -206,11 +211,14 @@ template <class F> int run_chunks(sfh_ctx* c, F issue_chunk, int per_chunk, cons
       if (rate > 0) hold = std::log(s.rsq / s.target) / rate < per_chunk - 0.5;
     }
   }
-  hipStreamSynchronize(c->stream);
-  hipMemcpy(&out, c->d_state, sizeof(CGState), hipMemcpyDeviceToHost);
-  hipEventDestroy(ev[0]);
-  hipEventDestroy(ev[1]);
-  return rc ? rc : hip_check(c, hipGetLastError(), "CG launches");
+  const int rs = hip_check(c, hipStreamSynchronize(c->stream), "CG chunks");
+  // the final state through the pinned ring, on the context stream
+  state_out(c->stream, c->d_state, ring[0]);
+  const int rf = hip_check(c, hipStreamSynchronize(c->stream), "CG state");
+  out = *ring[0];
+  if (rc) return rc;
+  if (rs) return rs;
+  return rf ? rf : hip_check(c, hipGetLastError(), "CG launches");
 }
 
 // Form of the mixed iteration per precision (1 f32, 2 f16): 0 fused kernel
@@ -327,11 +335,39 @@ static int cg_f64(sfh_ctx* c, const sfh_gauge* gauge, const sfh_cg_params* prm, 
     const std::string key = graph_key("f64", g, {gauge->U, x, b.r, b.p, b.t, b.v, st, part}, prm->mass,
                                       get_hop_variant(0, g) * 2 + (gauge->f64_su3 ? 1 : 0));
     if (int e = run_chunks(c, [&]() { return launch_chunk(c, key, direct); }, kChunk, s, fin)) return e;
+    // Stop on the FP64 true residual (compute_true_residual >= 2): once the
+    // iterated residual meets the target, r = phi - D D^dag x is formed and,
+    // while it misses the target, the recurrence restarts from it (p = r)
+    // with the iterations counted on.  The reference stops on the iterated
+    // residual alone (Fermions.cpp:594-600), which is mode 0 / 1.
+    for (int restarts = 0; prm->compute_true_residual >= 2 && fin.status == SFH_OK; ++restarts) {
+      if (int e = apply_ddag_dev(c, gauge, prm->mass, x, b.v)) return e;
+      sub_f64(c->stream, b.r, phi, b.v, n);
+      const double rr = dev_dot(c, b.r, b.r, n);
+      if (!std::isfinite(rr)) return set_err(c, SFH_E_NONFINITE_BETA, "CG produced non-finite beta.");
+      fin.rsq_true = rr;
+      fin.rupd += 1;
+      if (rr < s.target) break;
+      if (fin.iter >= prm->max_iter || restarts >= 64) {
+        fin.status = SFH_E_NOCONV;
+        fin.rsq = rr;
+        break;
+      }
+      hipMemcpyAsync(b.p, b.r, n * sizeof(double2), hipMemcpyDeviceToDevice, c->stream);
+      CGState r0 = fin;
+      r0.rsq = rr;
+      r0.alpha = r0.beta = 0.0;
+      r0.done = 0;
+      if (int e = upload_state(c, r0)) return e;
+      if (int e = run_chunks(c, [&]() { return launch_chunk(c, key, direct); }, kChunk, r0, fin)) return e;
+    }
   }
   if (stats) {
     stats->iterations = fin.iter;
+    stats->reliable_updates = prm->compute_true_residual >= 2 ? fin.rupd : 0;
     stats->rsq = fin.rsq;
-    stats->hbm_bytes = (double)fin.iter * 1824.0 * g.V;
+    if (prm->compute_true_residual >= 2 && fin.rupd > 0) stats->true_rel_resid = std::sqrt(fin.rsq_true / phi_sq);
+    stats->hbm_bytes = (double)fin.iter * 1824.0 * g.V + fin.rupd * 1700.0 * g.V;
   }
   return map_status(c, fin, prm->max_iter);
 }
@@ -460,7 +496,7 @@ int cg_solve(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* prm, const doubl
     hipEventElapsedTime(&ms, c->ev0, c->ev1);
     stats->seconds = ms * 1e-3;
     stats->status = rc;
-    if (rc == SFH_OK && prm->precision == SFH_PREC_F64 && prm->compute_true_residual && phi_sq >= 1e-28) {
+    if (rc == SFH_OK && prm->precision == SFH_PREC_F64 && prm->compute_true_residual == 1 && phi_sq >= 1e-28) {
       double2* ax = buf<double2>(c, "cg_t", n);
       double2* rr = buf<double2>(c, "cg_r", n);
       if (ax && rr && apply_ddag_dev(c, gauge, prm->mass, x, ax) == SFH_OK) {
@@ -492,12 +528,14 @@ static int time_mixed_t(sfh_ctx* c, sfh_gauge* gauge, int n, double* us_a, doubl
   hipMemsetAsync(b.v, 0, (size_t)kSpinorElems<T> * g.V * sizeof(typename Prec<T>::S), c->stream);
   hipMemsetAsync(b.x, 0, n3 * sizeof(float2), c->stream);
   // alpha = beta = 0 keeps kernel A idempotent; target 0 and no budget keep B running
-  CGState z{};
-  z.rsq = 1.0;
-  z.max_iter = 0x7fffffff;
-  z.ctr = c->d_ctr + kCtrKinds * kCtrPerKind;
+  CGState* z = c->h_state;  // pinned staging
+  *z = CGState{};
+  z->rsq = 1.0;
+  z->max_iter = 0x7fffffff;
+  z->ctr = c->d_ctr + kCtrKinds * kCtrPerKind;
   CGState* st = &c->d_state[1];
-  hipMemcpyAsync(st, &z, sizeof(CGState), hipMemcpyHostToDevice, c->stream);
+  hipMemcpyAsync(st, z, sizeof(CGState), hipMemcpyHostToDevice, c->stream);
+  if (int e = hip_check(c, hipStreamSynchronize(c->stream), "CG kernel timing state")) return e;
   double* part = c->d_partials + 4 * (size_t)kPartialSlot;
   const float m = 0.1f;
   const auto* U = reinterpret_cast<const typename Prec<T>::L*>(half ? (const void*)gauge->lp.u16
@@ -517,8 +555,10 @@ static int time_mixed_t(sfh_ctx* c, sfh_gauge* gauge, int n, double* us_a, doubl
   // sees the same state; B changes alpha and beta or only leaves partials
   const bool defer = fused && mix_defers(g);
   CGState* st_out = defer ? &c->d_state[3] : st;
-  if (int e = timed([&]() { mix_step_a<T>(c, gauge, fused, b, m, st, st_out, part, 0); }, us_a)) return e;
-  return timed([&]() { mix_b<T>(c->stream, g, U, b, m, st, part, defer, gauge->lp_su3, 0); }, us_b);
+  int rc = timed([&]() { mix_step_a<T>(c, gauge, fused, b, m, st, st_out, part, 0); }, us_a);
+  if (!rc) rc = timed([&]() { mix_b<T>(c->stream, g, U, b, m, st, part, defer, gauge->lp_su3, 0); }, us_b);
+  reset_aux_state(c);
+  return rc;
 }
 
 void set_mix_form(int precision, int form) {

@@ -26,6 +26,16 @@ def _raise(status, msg):
     raise cls(status, msg)
 
 
+def _true_mode(true_residual):
+    """sfh_cg_params.compute_true_residual: False / 0 none, True / 1 report
+    the FP64 true residual (FP64 CG), "stop" / 2 also stop on it."""
+    if true_residual == "stop":
+        return 2
+    if isinstance(true_residual, bool) or true_residual is None:
+        return int(bool(true_residual))
+    return int(true_residual)
+
+
 class Context:
     def __init__(self, device=0):
         self._lib = _abi.lib()
@@ -138,7 +148,7 @@ class Gauge:
            true_residual=False):
         a = self._spinor(phi)
         x = np.empty_like(a)
-        prm = _abi.CGParams(mass, tol, max_iter, _abi.PREC[precision], reliable_delta, int(bool(true_residual)))
+        prm = _abi.CGParams(mass, tol, max_iter, _abi.PREC[precision], reliable_delta, _true_mode(true_residual))
         stats = _abi.CGStats()
         self.ctx.check(self.ctx._lib.sfh_cg(self.ctx.h, self.h, ctypes.byref(prm), ptr(a), ptr(x),
                                             ctypes.byref(stats)))
@@ -146,7 +156,7 @@ class Gauge:
 
     def cg_field(self, phi_field, x_field, mass, tol=1e-12, max_iter=50000, precision="f64", reliable_delta=0.1,
                  true_residual=False):
-        prm = _abi.CGParams(mass, tol, max_iter, _abi.PREC[precision], reliable_delta, int(bool(true_residual)))
+        prm = _abi.CGParams(mass, tol, max_iter, _abi.PREC[precision], reliable_delta, _true_mode(true_residual))
         stats = _abi.CGStats()
         self.ctx.check(self.ctx._lib.sfh_cg_field(self.ctx.h, self.h, ctypes.byref(prm), phi_field.h,
                                                   x_field.h, ctypes.byref(stats)))
@@ -209,6 +219,14 @@ class Gauge:
         """Tr prod_t U_3 at each spatial site, complex [(x*L1 + y)*L2 + z]."""
         out = np.empty(2 * self.dims[0] * self.dims[1] * self.dims[2])
         self.ctx.check(self.ctx._lib.sfh_polyakov_grid(self.ctx.h, self.h, ptr(out)))
+        return out.view(np.complex128)
+
+    def polyakov_correlators(self, r2_max, r2_first=1):
+        """C(r^2) for r^2 = r2_first .. r2_max from the device links (grid and
+        sums on the device): complex array of r2_max - r2_first + 1 values."""
+        n = int(r2_max) - int(r2_first) + 1
+        out = np.empty(2 * n)
+        self.ctx.check(self.ctx._lib.sfh_polyakov_correlators(self.ctx.h, self.h, int(r2_first), n, ptr(out)))
         return out.view(np.complex128)
 
     def heatbath(self, beta, c11, c12, seed, sweep0=0, nsweeps=1):
@@ -300,6 +318,17 @@ class Field:
             pass
 
 
+def polyakov_correlators_grid(ctx, spatial, grid, r2_max, r2_first=1):
+    """C(r^2), r^2 = r2_first .. r2_max, of a host grid (complex, L0*L1*L2)."""
+    g = np.ascontiguousarray(np.asarray(grid, dtype=np.complex128)).view(np.float64)
+    sp = np.ascontiguousarray(np.asarray(spatial, dtype=np.int32))
+    assert sp.shape == (3,) and g.size == 2 * int(np.prod(sp))
+    n = int(r2_max) - int(r2_first) + 1
+    out = np.empty(2 * n)
+    ctx.check(ctx._lib.sfh_polyakov_correlators_grid(ctx.h, ptr(sp), ptr(g), int(r2_first), n, ptr(out)))
+    return out.view(np.complex128)
+
+
 def save_links(ctx, gauges, path):
     """Checkpoint in the reference's byte format (Lattice.cpp:810-828)."""
     arr = (ctypes.c_void_p * len(gauges))(*[g.h.value for g in gauges])
@@ -377,6 +406,20 @@ class Service:
         self._check(self._lib.sfh_service_add_chain(self.h, gauge.h, float(mass), int(max_inflight),
                                                     ctypes.byref(cid)))
         return cid.value
+
+    def update_chain(self, chain, gauge, mass):
+        """New links (gauge, same lattice) or mass for a chain with nothing
+        queued or in flight."""
+        self._check(self._lib.sfh_service_update_chain(self.h, int(chain), gauge.h, float(mass)))
+
+    def remove_chain(self, chain):
+        self._check(self._lib.sfh_service_remove_chain(self.h, int(chain)))
+
+    def memory(self):
+        """(bytes of solve-slot memory, slots refused for memory so far)."""
+        b, n = ctypes.c_size_t(), ctypes.c_long()
+        self._check(self._lib.sfh_service_memory(self.h, ctypes.byref(b), ctypes.byref(n)))
+        return b.value, n.value
 
     def submit(self, chain, rhs, x):
         t = ctypes.c_longlong()

@@ -79,3 +79,21 @@ def test_context_without_gpu_reports_no_device():
 def test_invalid_dims_rejected():
     out = np.empty(3 * 2 * 2 * 2 * 72)
     assert _abi.lib().sfh_random_links(_abi.ptr(_abi.dims_arr((3, 2, 2, 2))), 0.1, 1, _abi.ptr(out)) == 1
+
+
+def test_dropin_fields_reproducible_across_processes():
+    """The drop-in's field generators draw from streams keyed by the
+    configured seed and a stable thread index, not by an address: two
+    processes with the same seed print the same fields, another seed other
+    fields, and a second thread its own stream (host-side generation, no GPU)."""
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "staggered_fermions_hmc_amd",
+                       "compat", "dropin_selftest")
+    if not os.path.exists(exe):
+        pytest.skip("drop-in self test not built (run __graft_entry__.build())")
+    run = lambda seed: subprocess.run([exe, "--fields", str(seed)], capture_output=True, text=True, timeout=60,
+                                      check=True).stdout.split("\n")
+    a, b, c = run(11), run(11), run(12)
+    assert a == b
+    assert a[0] != c[0] and a[1] != c[1] and a[2] != c[2]
+    assert len({a[0], a[1], a[2]}) == 3

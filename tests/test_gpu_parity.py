@@ -150,9 +150,16 @@ def test_cg_f64_matches_oracle(ctx, dims, m):
     xr, it, rsq, _ = oc.cg(dims, m, U, phi, tol=1e-10)
     assert abs(st["iterations"] - it) <= 1
     assert rel(x, xr) < 1e-9
-    assert st["true_rel_resid"] < 2e-10
+    r = phi - oc.apply_ddag(dims, m, U, x)
+    assert abs(st["true_rel_resid"] - np.sqrt(r @ r / (phi @ phi))) < 1e-12
     # pseudofermion identity Y = D^dag X = chi
     assert rel(g.apply_d(m, x, dagger=True), chi) < 1e-8
+    # stopped on the FP64 true residual: at most a restart past the
+    # reference recurrence, and the true residual meets the tolerance
+    xs, ss = g.cg(phi, m, tol=1e-10, precision="f64", true_residual="stop")
+    r = phi - oc.apply_ddag(dims, m, U, xs)
+    assert np.sqrt(r @ r / (phi @ phi)) < 1e-10 and ss["true_rel_resid"] < 1e-10
+    assert it - 1 <= ss["iterations"] <= it + 32
 
 
 @pytest.fixture(params=[0, 1], ids=["fused", "split"])
@@ -218,6 +225,21 @@ def test_cg_error_paths(ctx):
     # the context stays usable after an error
     x, st = g.cg(phi, 0.1, tol=1e-10)
     assert st["iterations"] > 0
+    # a tolerance below the FP64 rounding floor of the true residual (about
+    # 1e-16 relative): mixed precision runs its iteration budget through
+    # repeated reliable updates and reports non-convergence with the
+    # reference's message (not a stalled state).  The source is scaled so
+    # that <p, v> stays far above the breakdown threshold of 1e-30.
+    big = 1e6 * phi
+    for prec in ("f32", "f16"):
+        with pytest.raises(RuntimeError, match="failed to converge after 600 iterations"):
+            g.cg(big, 0.1, tol=1e-17, max_iter=600, precision=prec)
+    # FP64 stopped on the true residual: meets it or reports non-convergence
+    try:
+        x, st = g.cg(big, 0.1, tol=1e-17, max_iter=600, precision="f64", true_residual="stop")
+        assert st["true_rel_resid"] < 1e-17
+    except RuntimeError as e:
+        assert "failed to converge after 600 iterations" in str(e)
 
 
 def test_cg_16_4_both_precisions(ctx):
@@ -227,10 +249,12 @@ def test_cg_16_4_both_precisions(ctx):
     U = dev.random_links(dims, 0.3, seed=21)
     g = dev.Gauge(ctx, dims, U)
     phi = dev.random_spinor(dims, seed=22)
-    x64, s64 = g.cg(phi, m, tol=1e-10, precision="f64", true_residual=True)
+    x64, s64 = g.cg(phi, m, tol=1e-10, precision="f64", true_residual="stop")
     x32, s32 = g.cg(phi, m, tol=1e-10, precision="f32")
     x16, s16 = g.cg(phi, m, tol=1e-10, precision="f16")
-    assert s64["true_rel_resid"] < 2e-10
+    r = phi - oc.apply_ddag(dims, m, U, x64)
+    assert np.sqrt(r @ r / (phi @ phi)) < 1e-10
+    assert s64["true_rel_resid"] < 1e-10
     assert s32["true_rel_resid"] < 1e-10
     assert s16["true_rel_resid"] < 1e-10
     assert rel(x32, x64) < 1e-8 and rel(x16, x64) < 1e-8
@@ -511,3 +535,25 @@ def test_mixed_cg_non_su3_links(ctx, precision):
         r = phi - oc.apply_ddag(dims, 0.05, links, x)
         assert np.sqrt(r @ r / (phi @ phi)) < 1e-10, st
         assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-8
+
+
+# ---------------------------------------------------------------- Polyakov correlators
+@pytest.mark.parametrize("dims", [(4, 4, 4, 4), (6, 6, 6, 4), (8, 4, 6, 4), (32, 32, 32, 32)])
+def test_polyakov_correlators_match_oracle(ctx, dims):
+    """Device grid and correlators (Lattice.cpp:868-948) for r^2 = 1 ..
+    3 (L/2)^2 + 1 against the oracle's serial sums: equal to rounding of the
+    summation order, relative to the scale of the summands."""
+    U = dev.random_links(dims, 0.6, seed=91)
+    g = dev.Gauge(ctx, dims, U)
+    L = np.asarray(dims[:3])
+    r2_max = int(((L // 2) ** 2).sum()) + 1 if dims[0] < 32 else 15  # Main.cpp: 1 .. Spatial_Size/2 - 1
+    got = g.polyakov_correlators(r2_max)
+    P = oc.polyakov_grid(dims, U)
+    scale = np.abs(P).max() ** 2
+    for r2 in range(1, r2_max + 1):
+        ref = oc.polyakov_correlator(dims[:3], P, r2)
+        assert abs(got[r2 - 1] - ref) < 1e-13 * scale, (r2, got[r2 - 1], ref)
+    # from a host grid, one r^2 at a time (the drop-in's call pattern)
+    for r2 in (1, 2, r2_max):
+        c = dev.polyakov_correlators_grid(ctx, dims[:3], P, r2, r2_first=r2)
+        assert abs(c[0] - oc.polyakov_correlator(dims[:3], P, r2)) < 1e-13 * scale

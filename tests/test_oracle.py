@@ -399,3 +399,94 @@ def test_hmc_trajectory_energy_bookkeeping():
     assert abs(H[1] - oc.action(dims, U0, 5.6, 1.0, 0.0)) < 1e-9
     assert abs(H[6] - (H[0] - H[3] + H[1] - H[4] + H[2] - H[5])) < 1e-9
     assert acc == (0.0 < np.exp(H[6]))
+
+
+def test_leapfrog_stage_order_momentum_first():
+    """Fermions.cpp:684-776: P half kick first, then (link update, kick)
+    steps, a last link update and a closing half kick:
+      P_1/2 = P_0 - (e/2) F(U_0), U_1 = exp(i e P_1/2) U_0,
+      P_3/2 = P_1/2 - e F(U_1),   U_2 = exp(i e P_3/2) U_1,
+      P_2   = P_3/2 - (e/2) F(U_2),
+    rebuilt from the oracle's force with numpy's eigendecomposition for the
+    exponential.  A position-first leapfrog (half link update, full kick,
+    half link update) is also reversible with an e^2 error, and fails this."""
+    dims = DIMS4
+    m = 0.2
+    U0 = F.random_links(dims, 0.3, seed=70)
+    P0 = F.random_momenta(dims, 71)
+    phi = oc.apply_d(dims, m, U0, F.gaussian_spinor(dims, 72))
+    beta, c11, c12, eps = 4.0, 5 / 3, -1 / 12, 0.07
+
+    def force(U, which):
+        if which & 2:
+            X, *_ = oc.cg(dims, m, U, phi, tol=1e-14, max_iter=10000)
+            Y = oc.apply_d(dims, m, U, X, dagger=True)
+            return oc.force(dims, U, X, Y, beta, c11, c12, which=which)
+        return oc.force(dims, U, None, None, beta, c11, c12, which=which)
+
+    def step_links(U, P):
+        E = F.expi_hermitian(eps * F.flat_to_mat(P).reshape(-1, 3, 3))
+        return F.mat_to_flat(np.einsum("lij,ljk->lik", E, F.flat_to_mat(U).reshape(-1, 3, 3)).reshape(
+            F.flat_to_mat(U).shape))
+
+    for which in (1, 3):
+        P = P0 - 0.5 * eps * force(U0, which)
+        U1 = step_links(U0, P)
+        P = P - eps * force(U1, which)
+        U2 = step_links(U1, P)
+        P2 = P - 0.5 * eps * force(U2, which)
+        Ur, Pr, _ = oc.leapfrog(dims, m, U0, P0, phi, beta=beta, c11=c11, c12=c12, eps=eps, nsteps=2,
+                                which=which, tol=1e-14, check=False)
+        assert np.abs(Ur - U2).max() < 1e-12, which
+        assert np.abs(Pr - P2).max() < 1e-9, which
+        # the position-first order lands elsewhere
+        Uh = step_links(U0, 0.5 * P0)  # exp(i (e/2) P0) U0
+        Pp = P0 - eps * force(Uh, which)
+        assert np.abs(step_links(Uh, 0.5 * Pp) - Ur).max() > 1e-4
+
+
+# ---------------------------------------------------------------- Polyakov correlators
+def _corr_pairs(spatial, P, r2):
+    """Brute force over every pair of grid points (m, n): the displacement
+    n - m wrapped into [-L/2, L/2) per axis (so -L/2 and not +L/2 stands for
+    the half-way neighbour), the pairs with |d|^2 = r^2 averaged."""
+    L = np.asarray(spatial)
+    pts = np.array(list(np.ndindex(*spatial)))
+    d = (pts[None, :, :] - pts[:, None, :]) % L
+    d = np.where(d >= L // 2, d - L, d)
+    sel = (d ** 2).sum(-1) == r2
+    if not sel.any():
+        return 0j
+    Pm = np.broadcast_to(P[:, None], sel.shape)[sel]
+    Pn = np.broadcast_to(P[None, :], sel.shape)[sel]
+    return (Pm * Pn.conj()).sum() / sel.sum()
+
+
+@pytest.mark.parametrize("spatial", [(4, 4, 4), (6, 6, 6), (8, 8, 8), (6, 4, 8)])
+def test_polyakov_correlator_against_pair_sum(spatial):
+    """correlator_over_fixed_distance_fast (Lattice.cpp:888-948) against an
+    independent sum over all pairs of grid points: pins the displacement set
+    (the +L/2 exclusion, the floor(sqrt r^2) bound), the conjugation order and
+    the normalisation.  At L = 6 and 8 some r^2 mix displacements with and
+    without an L/2 component, so counting +L/2 as well changes the average."""
+    rng = np.random.default_rng(sum(spatial))
+    n = int(np.prod(spatial))
+    P = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    L = np.asarray(spatial)
+    for r2 in range(1, int(((L // 2) ** 2).sum()) + 2):
+        ref = _corr_pairs(spatial, P, r2)
+        got = oc.polyakov_correlator(spatial, P, r2)
+        assert abs(got - ref) < 1e-12 * max(1.0, abs(ref)), (spatial, r2, got, ref)
+
+
+def test_polyakov_correlator_cold_and_empty():
+    """Cold links: P = 3 everywhere, so C = 9 wherever a displacement in
+    [-L/2, L/2)^3 has |d|^2 = r^2, else 0 (r^2 = 7, 15, ... are not sums of
+    three squares; some larger r^2 need a component beyond L/2)."""
+    dims = (8, 8, 8, 4)
+    P = oc.polyakov_grid(dims, F.cold_links(dims))
+    reach = {dx * dx + dy * dy + dz * dz for dx in range(-4, 4) for dy in range(-4, 4) for dz in range(-4, 4)}
+    for r2 in range(1, 52):
+        c = oc.polyakov_correlator(dims[:3], P, r2)
+        assert c == (9.0 if r2 in reach else 0.0), (r2, c)
+    assert 7 not in reach and 37 not in reach and 48 in reach
