@@ -21,7 +21,8 @@ under Poisson arrivals, ``precision`` the errors of the mixed-precision
 results against the FP64 oracle at 16^4, ``traj32`` a 32^4 Symanzik HMC
 trajectory in FP64 and FP16-mixed, ``heatbath`` Cabibbo-Marinari heatbath
 sweeps of the Symanzik action at 32^4 and 16^4, ``hex`` HEX smearing steps
-at 32^4 and 16^4.
+at 32^4 and 16^4, ``force`` the gauge and fermion force kernels at 32^4 and
+16^4.
 """
 import argparse
 import json
@@ -442,6 +443,60 @@ def bench_traj32(args, dev):
          "gauge_force_kernel_ms": round(force_ms, 3), "gauge_force_fp64_tflops": round(force_tfs, 2)})))
 
 
+# Symanzik gauge force per site: 4 links x (12 plaquette + 72 rectangle
+# staple products + U A) = 85 SU(3) products x 198 flop each.
+FORCE_FLOP_PER_SITE = 4 * 85 * 198
+# Fermion force + momentum kick per site (compulsory bytes): U read (72
+# reals), X and Y read (6 + 6), P read and written (72 + 72), FP64.
+FERMION_FORCE_BYTES_PER_SITE = (72 + 6 + 6 + 72 + 72) * 8
+
+
+def bench_force(args, dev):
+    """The force kernel as the leapfrog runs it (momentum kick fused) on 32^4
+    links exp(i 0.3 H), 16^4 beside it: the Symanzik gauge force (c11 = 5/3,
+    c12 = -1/12, beta = 4.0) against the FP64 VALU peak, the fermion force
+    against HBM, and both together (the leapfrog's kick), each the average of
+    `steps` back-to-back launches between HIP events on the library stream."""
+    ctx = dev.Context(0)
+    n = max(args.steps, 3)
+    res = {}
+    for dims in [(32, 32, 32, 32), (16, 16, 16, 16)]:
+        V = int(np.prod(dims))
+        g = dev.Gauge(ctx, dims, dev.random_links(dims, 0.3, seed=1234))
+        if args.kernels_only:
+            for which in (1, 2, 3):
+                g.time_force(which=which, n=2)
+            ctx.synchronize()
+            log(f"[bench] force kernels done {dims}")
+            continue
+        us = {w: g.time_force(which=w, n=n) for w in (1, 2, 3)}
+        res["x".join(map(str, dims))] = {
+            "gauge_ms": round(us[1] * 1e-3, 4),
+            "gauge_fp64_tflops": round(V * FORCE_FLOP_PER_SITE / (us[1] * 1e-6) / 1e12, 2),
+            "fermion_ms": round(us[2] * 1e-3, 4),
+            "fermion_tb_s": round(V * FERMION_FORCE_BYTES_PER_SITE / (us[2] * 1e-6) / 1e12, 3),
+            "both_ms": round(us[3] * 1e-3, 4)}
+        log(f"[bench] force {dims}: {res['x'.join(map(str, dims))]}")
+    if args.kernels_only:
+        return
+    top = res["32x32x32x32"]
+    out = _base_line(
+        round(1e3 / top["both_ms"], 3), "force evaluations/s", n, top["both_ms"], "f64",
+        "32^4 force evaluation as the leapfrog runs it: Symanzik gauge force (5/3, -1/12), beta 4.0, plus the "
+        "staggered fermion force, momentum kick fused, links exp(i 0.3 H)",
+        {"metric": "force evaluations/s (32^4, Symanzik + fermion force)", "sizes": res,
+         "roofline": {"bound": "mfma", "achieved": top["gauge_fp64_tflops"], "peak": PEAK_FP64_TFS,
+                      "unit": "TFLOP/s", "frac": round(top["gauge_fp64_tflops"] / PEAK_FP64_TFS, 4),
+                      "traffic": traffic_from_profiles("force_gauge", "32^4"),
+                      "kernel": "k_force, gauge part (85 x 198 flop per link); bound: FP64 vector ALU (no FP64 "
+                                "matrix-core advantage on this part)"},
+         "fermion_force_roofline": {"bound": "hbm", "achieved": round(top["fermion_tb_s"] * 1e3, 1),
+                                    "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                    "frac": round(top["fermion_tb_s"] * 1e3 / PEAK_HBM_GBS, 4),
+                                    "traffic": traffic_from_profiles("force_fermion", "32^4")}})
+    print(json.dumps(out))
+
+
 # One link update of the heatbath: 12 plaquette and 72 rectangle staple
 # products, about 3 more for the three SU(2) subgroup rotations (the 2 x 2
 # block of U A and two rotated rows each): 87 SU(3) products x 198 flop.
@@ -592,7 +647,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="32^4",
-                    choices=sorted(WORKLOADS) + ["batch", "serve", "precision", "traj32", "heatbath", "hex"])
+                    choices=sorted(WORKLOADS) + ["batch", "serve", "precision", "traj32", "heatbath", "hex", "force"])
     ap.add_argument("--rates", default="0.5,1,2", help="serve: arrival rates as multiples of the measured "
                                                      "sequential FP64 capacity")
     ap.add_argument("--requests", type=int, default=200, help="serve: requests per arrival rate")
@@ -623,6 +678,8 @@ def main():
         return bench_heatbath(args, dev)
     if args.workload == "hex":
         return bench_hex(args, dev)
+    if args.workload == "force":
+        return bench_force(args, dev)
 
     wl = WORKLOADS[args.workload]
     dims, mass = wl["dims"], wl["mass"]

@@ -156,6 +156,13 @@ int sfh_tune_hop(sfh_ctx* ctx, const sfh_gauge* g, int precision, int n, int* va
  * n back-to-back launches between HIP events on the context stream.  No
  * counterpart in the reference. */
 int sfh_time_hop(sfh_ctx* ctx, const sfh_gauge* g, int precision, int with_dot, int n, double* us);
+/* Average duration in microseconds of the force kernel as the leapfrog
+ * runs it on g's links (force selection `which`, Symanzik coefficients, the
+ * momentum kick P -= eps F fused, no force field stored), n back-to-back
+ * launches between HIP events on the context stream.  No counterpart in the
+ * reference. */
+int sfh_time_force(sfh_ctx* ctx, const sfh_gauge* g, double beta, double c11, double c12, unsigned which, int n,
+                   double* us);
 /* Form of the mixed-precision CG iteration, per precision (1 f32, 2 f16):
  * 0 = fused (the first hop recomputes the vector updates at its neighbours,
  * two kernels per iteration), 1 = split (in-place update kernel, then the

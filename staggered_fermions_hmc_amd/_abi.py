@@ -67,6 +67,7 @@ SIGNATURES = {
     "sfh_tune_hop": (_I, [_P, _P, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_D)]),
     "sfh_time_hop": (_I, [_P, _P, _I, _I, _I, ctypes.POINTER(_D)]),
     "sfh_time_cg_kernels": (_I, [_P, _P, _I, _I, ctypes.POINTER(_D), ctypes.POINTER(_D)]),
+    "sfh_time_force": (_I, [_P, _P, _D, _D, _D, ctypes.c_uint, _I, ctypes.POINTER(_D)]),
     "sfh_set_cg_form": (_I, [_I, _I]),
     "sfh_tune_cg_form": (_I, [_P, _P, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_D)]),
     "sfh_gauge_create": (_I, [_P, _P, ctypes.POINTER(_P)]),

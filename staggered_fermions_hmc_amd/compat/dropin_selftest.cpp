@@ -15,6 +15,10 @@
 #include "../../include/sfh.h"
 #include "Fermions_gpu.h"
 
+// the reference's checkpoint signatures (Lattice.h:43-44), CONFIG = 2
+bool save_all_lattice_configs(const std::array<Link_array, 2>& links_at_coupling, const std::string& filename);
+bool load_all_lattice_configs(std::array<Link_array, 2>& links_at_coupling, const std::string& filename);
+
 static int failures = 0;
 
 static void report(const char* name, bool ok, double value) {
@@ -312,6 +316,7 @@ static int run() {
 
   // checkpoints (Lattice.cpp:810-847): round trip, missing and short files
   {
+    // the reference's non-template signatures with CONFIG = 2 (lattice_io_dropin.cpp)
     std::array<Link_array, 2> saved{hot(0.4, 81), hot(0.4, 82)}, loaded{cold(), cold()};
     const std::string path = "/tmp/sfh_dropin_selftest_ckpt.bin";
     const bool ok_save = save_all_lattice_configs(saved, path);

@@ -426,6 +426,32 @@ int sfh_time_hop(sfh_ctx* c, const sfh_gauge* g, int precision, int with_dot, in
   return SFH_OK;
 }
 
+int sfh_time_force(sfh_ctx* c, const sfh_gauge* g, double beta, double c11, double c12, unsigned which, int n,
+                   double* us) {
+  if (!c || !g || n < 1 || !us || !which || (which & ~3u)) return SFH_E_INVALID;
+  Guard gd(c);
+  const Geom& gg = g->g;
+  double2* X = dbuf(c, "tf_x", 3 * (size_t)gg.V);
+  double2* Y = dbuf(c, "tf_y", 3 * (size_t)gg.V);
+  double2* P = dbuf(c, "tf_p", 36 * (size_t)gg.V);
+  if (!X || !Y || !P) return SFH_E_NOMEM;
+  hipMemsetAsync(X, 0, 3 * (size_t)gg.V * sizeof(double2), c->stream);
+  hipMemsetAsync(Y, 0, 3 * (size_t)gg.V * sizeof(double2), c->stream);
+  hipMemsetAsync(P, 0, 36 * (size_t)gg.V * sizeof(double2), c->stream);
+  // as the leapfrog runs it: no force field stored, P -= coef F fused
+  auto launch = [&]() { return force_field(c, gg, g->U, X, Y, beta, c11, c12, which, nullptr, P, 1e-3); };
+  if (int e = launch()) return e;
+  hipEventRecord(c->ev0, c->stream);
+  for (int k = 0; k < n; ++k)
+    if (int e = launch()) return e;
+  hipEventRecord(c->ev1, c->stream);
+  if (int e = hip_check(c, hipEventSynchronize(c->ev1), "force timing")) return e;
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, c->ev0, c->ev1);
+  *us = 1e3 * ms / n;
+  return SFH_OK;
+}
+
 int sfh_time_cg_kernels(sfh_ctx* c, const sfh_gauge* g, int precision, int n, double* us_a, double* us_b) {
   if (!c || !g || (precision != SFH_PREC_F32 && precision != SFH_PREC_F16) || n < 1 || !us_a || !us_b)
     return SFH_E_INVALID;

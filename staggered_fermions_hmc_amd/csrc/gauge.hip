@@ -253,7 +253,7 @@ __global__ __launch_bounds__(kCorrB) void k_poly_corr_fin(const double* __restri
 
 // Displacement tables of r^2 = r2_first .. r2_first + count - 1 in the
 // reference's loop order (dx, then dy, then dz ascending).
-static void corr_tables(const int L[3], int r2_first, int count, std::vector<int4>& disp, std::vector<int>& off) {
+void corr_tables(const int L[3], int r2_first, int count, std::vector<int4>& disp, std::vector<int>& off) {
   disp.clear();
   off.assign(1, 0);
   for (int r2 = r2_first; r2 < r2_first + count; ++r2) {
@@ -278,19 +278,23 @@ static void corr_tables(const int L[3], int r2_first, int count, std::vector<int
   }
 }
 
-// Correlators of the device grid P (L[0] x L[1] x L[2] complex) into out[2 k],
-// out[2 k + 1] = C(r2_first + k), k < count (host memory).
-static int poly_corr_dev(sfh_ctx* c, const int L[3], const double2* P, int r2_first, int count, double* out) {
-  std::vector<int4> disp;
-  std::vector<int> off;
-  corr_tables(L, r2_first, count, disp, off);
+void polyakov_grid_into(sfh_ctx* c, const Geom& g, const double2* U, double2* P) {
+  const int ns = g.L[0] * g.L[1] * g.L[2];
+  hipLaunchKernelGGL(k_polyakov, dim3((ns + 127) / 128), dim3(128), 0, c->stream, g, U, P);
+}
+
+int poly_corr_blocks(const int L[3]) { return (L[0] * L[1] * L[2] + kCorrB - 1) / kCorrB; }
+
+// Correlators of the device grid P (L[0] x L[1] x L[2] complex) with the
+// tables disp / off into out[2 k], out[2 k + 1] = C(r2_first + k), k < count
+// (host memory), through the device buffers d_disp (>= disp.size() entries),
+// d_off (count + 1), d_part (2 count poly_corr_blocks(L) doubles) and d_out
+// (2 count).  Synchronises the stream.
+int poly_corr_run(sfh_ctx* c, const int L[3], const double2* P, const std::vector<int4>& disp,
+                  const std::vector<int>& off, int4* d_disp, int* d_off, double* d_part, double* d_out, double* out) {
+  const int count = (int)off.size() - 1;
   const int ns = L[0] * L[1] * L[2];
-  const int nblk = (ns + kCorrB - 1) / kCorrB;
-  int4* d_disp = static_cast<int4*>(scratch(c, "pc_disp", (disp.size() + 1) * sizeof(int4)));
-  int* d_off = static_cast<int*>(scratch(c, "pc_off", off.size() * sizeof(int)));
-  double* d_part = static_cast<double*>(scratch(c, "pc_part", 2 * (size_t)count * nblk * sizeof(double)));
-  double* d_out = static_cast<double*>(scratch(c, "pc_out", 2 * (size_t)count * sizeof(double)));
-  if (!d_disp || !d_off || !d_part || !d_out) return SFH_E_NOMEM;
+  const int nblk = poly_corr_blocks(L);
   if (!disp.empty())
     hipMemcpyAsync(d_disp, disp.data(), disp.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream);
   hipMemcpyAsync(d_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, c->stream);
@@ -300,6 +304,19 @@ static int poly_corr_dev(sfh_ctx* c, const int L[3], const double2* P, int r2_fi
   hipMemcpyAsync(out, d_out, 2 * (size_t)count * sizeof(double), hipMemcpyDeviceToHost, c->stream);
   // the host tables stay alive until the stream has drained
   return hip_check(c, hipStreamSynchronize(c->stream), "polyakov correlators");
+}
+
+static int poly_corr_dev(sfh_ctx* c, const int L[3], const double2* P, int r2_first, int count, double* out) {
+  std::vector<int4> disp;
+  std::vector<int> off;
+  corr_tables(L, r2_first, count, disp, off);
+  const int nblk = poly_corr_blocks(L);
+  int4* d_disp = static_cast<int4*>(scratch(c, "pc_disp", (disp.size() + 1) * sizeof(int4)));
+  int* d_off = static_cast<int*>(scratch(c, "pc_off", off.size() * sizeof(int)));
+  double* d_part = static_cast<double*>(scratch(c, "pc_part", 2 * (size_t)count * nblk * sizeof(double)));
+  double* d_out = static_cast<double*>(scratch(c, "pc_out", 2 * (size_t)count * sizeof(double)));
+  if (!d_disp || !d_off || !d_part || !d_out) return SFH_E_NOMEM;
+  return poly_corr_run(c, L, P, disp, off, d_disp, d_off, d_part, d_out, out);
 }
 
 int polyakov_correlators(sfh_ctx* c, const Geom& g, const double2* U, int r2_first, int count, double* out) {

@@ -207,6 +207,14 @@ int polyakov_grid(sfh_ctx* c, const Geom& g, const double2* U, double* out);
 // into out (host, count complex), from the device links or a host grid.
 int polyakov_correlators(sfh_ctx* c, const Geom& g, const double2* U, int r2_first, int count, double* out);
 int polyakov_correlators_grid(sfh_ctx* c, const int L[3], const double* grid, int r2_first, int count, double* out);
+// Pieces of the correlators for callers that bring their own device buffers:
+// the displacement tables, the grid from device links, the block count of
+// the first kernel, and the run (see gauge.hip).
+void corr_tables(const int L[3], int r2_first, int count, std::vector<int4>& disp, std::vector<int>& off);
+void polyakov_grid_into(sfh_ctx* c, const Geom& g, const double2* U, double2* P);
+int poly_corr_blocks(const int L[3]);
+int poly_corr_run(sfh_ctx* c, const int L[3], const double2* P, const std::vector<int4>& disp,
+                  const std::vector<int>& off, int4* d_disp, int* d_off, double* d_part, double* d_out, double* out);
 
 // ------------------------------------------------------------------ heatbath (heatbath.hip)
 bool heatbath_dims_ok(const Geom& g);

@@ -16,6 +16,7 @@
 // solve alone: a chain's kernels and reduction order do not depend on the
 // rest of the batch.
 #include <algorithm>
+#include <atomic>
 #include <condition_variable>
 #include <deque>
 #include <map>
@@ -55,7 +56,26 @@ struct sfh_service {
   std::string worker_err;
   long completed = 0;
   long slots_refused = 0;  // admissions deferred because memory refused a new slot
+  // callers waiting for the engine lock: the worker lets them in between
+  // chunks instead of taking the lock straight back
+  std::atomic<int> want_engine{0};
 };
+
+namespace {
+
+// The engine lock for an API call: announced to the worker first, so a
+// busy worker hands it over at its next chunk boundary.
+struct EngineLock {
+  sfh_service* s;
+  std::unique_lock<std::mutex> lk;
+  explicit EngineLock(sfh_service* s_) : s(s_) {
+    ++s->want_engine;
+    lk = std::unique_lock<std::mutex>(s->emu);
+    --s->want_engine;
+  }
+};
+
+}  // namespace
 
 namespace {
 
@@ -77,6 +97,7 @@ bool slot_memory_ok(sfh_service* s, int V) {
 void worker_loop(sfh_service* s) {
   hipSetDevice(s->ctx->device);
   for (;;) {
+    while (s->want_engine.load() > 0) std::this_thread::yield();
     std::unique_lock<std::mutex> le(s->emu);
     std::vector<sfh_service::Req> admit;
     std::vector<sfh_service::Req> refused;
@@ -186,7 +207,7 @@ int sfh_service_create(int device, size_t mem_budget, double tol, int max_iter, 
 
 int sfh_service_add_chain(sfh_service* s, const sfh_gauge* g, double mass, int max_inflight, int* chain) {
   if (!s || !g || !chain || max_inflight < 1 || max_inflight > sfh::kMaxRhs) return SFH_E_INVALID;
-  std::lock_guard<std::mutex> le(s->emu);
+  EngineLock le(s);
   // a chain one solve of which can never fit the budget is refused now;
   // its slots are allocated as its requests are admitted
   const size_t need = sfh::Engine<float>::slot_bytes(g->g.V);
@@ -209,7 +230,12 @@ int sfh_service_add_chain(sfh_service* s, const sfh_gauge* g, double mass, int m
 // flight, e.g. after each molecular-dynamics step of its trajectory.
 int sfh_service_update_chain(sfh_service* s, int chain, const sfh_gauge* g, double mass) {
   if (!s || !g) return SFH_E_INVALID;
-  std::lock_guard<std::mutex> le(s->emu);
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (chain >= 0 && chain < (int)s->chain_pending.size() && s->chain_pending[chain] != 0)
+      return sfh::set_err(s->ctx, SFH_E_INVALID, "chain has requests queued or in flight");
+  }
+  EngineLock le(s);
   {
     std::lock_guard<std::mutex> lk(s->mu);
     if (chain < 0 || chain >= (int)s->chain_V.size() || s->chain_V[chain] < 0)
@@ -225,7 +251,7 @@ int sfh_service_update_chain(sfh_service* s, int chain, const sfh_gauge* g, doub
 // for other chains.  The chain id is not reused.
 int sfh_service_remove_chain(sfh_service* s, int chain) {
   if (!s) return SFH_E_INVALID;
-  std::lock_guard<std::mutex> le(s->emu);
+  EngineLock le(s);
   std::lock_guard<std::mutex> lk(s->mu);
   if (chain < 0 || chain >= (int)s->chain_V.size() || s->chain_V[chain] < 0)
     return sfh::set_err(s->ctx, SFH_E_INVALID, "no such chain");
@@ -239,7 +265,7 @@ int sfh_service_remove_chain(sfh_service* s, int chain) {
 // Solve-slot memory of the service and how often a slot was refused.
 int sfh_service_memory(sfh_service* s, size_t* slot_bytes, long* refusals) {
   if (!s) return SFH_E_INVALID;
-  std::lock_guard<std::mutex> le(s->emu);
+  EngineLock le(s);
   std::lock_guard<std::mutex> lk(s->mu);
   if (slot_bytes) *slot_bytes = s->eng->bytes();
   if (refusals) *refusals = s->slots_refused;

@@ -205,6 +205,14 @@ class Gauge:
                                                          ctypes.byref(ua), ctypes.byref(ub)))
         return ua.value, ub.value
 
+    def time_force(self, beta=4.0, c11=5.0 / 3.0, c12=-1.0 / 12.0, which=1, n=10):
+        """Average microseconds of the force kernel as the leapfrog runs it
+        (momentum kick fused, no force field stored)."""
+        us = ctypes.c_double()
+        self.ctx.check(self.ctx._lib.sfh_time_force(self.ctx.h, self.h, float(beta), float(c11), float(c12),
+                                                    int(which), int(n), ctypes.byref(us)))
+        return us.value
+
     def action(self, beta, c11, c12):
         out = ctypes.c_double()
         self.ctx.check(self.ctx._lib.sfh_action(self.ctx.h, self.h, beta, c11, c12, ctypes.byref(out)))

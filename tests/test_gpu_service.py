@@ -83,3 +83,102 @@ def test_service_memory_budget_refuses_chain(ctx):
             svc.add_chain(chains[1]["g"], chains[1]["mass"], max_inflight=4)
     finally:
         svc.close()
+
+
+def test_service_update_chain_uses_new_links(ctx):
+    """A chain whose links change between requests (an MD step) is refreshed
+    with update_chain; the next result equals a solo solve on the new links
+    bit for bit, and differs from the one on the old links.  Updating a
+    chain with requests queued or in flight is refused."""
+    chains = make_chains(ctx, seed=870)
+    c = chains[1]
+    svc = dev.Service(0, tol=1e-10)
+    try:
+        cid = svc.add_chain(c["g"], c["mass"], max_inflight=2)
+        b = dev.random_spinor(c["dims"], seed=871)
+        rf, x0, x1 = dev.Field(c["g"], b), dev.Field(c["g"]), dev.Field(c["g"])
+        # eight requests on two slots: some are still queued when update_chain runs
+        busy = [dev.Field(c["g"]) for _ in range(8)]
+        tickets = [svc.submit(cid, rf, xb) for xb in busy]
+        with pytest.raises(SfhError):
+            svc.update_chain(cid, c["g"], c["mass"])
+        for t in tickets:
+            svc.wait(t, timeout=120)
+        svc.wait(svc.submit(cid, rf, x0), timeout=120)
+        # an MD step on the chain's own device links, then the refresh
+        P = dev.random_momenta(c["dims"], seed=872)
+        c["g"].link_update(0.05, P)
+        U1 = c["g"].download()
+        svc.update_chain(cid, c["g"], c["mass"])
+        svc.wait(svc.submit(cid, rf, x1), timeout=120)
+        x = x1.download()
+        res = b - oc.apply_ddag(c["dims"], c["mass"], U1, x)
+        assert np.sqrt(res @ res / (b @ b)) < 1e-10
+        xs = dev.Field(c["g"])
+        dev.cg_batch(ctx, [(c["g"], c["mass"], [rf], [xs])], tol=1e-10, precision="f32")
+        assert np.array_equal(xs.download(), x)
+        assert not np.array_equal(x0.download(), x)
+    finally:
+        svc.close()
+
+
+def test_service_remove_chain_reuses_slots(ctx):
+    """A removed chain's solve slots serve the next chain of the same lattice
+    without new device memory; the removed chain refuses requests."""
+    chains = make_chains(ctx, seed=880)
+    svc = dev.Service(0, tol=1e-10)
+    try:
+        a = chains[1]
+        ida = svc.add_chain(a["g"], a["mass"], max_inflight=2)
+        fields = [(dev.Field(a["g"], dev.random_spinor(a["dims"], seed=881 + j)), dev.Field(a["g"])) for j in range(2)]
+        for t in [svc.submit(ida, r, x) for r, x in fields]:
+            assert svc.wait(t, timeout=120)["status"] == 0
+        held, _ = svc.memory()
+        assert held > 0
+        svc.remove_chain(ida)
+        with pytest.raises(SfhError):
+            svc.submit(ida, *fields[0])
+        g2 = dev.Gauge(ctx, a["dims"], dev.random_links(a["dims"], 0.3, seed=889))
+        idb = svc.add_chain(g2, 0.04, max_inflight=2)
+        b = dev.random_spinor(a["dims"], seed=890)
+        rf, xf = dev.Field(g2, b), dev.Field(g2)
+        st = svc.wait(svc.submit(idb, rf, xf), timeout=120)
+        assert st["status"] == 0 and st["true_rel_resid"] < 1e-10
+        assert svc.memory()[0] == held  # no new slot memory
+    finally:
+        svc.close()
+
+
+def test_service_admission_by_memory_budget(ctx):
+    """A budget of two slots for a chain allowed eight in flight: requests
+    beyond two wait for a slot (each refusal is counted) and all complete,
+    each equal to its solo solve."""
+    chains = make_chains(ctx, seed=900)
+    c = chains[1]
+    slot = None
+    probe = dev.Service(0, tol=1e-10)
+    try:
+        pid = probe.add_chain(c["g"], c["mass"], max_inflight=1)
+        rf, xf = dev.Field(c["g"], dev.random_spinor(c["dims"], seed=901)), dev.Field(c["g"])
+        probe.wait(probe.submit(pid, rf, xf), timeout=120)
+        slot, _ = probe.memory()
+    finally:
+        probe.close()
+    svc = dev.Service(0, mem_budget=2 * slot + slot // 2, tol=1e-10)
+    try:
+        cid = svc.add_chain(c["g"], c["mass"], max_inflight=8)
+        reqs = []
+        for j in range(6):
+            b = dev.random_spinor(c["dims"], seed=910 + j)
+            rf, xf = dev.Field(c["g"], b), dev.Field(c["g"])
+            reqs.append((rf, xf, svc.submit(cid, rf, xf)))
+        for rf, xf, t in reqs:
+            assert svc.wait(t, timeout=120)["status"] == 0
+        held, refused = svc.memory()
+        assert held == 2 * slot and refused >= 1
+        for rf, xf, t in reqs:
+            xs = dev.Field(c["g"])
+            dev.cg_batch(ctx, [(c["g"], c["mass"], [rf], [xs])], tol=1e-10, precision="f32")
+            assert np.array_equal(xs.download(), xf.download())
+    finally:
+        svc.close()
