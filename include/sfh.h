@@ -341,6 +341,16 @@ int sfh_polyakov_correlators(sfh_ctx* ctx, const sfh_gauge* g, int r2_first, int
  * that precompute_polyakov_grid returns. */
 int sfh_polyakov_correlators_grid(sfh_ctx* ctx, const int spatial[3], const double* grid, int r2_first, int count,
                                   double* out);
+/* sfh_polyakov_correlators without any device allocation.  No counterpart
+ * in the reference: for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Its work buffers are one static device array of the library
+ * per GPU (9.3 MiB), so it serves lattices of at most 2^18 spatial sites
+ * (L0 L1 L2), at most 256 values of r^2 per call and at most 65536
+ * displacements in total; larger requests fail with SFH_E_INVALID.  Calls
+ * on one GPU share that array and are serialised by a per-device lock in
+ * this process; results equal sfh_polyakov_correlators' bit for bit. */
+int sfh_polyakov_correlators_noalloc(sfh_ctx* ctx, const sfh_gauge* g, int r2_first, int count, double* out);
 /* Host link arrays (n of them, doubles_each doubles each) written back to
  * back as raw little-endian doubles, and read back: the byte format of
  * save_all_lattice_configs / load_all_lattice_configs (Lattice.cpp:810-847),

@@ -110,6 +110,7 @@ SIGNATURES = {
     "sfh_polyakov_grid": (_I, [_P, _P, _P]),
     "sfh_polyakov_correlators": (_I, [_P, _P, _I, _I, _P]),
     "sfh_polyakov_correlators_grid": (_I, [_P, _P, _P, _I, _I, _P]),
+    "sfh_polyakov_correlators_noalloc": (_I, [_P, _P, _I, _I, _P]),
     "sfh_links_save": (_I, [_P, _I, ctypes.c_size_t, ctypes.c_char_p]),
     "sfh_links_load": (_I, [_P, _I, ctypes.c_size_t, ctypes.c_char_p]),
     "sfh_heatbath": (_I, [_P, _P, _D, _D, _D, ctypes.c_ulonglong, ctypes.c_uint, _I, _P, _P]),

@@ -262,6 +262,8 @@ int sfh_create(int device, sfh_ctx** out) {
          hipMemcpyAsync(c->d_state, z, sizeof z, hipMemcpyHostToDevice, c->stream) == hipSuccess &&
          hipStreamSynchronize(c->stream) == hipSuccess;
   }
+  // device address of the allocation-free entry points' work area (once per device)
+  ok = ok && noalloc_init(device) == SFH_OK;
   if (!ok) {
     sfh_destroy(c);
     return SFH_E_HIP;

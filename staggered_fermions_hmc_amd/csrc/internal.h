@@ -215,6 +215,10 @@ void polyakov_grid_into(sfh_ctx* c, const Geom& g, const double2* U, double2* P)
 int poly_corr_blocks(const int L[3]);
 int poly_corr_run(sfh_ctx* c, const int L[3], const double2* P, const std::vector<int4>& disp,
                   const std::vector<int>& off, int4* d_disp, int* d_off, double* d_part, double* d_out, double* out);
+// Allocation-free entry points (noalloc.hip): per-device address of the
+// library's static device work area, looked up once per device by the
+// context's creation.
+int noalloc_init(int device);
 
 // ------------------------------------------------------------------ heatbath (heatbath.hip)
 bool heatbath_dims_ok(const Geom& g);
