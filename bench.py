@@ -169,6 +169,16 @@ def bench_batch(args, dev):
     systems = make_batch_systems(dev, ctx)
     nrhs = sum(len(s["rf"]) for s in systems)
     spec = [(s["g"], s["mass"], s["rf"], s["xf"]) for s in systems]
+    if args.kernels_only:
+        # one batched solve of the 64 systems (default launch shape), then one
+        # FP32 link conversion of a 24^4 lattice (reads 576 B, writes 288 B per
+        # site) that calibrates the traffic counters
+        dev.cg_batch(ctx, spec, tol=1e-10, precision="f32")
+        g = dev.Gauge(ctx, (24, 24, 24, 24), dev.random_links((24, 24, 24, 24), 0.3, seed=5))
+        g.time_hop("f32", False, 1)
+        ctx.synchronize()
+        log("[bench] batch kernels done")
+        return
     dev.cg_batch(ctx, spec[:2], tol=1e-10, precision="f32")  # warm-up
     # autotuner: launch shape of the batched hop for this batch (untimed)
     shape_threads, shape_tile, shape_us = dev.tune_batch(ctx, spec, tol=1e-10, precision="f32")
@@ -652,6 +662,12 @@ def main():
                                                      "sequential FP64 capacity")
     ap.add_argument("--requests", type=int, default=200, help="serve: requests per arrival rate")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="solve workloads: launch each CG chunk's kernels directly instead of replaying HIP graphs "
+                         "(the same kernels; for kernel tracers)")
+    ap.add_argument("--legs", default="both", choices=["both", "mixed", "f64"],
+                    help="solve workloads: time the mixed-precision leg, the FP64 leg or both (one leg prints its "
+                         "timings to stderr instead of the JSON line, e.g. for a kernel trace of one leg)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the kernel timing loops of the workload (the hop in FP64 and in the inner precision, "
                          "the two steps of the mixed iteration), e.g. under a hardware-counter profiler; the first "
@@ -689,6 +705,8 @@ def main():
     b = dev.random_spinor(dims, seed=5678)
 
     ctx = dev.Context(0)
+    if args.no_graphs:
+        ctx.set_graph_replay(False)
     g = dev.Gauge(ctx, dims, U)
     if args.kernels_only:
         out = {"workload": args.workload, "lattice": "x".join(map(str, dims))}
@@ -714,7 +732,8 @@ def main():
         return time.perf_counter() - t0, st
 
     res = {}
-    for prec in (wl["mixed"], "f64"):
+    legs = {"both": (wl["mixed"], "f64"), "mixed": (wl["mixed"],), "f64": ("f64",)}[args.legs]
+    for prec in legs:
         for _ in range(args.warmup):
             solve(prec)
         ctx.synchronize()
@@ -747,6 +766,12 @@ def main():
         log(f"[bench] {prec}: {args.steps} solves in {total:.3f}s, iters {stats['iterations']}, "
             f"reliable updates {stats['reliable_updates']}, true residual {true_res:.2e}, hop {hop_us:.2f} us")
 
+    if args.legs != "both":
+        r = res[legs[0]]
+        log(json.dumps({"leg": legs[0], "solves_per_s": args.steps / r["total"], "iterations": r["stats"]["iterations"],
+                        "true_rel_residual": r["true_res"], "hop_us": r["hop_us"], "hop_dot_us": r["hop_dot_us"],
+                        "mix_us": r["mix_us"]}))
+        return
     mixed = res[wl["mixed"]]
     f64 = res["f64"]
     value = args.steps / mixed["total"]

@@ -138,6 +138,11 @@ int sfh_synchronize(sfh_ctx* ctx);
  * timed launches. */
 int sfh_set_profiling(sfh_ctx* ctx, int on);
 int sfh_kernel_time(sfh_ctx* ctx, const char* name, double* total_ms, long* launches);
+/* on = 1 (the default): the CG solvers replay each chunk of iterations as a
+ * HIP graph; on = 0: they launch the chunk's kernels directly (the same
+ * kernels and results), e.g. under a kernel tracer.  No counterpart in the
+ * reference. */
+int sfh_set_graph_replay(sfh_ctx* ctx, int on);
 
 /* Hop kernel form, per precision (0 f64, 1 f32, 2 f16): 0 = one lane per
  * site with all four directions' loads in flight, 1 = four lanes per site

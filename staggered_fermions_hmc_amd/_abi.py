@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libsfh.so")
+# SFH_LIBRARY names another build of the library (for comparing builds)
+LIB_PATH = os.environ.get("SFH_LIBRARY") or os.path.join(_HERE, "libsfh.so")
 
 _D = ctypes.c_double
 _I = ctypes.c_int
@@ -24,6 +25,7 @@ STATUS_NAMES = {
     0: "SFH_OK", 1: "SFH_E_INVALID", 2: "SFH_E_NOMEM", 3: "SFH_E_HIP", 4: "SFH_E_NODEVICE",
     5: "SFH_E_BREAKDOWN", 6: "SFH_E_NONFINITE_ALPHA", 7: "SFH_E_NONFINITE_BETA", 8: "SFH_E_NOCONV",
     9: "SFH_E_NOT_HERMITIAN", 10: "SFH_E_EXP_QCUBED", 11: "SFH_E_EXP_NEG_TRACE", 12: "SFH_E_SAMPLER",
+    13: "SFH_E_IO",
 }
 PREC = {"f64": 0, "f32": 1, "f16": 2}
 FORCE_GAUGE = 1
@@ -63,6 +65,7 @@ SIGNATURES = {
     "sfh_synchronize": (_I, [_P]),
     "sfh_set_profiling": (_I, [_P, _I]),
     "sfh_kernel_time": (_I, [_P, ctypes.c_char_p, ctypes.POINTER(_D), ctypes.POINTER(ctypes.c_long)]),
+    "sfh_set_graph_replay": (_I, [_P, _I]),
     "sfh_set_hop_variant": (_I, [_I, _I]),
     "sfh_tune_hop": (_I, [_P, _P, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_D)]),
     "sfh_time_hop": (_I, [_P, _P, _I, _I, _I, ctypes.POINTER(_D)]),

@@ -187,7 +187,8 @@ int md_kick(sfh_ctx* c, sfh_gauge* gauge, const sfh_md_params* p, const double2*
     KTimer kt(c, "hop_f64");
     hop_f64(c->stream, g, gauge->U, X, Y, p->mass, -1.0, nullptr, nullptr, nullptr, 0, gauge->f64_su3);
   }
-  return force_field(c, g, gauge->U, X, Y, p->beta, p->c11, p->c12, p->which, nullptr, P, coef);
+  if (int e = ensure_su3(c, gauge)) return e;
+  return force_field(c, g, gauge->U, X, Y, p->beta, p->c11, p->c12, p->which, nullptr, P, coef, gauge->f64_su3);
 }
 
 int md_leapfrog(sfh_ctx* c, sfh_gauge* gauge, const sfh_md_params* p, const double2* phi, double2* P,
@@ -316,6 +317,13 @@ int sfh_set_profiling(sfh_ctx* c, int on) {
   return SFH_OK;
 }
 
+int sfh_set_graph_replay(sfh_ctx* c, int on) {
+  if (!c) return SFH_E_INVALID;
+  Guard gd(c);
+  c->graph_replay = on != 0;
+  return SFH_OK;
+}
+
 int sfh_kernel_time(sfh_ctx* c, const char* name, double* total_ms, long* launches) {
   if (!c || !name) return SFH_E_INVALID;
   Guard gd(c);
@@ -441,7 +449,10 @@ int sfh_time_force(sfh_ctx* c, const sfh_gauge* g, double beta, double c11, doub
   hipMemsetAsync(Y, 0, 3 * (size_t)gg.V * sizeof(double2), c->stream);
   hipMemsetAsync(P, 0, 36 * (size_t)gg.V * sizeof(double2), c->stream);
   // as the leapfrog runs it: no force field stored, P -= coef F fused
-  auto launch = [&]() { return force_field(c, gg, g->U, X, Y, beta, c11, c12, which, nullptr, P, 1e-3); };
+  if (int e = ensure_su3(c, const_cast<sfh_gauge*>(g))) return e;
+  auto launch = [&]() {
+    return force_field(c, gg, g->U, X, Y, beta, c11, c12, which, nullptr, P, 1e-3, g->f64_su3);
+  };
   if (int e = launch()) return e;
   hipEventRecord(c->ev0, c->stream);
   for (int k = 0; k < n; ++k)
@@ -786,7 +797,8 @@ int sfh_force(sfh_ctx* c, const sfh_gauge* g, double beta, double c11, double c1
   }
   double2* dF = dbuf(c, "force", 36 * (size_t)gg.V);
   if (!dF) return SFH_E_NOMEM;
-  if (int e = force_field(c, gg, g->U, dx, dy, beta, c11, c12, which, dF, nullptr, 0.0)) return e;
+  if (int e = ensure_su3(c, const_cast<sfh_gauge*>(g))) return e;
+  if (int e = force_field(c, gg, g->U, dx, dy, beta, c11, c12, which, dF, nullptr, 0.0, g->f64_su3)) return e;
   return download_links(c, gg, dF, F);
 }
 

@@ -727,7 +727,7 @@ int Engine<T>::rebuild() {
   hipMemcpyAsync(dmb_, mb.data(), mb.size() * sizeof(int2), hipMemcpyHostToDevice, c_->stream);
   hipMemcpyAsync(dids_, mapped_.data(), mapped_.size() * sizeof(int), hipMemcpyHostToDevice, c_->stream);
   if (int e = hip_check(c_, hipStreamSynchronize(c_->stream), "engine tables")) return e;
-  if (!use_graph_) return SFH_OK;
+  if (!use_graph_ || !c_->graph_replay) return SFH_OK;
   if (hipStreamBeginCapture(c_->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
     return set_err(c_, SFH_E_HIP, "engine graph capture");
   launch_chunk();

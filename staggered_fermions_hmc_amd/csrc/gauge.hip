@@ -40,6 +40,7 @@ __device__ __forceinline__ M mi_antiherm(const M& A, double s) {
 
 // Total force on link (i, mu) and optional P -= coef F (Fermions.cpp:372-384,
 // :443-515, :640-659, :690-733).
+template <bool R12>
 __global__ __launch_bounds__(128) void k_force(Geom g, const double2* __restrict__ U, const double2* __restrict__ X,
                                                const double2* __restrict__ Y, double beta, double c11, double c12,
                                                unsigned which, double2* __restrict__ F, double2* __restrict__ P,
@@ -55,8 +56,8 @@ __global__ __launch_bounds__(128) void k_force(Geom g, const double2* __restrict
   const Site4 n = site_of(g, i);
   M f = mzero<double>();
   if (which & SFH_FORCE_GAUGE) {
-    const M A = staples(U, g, n, mu, c11, c12);
-    const M Q = mmul(ldL(U, g, i, mu), A);
+    M Q = ldLs_after<R12>(U, g, i, mu, 0.0);
+    rmul_nf(Q, staples_fused<R12>(U, g, n, mu, c11, c12));
     f = traceless(mi_antiherm(Q, beta / 12.0));
   }
   if (which & SFH_FORCE_FERMION) {
@@ -104,11 +105,15 @@ __global__ __launch_bounds__(128) void k_force(Geom g, const double2* __restrict
 }
 
 int force_field(sfh_ctx* c, const Geom& g, const double2* U, const double2* X, const double2* Y, double beta,
-                double c11, double c12, unsigned which, double2* F, double2* P, double coef) {
+                double c11, double c12, unsigned which, double2* F, double2* P, double coef, bool r12) {
   KTimer kt(c, "force");
   const int nb = 4 * ((g.Vh + kRangeHalf - 1) / kRangeHalf);
-  hipLaunchKernelGGL(k_force, dim3(nb), dim3(2 * kRangeHalf), 0, c->stream, g, U, X, Y, beta, c11, c12, which, F, P,
-                     coef, nb);
+  if (r12)
+    hipLaunchKernelGGL(k_force<true>, dim3(nb), dim3(2 * kRangeHalf), 0, c->stream, g, U, X, Y, beta, c11, c12,
+                       which, F, P, coef, nb);
+  else
+    hipLaunchKernelGGL(k_force<false>, dim3(nb), dim3(2 * kRangeHalf), 0, c->stream, g, U, X, Y, beta, c11, c12,
+                       which, F, P, coef, nb);
   return hip_check(c, hipGetLastError(), "force kernel launch");
 }
 

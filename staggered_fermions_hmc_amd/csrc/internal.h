@@ -92,6 +92,7 @@ struct sfh_ctx {
   unsigned* aux_ctr() const { return sfh::ctr_of(d_ctr + sfh::kCtrKinds * sfh::kCtrPerKind, sfh::CNT_AUX); }
   double* h_scalar = nullptr;
   bool profiling = false;
+  bool graph_replay = true;  // replay CG chunks as HIP graphs (false: direct launches)
   int prof_every = 1;   // time one launch in prof_every per kernel name
   std::map<std::string, long> prof_seen;
   std::map<std::string, std::pair<double, long>> ktime;
@@ -195,8 +196,9 @@ void ru_sub(hipStream_t s, const Geom& g, double2* r, const double2* phi, const 
             double* partials);
 
 // ------------------------------------------------------------------ gauge (gauge.hip)
+// r12: the links are SU(3) to kSu3TolF64; the gauge force reads two rows.
 int force_field(sfh_ctx* c, const Geom& g, const double2* U, const double2* X, const double2* Y, double beta,
-                double c11, double c12, unsigned which, double2* F, double2* P, double coef);
+                double c11, double c12, unsigned which, double2* F, double2* P, double coef, bool r12);
 double gauge_action(sfh_ctx* c, const Geom& g, const double2* U, double beta, double c11, double c12);
 double avg_plaq(sfh_ctx* c, const Geom& g, const double2* U);
 double tr_p2(sfh_ctx* c, const Geom& g, const double2* P);

@@ -118,7 +118,7 @@ int upload_state(sfh_ctx* c, const CGState& s) {
 // buffers, lattice and parameters they were captured with.  With kernel
 // profiling on, chunks are launched directly so every kernel can be timed.
 template <class F> int launch_chunk(sfh_ctx* c, const std::string& key, F direct) {
-  if (c->profiling) {
+  if (c->profiling || !c->graph_replay) {
     direct();
     return SFH_OK;
   }

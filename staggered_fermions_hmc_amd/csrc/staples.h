@@ -24,6 +24,21 @@ struct Site4 {
     x2 = d == 2 ? v : x2;
     x3 = d == 3 ? v : x3;
   }
+  // the same move with every choice written as a flat select on the
+  // direction (uniform in the staple kernels): nested choices became trees of
+  // uniform branches, and the register allocator then kept the union of the
+  // values of both sides of every branch alive
+  __device__ __forceinline__ void move_f(const Geom& g, int d, int s) {
+    const bool f0 = d == 0, f1 = d == 1, f2 = d == 2, f3 = d == 3;
+    const int L = (f0 ? g.L[0] : 0) + (f1 ? g.L[1] : 0) + (f2 ? g.L[2] : 0) + (f3 ? g.L[3] : 0);
+    int v = (f0 ? x0 : 0) + (f1 ? x1 : 0) + (f2 ? x2 : 0) + (f3 ? x3 : 0) + s;
+    v += v < 0 ? L : 0;
+    v -= v >= L ? L : 0;
+    x0 = f0 ? v : x0;
+    x1 = f1 ? v : x1;
+    x2 = f2 ? v : x2;
+    x3 = f3 ? v : x3;
+  }
   __device__ __forceinline__ int index(const Geom& g) const {
     const int p = (x0 + x1 + x2 + x3) & 1;
     return p * g.Vh + ((x0 * g.L[1] + x1) * g.L[2] + x2) * g.L3h + (x3 >> 1);
@@ -214,6 +229,225 @@ static __device__ M staples(const double2* __restrict__ U, const Geom& g, const 
 #pragma unroll 1
       for (int k = 0; k < 6; ++k) A = madd(A, mscale(c12, walk<IX>(U, g, x, mu, nu, kRectPath[k])));
     }
+  }
+  return A;
+}
+
+// Link (i, d) with a wave-uniform base per element (d uniform, V a kernel
+// argument) and a 32-bit lane offset: the nine loads take a scalar base and
+// one vector offset, with no per-element 64-bit address arithmetic in the
+// vector ALU.  i < 2^24, so i * 16 fits the 32-bit offset.
+__device__ __forceinline__ M ldLs(const double2* __restrict__ U, const Geom& g, int i, int d) {
+  const size_t es = (size_t)g.V * sizeof(double2);
+  const char* base = reinterpret_cast<const char*>(U) + (size_t)(d * 9) * es;
+  const unsigned off = (unsigned)i * (unsigned)sizeof(double2);
+  M m;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) {
+    const double2 v = *reinterpret_cast<const double2*>(base + e * es + off);
+    m.e[e] = {v.x, v.y};
+  }
+  return m;
+}
+
+// t <- t m and t <- t m^dag with every complex multiply-add written as two
+// real fused multiply-adds per component (the plain expressions compile to a
+// multiply, a fused multiply-add and an add): 12 FP64 instructions per
+// element instead of 16.
+__device__ __forceinline__ void rmul_nf(M& t, const M& m) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const C a0 = t.e[3 * i], a1 = t.e[3 * i + 1], a2 = t.e[3 * i + 2];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const C b0 = m.e[j], b1 = m.e[3 + j], b2 = m.e[6 + j];
+      double re = a0.re * b0.re, im = a0.re * b0.im;
+      re = fma(-a0.im, b0.im, re);
+      im = fma(a0.im, b0.re, im);
+      re = fma(a1.re, b1.re, re);
+      re = fma(-a1.im, b1.im, re);
+      im = fma(a1.re, b1.im, im);
+      im = fma(a1.im, b1.re, im);
+      re = fma(a2.re, b2.re, re);
+      re = fma(-a2.im, b2.im, re);
+      im = fma(a2.re, b2.im, im);
+      im = fma(a2.im, b2.re, im);
+      t.e[3 * i + j] = {re, im};
+    }
+  }
+}
+__device__ __forceinline__ void rmul_df(M& t, const M& m) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const C a0 = t.e[3 * i], a1 = t.e[3 * i + 1], a2 = t.e[3 * i + 2];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      // row j of m, conjugated
+      const C b0 = m.e[3 * j], b1 = m.e[3 * j + 1], b2 = m.e[3 * j + 2];
+      double re = a0.re * b0.re, im = -a0.re * b0.im;
+      re = fma(a0.im, b0.im, re);
+      im = fma(a0.im, b0.re, im);
+      re = fma(a1.re, b1.re, re);
+      re = fma(a1.im, b1.im, re);
+      im = fma(-a1.re, b1.im, im);
+      im = fma(a1.im, b1.re, im);
+      re = fma(a2.re, b2.re, re);
+      re = fma(a2.im, b2.im, re);
+      im = fma(-a2.re, b2.im, im);
+      im = fma(a2.im, b2.re, im);
+      t.e[3 * i + j] = {re, im};
+    }
+  }
+}
+
+// ldLs whose lane offset passes through an empty asm statement that also
+// takes `dep`: the load cannot issue before `dep` is computed.  Chaining each
+// link load of the staple paths to the product before it keeps the compiler
+// from hoisting the loads of later paths (independent of earlier ones) to the
+// top of the loop body, which held dozens of links in registers at once and
+// spilled.
+// R12: SU(3) links, elements 0-5 read and row 2 rebuilt as conj(row0 x row1).
+template <bool R12 = false>
+__device__ __forceinline__ M ldLs_after(const double2* __restrict__ U, const Geom& g, int i, int d, double dep) {
+  const size_t es = (size_t)g.V * sizeof(double2);
+  const char* base = reinterpret_cast<const char*>(U) + (size_t)(d * 9) * es;
+  unsigned off = (unsigned)i * (unsigned)sizeof(double2);
+  asm volatile("" : "+v"(off) : "v"(dep));
+  M m;
+#pragma unroll
+  for (int e = 0; e < (R12 ? 6 : 9); ++e) {
+    const double2 v = *reinterpret_cast<const double2*>(base + e * es + off);
+    m.e[e] = {v.x, v.y};
+  }
+  if constexpr (R12) {
+    m.e[6] = conj(m.e[1] * m.e[5] - m.e[2] * m.e[4]);
+    m.e[7] = conj(m.e[2] * m.e[3] - m.e[0] * m.e[5]);
+    m.e[8] = conj(m.e[0] * m.e[4] - m.e[1] * m.e[3]);
+  }
+  return m;
+}
+
+// a += s b (fused multiply-adds)
+__device__ __forceinline__ void maxpy(M& a, double s, const M& b) {
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    a.e[k].re = fma(s, b.e[k].re, a.e[k].re);
+    a.e[k].im = fma(s, b.e[k].im, a.e[k].im);
+  }
+}
+
+__device__ __forceinline__ Site4 step(const Geom& g, Site4 x, int d, int s) {
+  x.move(g, d, s);
+  return x;
+}
+
+// The link of the step from x along direction d with sign S: U_d(x) for
+// S > 0, U_d(x - d)^dag for S < 0; x moves to x + S d.  t <- t L, or t = L
+// for the first step of a path.  The sign is a compile-time constant: a
+// runtime sign would make the compiler evaluate both branches.
+// Software pipelining of a path with prefetch distance one: the load of a
+// step waits for `dep`, an element of the partial product as it stood before
+// the previous step, so the link of step k + 1 can be in flight while step k
+// multiplies, but no later link is loaded early.
+template <int S, bool R12>
+__device__ __forceinline__ void mul_step(M& t, const double2* __restrict__ U, const Geom& g, Site4& x, int d,
+                                         double2& dep) {
+  if constexpr (S > 0) {
+    const M m = ldLs_after<R12>(U, g, x.index(g), d, dep.y);
+    x.move_f(g, d, +1);
+    dep.y = dep.x;
+    dep.x = t.e[8].im;
+    rmul_nf(t, m);
+  } else {
+    x.move_f(g, d, -1);
+    const M m = ldLs_after<R12>(U, g, x.index(g), d, dep.y);
+    dep.y = dep.x;
+    dep.x = t.e[8].im;
+    rmul_df(t, m);
+  }
+}
+// the first link of a path
+template <int S, bool R12>
+__device__ __forceinline__ M first_step(const double2* __restrict__ U, const Geom& g, Site4& x, int d, double2& dep) {
+  if constexpr (S > 0) {
+    const M m = ldLs_after<R12>(U, g, x.index(g), d, dep.y);
+    x.move_f(g, d, +1);
+    return m;
+  } else {
+    x.move_f(g, d, -1);
+    return mdag(ldLs_after<R12>(U, g, x.index(g), d, dep.y));
+  }
+}
+
+// One half of the staple sum of plane (mu, nu) (see staples_fused), added to A.
+template <int S, bool R12>
+__device__ __forceinline__ void staple_half(M& A, const double2* __restrict__ U, const Geom& g, const Site4& nm,
+                                            int mu, int nu, double c11, double c12) {
+  double2 dep = make_double2(A.e[8].im, A.e[8].im);
+  Site4 x = nm;
+  M t = first_step<S, R12>(U, g, x, nu, dep);
+  mul_step<-1, R12>(t, U, g, x, mu, dep);  // a, x = n + S nu
+  const Site4 xa = x;
+  M Z = mscale(c11, t);
+  if (c12 != 0.0) {
+    mul_step<-1, R12>(t, U, g, x, mu, dep);
+    mul_step<-S, R12>(t, U, g, x, nu, dep);
+    mul_step<+1, R12>(t, U, g, x, mu, dep);
+    maxpy(A, c12, t);  // link second
+    x = nm;
+    t = first_step<+1, R12>(U, g, x, mu, dep);
+    mul_step<S, R12>(t, U, g, x, nu, dep);
+    mul_step<-1, R12>(t, U, g, x, mu, dep);
+    mul_step<-1, R12>(t, U, g, x, mu, dep);
+    maxpy(Z, c12, t);  // link first, up to its last step
+    x = nm;
+    t = first_step<S, R12>(U, g, x, nu, dep);
+    mul_step<S, R12>(t, U, g, x, nu, dep);
+    mul_step<-1, R12>(t, U, g, x, mu, dep);
+    mul_step<-S, R12>(t, U, g, x, nu, dep);
+    maxpy(Z, c12, t);  // 2 x 1 in nu, up to its last step
+  }
+  x = xa;
+  mul_step<-S, R12>(Z, U, g, x, nu, dep);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) A.e[k] = A.e[k] + Z.e[k];
+}
+
+// c11 A1 + c12 A2 for link (n, mu), the staple sums of staples() with the
+// paths of each plane grouped by shared factors (Lattice.cpp:249-507).  The
+// eight paths of plane (mu, nu) split into the four that leave n + mu
+// towards +nu or end on U_nu(n)^dag (the plaquette above and rectangle types
+// 2, 4 and 6) and the four mirrored ones towards -nu (below, types 1, 5 and
+// 3); with S = +1 or -1 for the two halves and steps written (direction,
+// sign) from n + mu:
+//   plaquette   (nu, S) (mu, -) (nu, -S)
+//   2 x 1 in nu (nu, S) (nu, S) (mu, -) (nu, -S) (nu, -S)
+//   link first  (mu, +) (nu, S) (mu, -) (mu, -) (nu, -S)
+//   link second (nu, S) (mu, -) (mu, -) (nu, -S) (mu, +)
+// so with a = (nu, S) (mu, -), the half sums to
+//   [c11 a + c12 (mu, +) (nu, S) (mu, -) (mu, -) + c12 (nu, S) (nu, S) (mu, -) (nu, -S)] (nu, -S)
+//   + c12 a (mu, -) (nu, -S) (mu, +):
+// 11 SU(3) products and 14 link loads per half instead of 14 and 18.  Every
+// product keeps its path's left-to-right order; only the sums are regrouped,
+// so the result equals staples() to rounding.  mu is uniform over the
+// workgroup and nu over the loop, so every link base stays scalar.
+template <bool R12 = false>
+__device__ __forceinline__ M staples_fused(const double2* __restrict__ U, const Geom& g, const Site4& n, int mu,
+                                          double c11, double c12) {
+  M A = mzero<double>();
+  Site4 nm = n;
+  nm.move_f(g, mu, +1);  // n + mu
+  // one half per loop iteration, the two signs in loops of their own: a
+  // branch between the halves inside one loop kept both halves' values live
+#pragma unroll 1
+  for (int nu = 0; nu < 4; ++nu) {
+    if (nu == mu) continue;
+    staple_half<+1, R12>(A, U, g, nm, mu, nu, c11, c12);
+  }
+#pragma unroll 1
+  for (int nu = 0; nu < 4; ++nu) {
+    if (nu == mu) continue;
+    staple_half<-1, R12>(A, U, g, nm, mu, nu, c11, c12);
   }
   return A;
 }

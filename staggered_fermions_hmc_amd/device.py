@@ -73,6 +73,10 @@ class Context:
     def set_profiling(self, on):
         self.check(self._lib.sfh_set_profiling(self.h, int(on)))
 
+    def set_graph_replay(self, on):
+        """True (default): CG chunks replay as HIP graphs; False: direct launches."""
+        self.check(self._lib.sfh_set_graph_replay(self.h, int(bool(on))))
+
     def kernel_time(self, name):
         ms = ctypes.c_double()
         n = ctypes.c_long()
