@@ -20,8 +20,8 @@ constexpr int kBatchChunk = 16;
 // Launch shape of the batched hop, chosen by the autotuner.  hop_threads
 // (128, 256 or 512 lanes, one site each, half per parity) applies to the
 // first hop of an iteration, t = D^dag p, which has no reduction; rhs_tile
-// (1, 2, 4 or 8) is how many right-hand sides of a system one block applies
-// its links to, for both hops.  Neither changes any solve's arithmetic or
+// (1, 2, 4 or 8) is how many right-hand sides of a system one lane applies
+// each loaded link to (their accumulators held in registers), for both hops.  Neither changes any solve's arithmetic or
 // reduction order: the second hop (with the reduction) always runs 256 lanes
 // per block.
 struct BatchConfig {

@@ -2,10 +2,11 @@
 // and up to 8 right-hand sides per lattice sharing its links) advanced
 // together, one launch per CG phase for every active solve.
 //
-// * The hop kernel walks a block map of (system, block) pairs, one lane per
-//   site; each block applies its links to every active right-hand side of
-//   its system in turn (multi-RHS: the links come from HBM once per block,
-//   from cache for the other right-hand sides).
+// * The hop kernel walks a block map of (system, block, right-hand-side
+//   tile) entries, one lane per site; a lane loads each of its links once
+//   and applies it to every active right-hand side of its tile, whose
+//   accumulators it keeps in registers (multi-RHS: the links cross HBM once
+//   per tile).
 // * The vector kernels walk a map of (slot, block) pairs, a slot being one
 //   right-hand side in flight.
 // * Every slot has its own device CGState and reduction partials, and a
@@ -40,44 +41,36 @@ namespace {
 constexpr int kBlas = 256;
 template <class T> constexpr bool kMixedT = std::is_same<T, float>::value;
 
-// Neighbour indices and signs of one site in all four directions.
-template <class R> struct SiteNb {
-  int fw[4], bw[4];
-  R sf[4], sb[4];
-};
-
-// acc = sum_mu eta [bup U_mu(n) in(n+mu) - bdn U_mu(n-mu)^dag in(n-mu)];
-// R12: SU(3) links read as two rows.
-template <class T, bool R12 = false, class LT>
-__device__ __forceinline__ void site_hop(const LT* __restrict__ U, const typename Prec<T>::S* __restrict__ in, int V,
-                                         int i, int q, const SiteNb<typename Prec<T>::R>& nb,
-                                         vec3<typename Prec<T>::R>& acc) {
-  using R = typename Prec<T>::R;
-#pragma unroll
-  for (int c = 0; c < 3; ++c) acc.c[c] = {R(0), R(0)};
-#pragma unroll
-  for (int mu = 0; mu < 4; ++mu) {
-    mat3<R> Uf, Ub;
-    vec3<R> vf, vb;
-    load_link<T, R12>(U, V, mu, i, Uf);
-    load_spinor<T>(in, nullptr, V, q + nb.fw[mu], vf);
-    load_link<T, R12>(U, V, mu, q + nb.bw[mu], Ub);
-    load_spinor<T>(in, nullptr, V, q + nb.bw[mu], vb);
-    acc_mv(acc, Uf, vf, nb.sf[mu]);
-    acc_mdv(acc, Ub, vb, -nb.sb[mu]);
-  }
+// Block-map entry of this workgroup: workgroups reach the 8 XCDs round-robin,
+// so runs of kRun consecutive entries go to one XCD (neighbouring ranges of a
+// system share its L2) while the runs themselves alternate between XCDs, so
+// entries of very different cost (1 to 8 right-hand sides) stay spread over
+// all of them.  A bijection on [0, nb): the last nb mod (8 kRun) entries keep
+// their own index.
+constexpr int kRun = 16;
+__device__ __forceinline__ int map_entry(int nb) {
+  const int k = blockIdx.x;
+  if (k >= nb - nb % (8 * kRun)) return k;
+  const int x = k & 7, j = k >> 3;
+  return ((j / kRun) * 8 + x) * kRun + j % kRun;
 }
 
-// One lane per site over (system, block) pairs; each block covers one
-// checkerboard range on both parities (as the single-system hop) and applies
-// the hop to every active right-hand side of its system in turn: the links
-// come from HBM once and from cache for the other right-hand sides.
+// One lane per site over (system, block, right-hand-side tile) entries; each
+// block covers one checkerboard range on both parities (as the single-system
+// hop).  A lane loads the two links of a direction once and applies them to
+// every active right-hand side of its tile (KT at most), whose accumulators
+// it holds in registers: the links cross HBM once per tile, not once per
+// right-hand side.  Each right-hand side's sum runs over the directions in
+// one fixed order (forward, then backward term), so the tile changes no
+// result.
 // phase DOT=false: t = D^dag p ; DOT=true: v = D t and <p, v>.
-template <class T, bool DOT, bool R12>
+template <class T, bool DOT, bool R12, int KT>
 __global__ __launch_bounds__(512) void kb_hop(const BSys<T>* __restrict__ sys, const BSlot<T>* __restrict__ slots,
                                               const int4* __restrict__ bmap, typename Prec<T>::R hb) {
   using R = typename Prec<T>::R;
-  const int4 m = bmap[blockIdx.x];  // system, block, right-hand sides [z, w) of the system
+  using SP = typename Prec<T>::S;
+  // system, block, right-hand sides [z, w) of the system
+  const int4 m = bmap[map_entry(gridDim.x)];
   const BSys<T>& S = sys[m.x];  // read in place: a private copy with a dynamically indexed slot[] spills
   const int lb = m.y;
   const Geom& g = S.g;
@@ -88,33 +81,59 @@ __global__ __launch_bounds__(512) void kb_hop(const BSys<T>* __restrict__ sys, c
   const int V = g.V;
   const int i = p * g.Vh + (valid ? cb : 0);
   const int q = (1 - p) * g.Vh;
-  SiteNb<R> nb;
+  const int nj = m.w - m.z;
+  __shared__ int sdone[KT];
+  if (threadIdx.x < KT) sdone[threadIdx.x] = threadIdx.x < nj ? slots[S.slot[m.z + threadIdx.x]].st->done : 1;
+  __syncthreads();
+  bool act[KT];
+  const SP* in[KT];
+#pragma unroll
+  for (int j = 0; j < KT; ++j) {
+    act[j] = !sdone[j];
+    const BSlot<T>& L = slots[S.slot[m.z + (act[j] ? j : 0)]];
+    in[j] = DOT ? L.t : L.p;
+  }
+  vec3<R> acc[KT];
+#pragma unroll
+  for (int j = 0; j < KT; ++j)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc[j].c[c] = {R(0), R(0)};
   if (valid) {
     int n[4];
     cb_coords(g, cb, p, n);
 #pragma unroll
-    for (int mu = 0; mu < 4; ++mu) neighbours(g, cb, n, mu, nb.fw[mu], nb.bw[mu], nb.sf[mu], nb.sb[mu]);
+    for (int mu = 0; mu < 4; ++mu) {
+      int fw, bw;
+      R sf, sb;
+      neighbours(g, cb, n, mu, fw, bw, sf, sb);
+      mat3<R> Uf, Ub;
+      load_link<T, R12>(S.U, V, mu, i, Uf);
+      load_link<T, R12>(S.U, V, mu, q + bw, Ub);
+#pragma unroll
+      for (int j = 0; j < KT; ++j) {
+        if (!act[j]) continue;
+        vec3<R> vf, vb;
+        load_spinor<T>(in[j], nullptr, V, q + fw, vf);
+        load_spinor<T>(in[j], nullptr, V, q + bw, vb);
+        acc_mv(acc[j], Uf, vf, sf);
+        acc_mdv(acc[j], Ub, vb, -sb);
+      }
+    }
   }
-  __shared__ int sdone;
-  for (int j = m.z; j < m.w; ++j) {
-    const BSlot<T> L = slots[S.slot[j]];
-    if (threadIdx.x == 0) sdone = L.st->done;
-    __syncthreads();
-    const int dn = sdone;
-    __syncthreads();
-    if (dn) continue;
-    const typename Prec<T>::S* in = DOT ? L.t : L.p;
-    typename Prec<T>::S* out = DOT ? L.v : L.t;
+#pragma unroll
+  for (int j = 0; j < KT; ++j) {
+    if (!act[j]) continue;
+    const BSlot<T>& L = slots[S.slot[m.z + j]];
+    SP* out = DOT ? L.v : L.t;
     // mixed precision: <p,v>, <r,v>, <v,v>, <r,r> for FinFused; FP64: <p,v>
     double d[4] = {0.0, 0.0, 0.0, 0.0};
     if (valid) {
-      vec3<R> acc, self, res;
-      site_hop<T, R12>(S.U, in, V, i, q, nb, acc);
-      load_spinor<T>(in, nullptr, V, i, self);
+      vec3<R> self, res;
+      load_spinor<T>(in[j], nullptr, V, i, self);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        res.c[c].re = S.mass * self.c[c].re + hb * acc.c[c].re;
-        res.c[c].im = S.mass * self.c[c].im + hb * acc.c[c].im;
+        res.c[c].re = S.mass * self.c[c].re + hb * acc[j].c[c].re;
+        res.c[c].im = S.mass * self.c[c].im + hb * acc[j].c[c].im;
       }
       store_spinor<T>(out, nullptr, V, i, res);
       if constexpr (DOT) {
@@ -138,6 +157,18 @@ __global__ __launch_bounds__(512) void kb_hop(const BSys<T>* __restrict__ sys, c
       if constexpr (kMixedT<T>) grid_reduce_n<4>(d, L.part, ctr_of(L.st->ctr, CNT_PV), S.nb_hop, lb, FinFused{L.st});
       else grid_reduce(d[0], L.part, ctr_of(L.st->ctr, CNT_PV), S.nb_hop, lb, FinPV{L.st});
     }
+  }
+}
+
+// Launch of the batched hop with the tile length as a compile-time bound.
+template <class T, bool DOT, bool R12>
+void launch_kb_hop(int tile, int nblocks, int threads, hipStream_t st, const BSys<T>* sys, const BSlot<T>* slots,
+                   const int4* bmap, typename Prec<T>::R hb) {
+  switch (tile) {
+    case 1: hipLaunchKernelGGL((kb_hop<T, DOT, R12, 1>), dim3(nblocks), dim3(threads), 0, st, sys, slots, bmap, hb); break;
+    case 2: hipLaunchKernelGGL((kb_hop<T, DOT, R12, 2>), dim3(nblocks), dim3(threads), 0, st, sys, slots, bmap, hb); break;
+    case 4: hipLaunchKernelGGL((kb_hop<T, DOT, R12, 4>), dim3(nblocks), dim3(threads), 0, st, sys, slots, bmap, hb); break;
+    default: hipLaunchKernelGGL((kb_hop<T, DOT, R12, 8>), dim3(nblocks), dim3(threads), 0, st, sys, slots, bmap, hb); break;
   }
 }
 
@@ -252,43 +283,89 @@ __global__ __launch_bounds__(kBlas) void kb_ru_acc(const BSlot<T>* __restrict__ 
   }
 }
 
-template <class T, int PHASE, bool R12>
+// FP64 hops of the reliable update (t64 = D^dag x64, then ax = D t64) for
+// every slot of the tile that has one pending, with the same register tile
+// as kb_hop over the FP64 links.
+template <class T, int PHASE, bool R12, int KT>
 __global__ __launch_bounds__(256) void kb_ru_hop(const BSys<T>* __restrict__ sys, const BSlot<T>* __restrict__ slots,
                                                  const int4* __restrict__ bmap) {
-  const int4 m = bmap[blockIdx.x];
+  const int4 m = bmap[map_entry(gridDim.x)];
   const BSys<T>& S = sys[m.x];  // read in place: a private copy with a dynamically indexed slot[] spills
+  const int nj = m.w - m.z;
+  __shared__ int spend[KT];
+  if (threadIdx.x < KT) spend[threadIdx.x] = threadIdx.x < nj ? slots[S.slot[m.z + threadIdx.x]].st->ru_pending : 0;
+  __syncthreads();
+  bool act[KT];
   int any = 0;
-  for (int j = m.z; j < m.w; ++j) any |= slots[S.slot[j]].st->ru_pending;
+#pragma unroll
+  for (int j = 0; j < KT; ++j) {
+    act[j] = spend[j] != 0;
+    any |= spend[j];
+  }
   if (!any) return;  // uniform over the block
   const int lb = m.y;
   const Geom& g = S.g;
   const int hw = blockDim.x >> 1;
   const int p = threadIdx.x >= hw;
   const int cb = range_start(g, lb, hw) + (threadIdx.x - p * hw);
-  const bool valid = cb < g.Vh;
+  if (cb >= g.Vh) return;  // no barrier below
   const int V = g.V;
-  const int i = p * g.Vh + (valid ? cb : 0);
+  const int i = p * g.Vh + cb;
   const int q = (1 - p) * g.Vh;
-  SiteNb<double> nb;
-  if (valid) {
-    int n[4];
-    cb_coords(g, cb, p, n);
+  const double2* in[KT];
 #pragma unroll
-    for (int mu = 0; mu < 4; ++mu) neighbours(g, cb, n, mu, nb.fw[mu], nb.bw[mu], nb.sf[mu], nb.sb[mu]);
+  for (int j = 0; j < KT; ++j) {
+    const BSlot<T>& L = slots[S.slot[m.z + (act[j] ? j : 0)]];
+    in[j] = PHASE == 0 ? L.x64 : L.t64;
+  }
+  vec3<double> acc[KT];
+#pragma unroll
+  for (int j = 0; j < KT; ++j)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc[j].c[c] = {0.0, 0.0};
+  int n[4];
+  cb_coords(g, cb, p, n);
+#pragma unroll
+  for (int mu = 0; mu < 4; ++mu) {
+    int fw, bw;
+    double sf, sb;
+    neighbours(g, cb, n, mu, fw, bw, sf, sb);
+    mat3<double> Uf, Ub;
+    load_link<double, R12>(S.U64, V, mu, i, Uf);
+    load_link<double, R12>(S.U64, V, mu, q + bw, Ub);
+#pragma unroll
+    for (int j = 0; j < KT; ++j) {
+      if (!act[j]) continue;
+      vec3<double> vf, vb;
+      load_spinor<double>(in[j], nullptr, V, q + fw, vf);
+      load_spinor<double>(in[j], nullptr, V, q + bw, vb);
+      acc_mv(acc[j], Uf, vf, sf);
+      acc_mdv(acc[j], Ub, vb, -sb);
+    }
   }
   const double hb = PHASE == 0 ? -0.5 : 0.5;
-  for (int j = m.z; j < m.w; ++j) {
-    const BSlot<T> L = slots[S.slot[j]];
-    if (!valid || !L.st->ru_pending) continue;  // no barrier in this loop
-    const double2* in = PHASE == 0 ? L.x64 : L.t64;
+#pragma unroll
+  for (int j = 0; j < KT; ++j) {
+    if (!act[j]) continue;
+    const BSlot<T>& L = slots[S.slot[m.z + j]];
     double2* out = PHASE == 0 ? L.t64 : L.ax;
-    vec3<double> acc, self;
-    site_hop<double, R12>(S.U64, in, V, i, q, nb, acc);
-    load_spinor<double>(in, nullptr, V, i, self);
+    vec3<double> self;
+    load_spinor<double>(in[j], nullptr, V, i, self);
 #pragma unroll
     for (int c = 0; c < 3; ++c)
-      out[(size_t)c * V + i] = make_double2(S.mass64 * self.c[c].re + hb * acc.c[c].re,
-                                            S.mass64 * self.c[c].im + hb * acc.c[c].im);
+      out[(size_t)c * V + i] = make_double2(S.mass64 * self.c[c].re + hb * acc[j].c[c].re,
+                                            S.mass64 * self.c[c].im + hb * acc[j].c[c].im);
+  }
+}
+
+template <class T, int PHASE, bool R12>
+void launch_kb_ru_hop(int tile, int nblocks, hipStream_t st, const BSys<T>* sys, const BSlot<T>* slots,
+                      const int4* bmap) {
+  switch (tile) {
+    case 1: hipLaunchKernelGGL((kb_ru_hop<T, PHASE, R12, 1>), dim3(nblocks), dim3(256), 0, st, sys, slots, bmap); break;
+    case 2: hipLaunchKernelGGL((kb_ru_hop<T, PHASE, R12, 2>), dim3(nblocks), dim3(256), 0, st, sys, slots, bmap); break;
+    case 4: hipLaunchKernelGGL((kb_ru_hop<T, PHASE, R12, 4>), dim3(nblocks), dim3(256), 0, st, sys, slots, bmap); break;
+    default: hipLaunchKernelGGL((kb_ru_hop<T, PHASE, R12, 8>), dim3(nblocks), dim3(256), 0, st, sys, slots, bmap); break;
   }
 }
 
@@ -363,16 +440,13 @@ void Engine<T>::launch_chunk() {
   // FP64: the reference recurrence, hop, hop + <p,v>, x/r update, p update
   for (int it = 0; it < chunk_; ++it) {
     if constexpr (kMixed) hipLaunchKernelGGL((kb_upd<T>), dim3(nbb_), dim3(kBlas), 0, c_->stream, dslot_, dmb_);
+    const int tile = cfg_.rhs_tile;
     if (r12_) {
-      hipLaunchKernelGGL((kb_hop<T, false, true>), dim3(nbh0_), dim3(cfg_.hop_threads), 0, c_->stream, dsys_,
-                         dslot_, dmh0_, (R)(-0.5));
-      hipLaunchKernelGGL((kb_hop<T, true, true>), dim3(nbh_), dim3(256), 0, c_->stream, dsys_, dslot_, dmh_,
-                         (R)(0.5));
+      launch_kb_hop<T, false, true>(tile, nbh0_, cfg_.hop_threads, c_->stream, dsys_, dslot_, dmh0_, (R)(-0.5));
+      launch_kb_hop<T, true, true>(tile, nbh_, 256, c_->stream, dsys_, dslot_, dmh_, (R)(0.5));
     } else {
-      hipLaunchKernelGGL((kb_hop<T, false, false>), dim3(nbh0_), dim3(cfg_.hop_threads), 0, c_->stream, dsys_, dslot_,
-                         dmh0_, (R)(-0.5));
-      hipLaunchKernelGGL((kb_hop<T, true, false>), dim3(nbh_), dim3(256), 0, c_->stream, dsys_, dslot_, dmh_,
-                         (R)(0.5));
+      launch_kb_hop<T, false, false>(tile, nbh0_, cfg_.hop_threads, c_->stream, dsys_, dslot_, dmh0_, (R)(-0.5));
+      launch_kb_hop<T, true, false>(tile, nbh_, 256, c_->stream, dsys_, dslot_, dmh_, (R)(0.5));
     }
     if constexpr (!kMixed) {
       hipLaunchKernelGGL((kb_xr<T>), dim3(nbb_), dim3(kBlas), 0, c_->stream, dslot_, dmb_);
@@ -388,11 +462,11 @@ void Engine<T>::launch_tail() {
   if constexpr (kMixed) {
     hipLaunchKernelGGL((kb_ru_acc<T>), dim3(nbb_), dim3(kBlas), 0, c_->stream, dslot_, dmb_);
     if (r12_64_) {
-      hipLaunchKernelGGL((kb_ru_hop<T, 0, true>), dim3(nbh_), dim3(256), 0, c_->stream, dsys_, dslot_, dmh_);
-      hipLaunchKernelGGL((kb_ru_hop<T, 1, true>), dim3(nbh_), dim3(256), 0, c_->stream, dsys_, dslot_, dmh_);
+      launch_kb_ru_hop<T, 0, true>(cfg_.rhs_tile, nbh_, c_->stream, dsys_, dslot_, dmh_);
+      launch_kb_ru_hop<T, 1, true>(cfg_.rhs_tile, nbh_, c_->stream, dsys_, dslot_, dmh_);
     } else {
-      hipLaunchKernelGGL((kb_ru_hop<T, 0, false>), dim3(nbh_), dim3(256), 0, c_->stream, dsys_, dslot_, dmh_);
-      hipLaunchKernelGGL((kb_ru_hop<T, 1, false>), dim3(nbh_), dim3(256), 0, c_->stream, dsys_, dslot_, dmh_);
+      launch_kb_ru_hop<T, 0, false>(cfg_.rhs_tile, nbh_, c_->stream, dsys_, dslot_, dmh_);
+      launch_kb_ru_hop<T, 1, false>(cfg_.rhs_tile, nbh_, c_->stream, dsys_, dslot_, dmh_);
     }
     hipLaunchKernelGGL((kb_ru_sub<T>), dim3(nbb_), dim3(kBlas), 0, c_->stream, dslot_, dmb_);
     hipLaunchKernelGGL((kb_ru_apply<T>), dim3(nbb_), dim3(kBlas), 0, c_->stream, dslot_, dmb_);
@@ -640,8 +714,9 @@ int Engine<T>::admit(int sys, const double2* b, double2* x, long long ticket) {
 
 // Block map of one hop: (system, block, right-hand-side range) entries, one
 // lane per site at `threads` lanes per block, right-hand sides in tiles of
-// `tile`; the tiles of one block index are adjacent, so a block's links are
-// still in L2 for the next tile.
+// `tile` (a lane holds a tile's accumulators in registers and loads its links
+// once per tile); the tiles of one block index are adjacent, so a block's
+// links are still in L2 for the next tile.
 template <class T>
 void Engine<T>::hop_map(int threads, int tile, std::vector<int4>& out) const {
   out.clear();
@@ -753,11 +828,11 @@ int Engine<T>::time_hop(const BatchConfig& cfg, int n, double* us) {
   hipMemcpyAsync(dm, m.data(), m.size() * sizeof(int4), hipMemcpyHostToDevice, c_->stream);
   auto launch = [&]() {
     if (r12_)
-      hipLaunchKernelGGL((kb_hop<T, false, true>), dim3((int)m.size()), dim3(cfg.hop_threads), 0, c_->stream, dsys_,
-                         dslot_, dm, (R)(-0.5));
+      launch_kb_hop<T, false, true>(cfg.rhs_tile, (int)m.size(), cfg.hop_threads, c_->stream, dsys_, dslot_, dm,
+                                    (R)(-0.5));
     else
-      hipLaunchKernelGGL((kb_hop<T, false, false>), dim3((int)m.size()), dim3(cfg.hop_threads), 0, c_->stream, dsys_,
-                         dslot_, dm, (R)(-0.5));
+      launch_kb_hop<T, false, false>(cfg.rhs_tile, (int)m.size(), cfg.hop_threads, c_->stream, dsys_, dslot_, dm,
+                                     (R)(-0.5));
   };
   launch();
   hipEventRecord(c_->ev0, c_->stream);
