@@ -327,7 +327,7 @@ int sfh_avg_plaq(sfh_ctx* ctx, const sfh_gauge* g, double* out);
 /* Polyakov loop trace Tr prod_t U_3(x,y,z,t) at every spatial site, complex
  * (re, im) pairs at [(x*L1 + y)*L2 + z].  Replaces precompute_polyakov_grid /
  * poly_loop_at_spat_coord (Lattice.cpp:849-884); the correlators of
- * Lattice.cpp:888-948 are cheap host sums over this grid. */
+ * Lattice.cpp:886-948 are sums over this grid. */
 int sfh_polyakov_grid(sfh_ctx* ctx, const sfh_gauge* g, double* out);
 /* Polyakov-loop correlators of the device-resident links:
  *   C(r^2) = (1/N) sum_m sum_{d in D(r^2)} P(m) conj(P(m + d)),
@@ -337,12 +337,12 @@ int sfh_polyakov_grid(sfh_ctx* ctx, const sfh_gauge* g, double* out);
  * L_k, N = (L0 L1 L2) |D(r^2)|, and C = 0 when D(r^2) is empty.
  * out[2k], out[2k+1] = Re, Im C(r2_first + k) for k < count.  Replaces the
  * measurement step precompute_polyakov_grid + correlator_over_fixed_distance_fast
- * (Lattice.cpp:868-948) as Main.cpp:219-228 calls it (r2_first = 1,
+ * (Lattice.cpp:869-948) as Main.cpp:219-228 calls it (r2_first = 1,
  * count = maxdistance). */
 int sfh_polyakov_correlators(sfh_ctx* ctx, const sfh_gauge* g, int r2_first, int count, double* out);
 /* The same correlators of a host grid (complex (re, im) pairs at
  * [(x*L1 + y)*L2 + z], spatial = {L0, L1, L2}): replaces
- * correlator_over_fixed_distance_fast (Lattice.cpp:888-948) on the grid
+ * correlator_over_fixed_distance_fast (Lattice.cpp:886-948) on the grid
  * that precompute_polyakov_grid returns. */
 int sfh_polyakov_correlators_grid(sfh_ctx* ctx, const int spatial[3], const double* grid, int r2_first, int count,
                                   double* out);

@@ -638,7 +638,7 @@ int oc_polyakov_grid(const int* dims, const double* U, double* out) {
   return OK;
 }
 
-// correlator_over_fixed_distance_fast, Lattice.cpp:888-948: the
+// correlator_over_fixed_distance_fast, Lattice.cpp:886-948: the
 // displacements (dx, dy, dz) in ascending loop order with
 // |d_k| <= min(floor(sqrt(r^2)), L_k / 2), d_k = L_k / 2 skipped for even L_k
 // and d.d = r^2 (:892-913, one Spatial_Size generalised to L0, L1, L2); then

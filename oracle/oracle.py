@@ -214,7 +214,7 @@ def polyakov_grid(dims, U):
 
 
 def polyakov_correlator(spatial, grid, r_squared):
-    """correlator_over_fixed_distance_fast (Lattice.cpp:888-948) of a complex
+    """correlator_over_fixed_distance_fast (Lattice.cpp:886-948) of a complex
     grid [(x*L1 + y)*L2 + z] for one r^2, in the reference's summation order."""
     sp = np.ascontiguousarray(np.asarray(spatial, dtype=np.int32))
     g = np.ascontiguousarray(np.asarray(grid, dtype=np.complex128)).view(np.float64)

@@ -502,7 +502,7 @@ std::vector<complex> precompute_polyakov_grid(const Link_array& arr) {
   return grid;
 }
 
-// Lattice.cpp:888-948, summed on the device.
+// Lattice.cpp:886-948, summed on the device.
 complex correlator_over_fixed_distance_fast(const std::vector<complex>& P_grid, int r_squared) {
   const int L = config().Spatial_Size;
   if ((long)P_grid.size() != (long)L * L * L) throw std::invalid_argument("Polyakov grid has the wrong size");

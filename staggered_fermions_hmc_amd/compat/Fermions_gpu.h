@@ -82,7 +82,7 @@ std::vector<Link_array> Apply_4HEX_smearing_with_history(const Link_array& U_thi
 // Measurement step of Main.cpp:219-228 (Lattice.h:84-85): the Polyakov grid
 // Tr prod_t U_3 at every spatial site [(x*L + y)*L + z] (Lattice.cpp:868-884)
 // and the correlator over the displacements with d.d = r_squared
-// (Lattice.cpp:888-948), both computed on the device.
+// (Lattice.cpp:886-948), both computed on the device.
 std::vector<complex> precompute_polyakov_grid(const Link_array& arr);
 complex correlator_over_fixed_distance_fast(const std::vector<complex>& P_grid, int r_squared);
 

@@ -203,7 +203,7 @@ int polyakov_grid(sfh_ctx* c, const Geom& g, const double2* U, double* out) {
 // ------------------------------------------------------------------ Polyakov-loop correlators
 // C(r^2) = (1/N) sum_m sum_{d in D(r^2)} P(m) conj(P(m + d)), N = |grid| |D(r^2)|,
 // with D(r^2) the integer displacements of correlator_over_fixed_distance_fast
-// (Lattice.cpp:888-948): |d_k| <= min(floor(sqrt(r^2)), L_k / 2), d_k = +L_k / 2
+// (Lattice.cpp:886-948): |d_k| <= min(floor(sqrt(r^2)), L_k / 2), d_k = +L_k / 2
 // left out (it is the same neighbour as -L_k / 2), d . d = r^2; C = 0 when D
 // is empty.  One lane per grid point m and one block row per r^2; each lane
 // sums its displacements in table order, blocks reduce in a fixed order, and a
