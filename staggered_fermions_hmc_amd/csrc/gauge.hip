@@ -106,6 +106,8 @@ __global__ __launch_bounds__(128) void k_force(Geom g, const double2* __restrict
 
 int force_field(sfh_ctx* c, const Geom& g, const double2* U, const double2* X, const double2* Y, double beta,
                 double c11, double c12, unsigned which, double2* F, double2* P, double coef, bool r12) {
+  for (int d = 0; d < 4; ++d)  // the staple walks pack each coordinate into 8 bits
+    if (g.L[d] > 255) return set_err(c, SFH_E_INVALID, "force: lattice extents above 255");
   KTimer kt(c, "force");
   const int nb = 4 * ((g.Vh + kRangeHalf - 1) / kRangeHalf);
   if (r12)

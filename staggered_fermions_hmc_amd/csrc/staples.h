@@ -341,6 +341,132 @@ __device__ __forceinline__ Site4 step(const Geom& g, Site4 x, int d, int s) {
   return x;
 }
 
+// A site of the force's staple walks: the four coordinates packed 8 bits
+// each (x_d in bits 8d .. 8d + 7; every extent is below 256), the
+// lexicographic index and the parity's offset p V/2, all carried along by
+// every move.  The direction of a move is wave-uniform, so its extent and
+// stride are scalar selects; a move then costs about ten vector
+// instructions and a link index two, where recomputing the index from four
+// coordinates after each move took about twenty-five.
+struct SiteW {
+  unsigned c;
+  int lex, pv;
+  __device__ __forceinline__ static SiteW of(const Geom& g, const Site4& n) {
+    SiteW w;
+    w.c = (unsigned)n.x0 | ((unsigned)n.x1 << 8) | ((unsigned)n.x2 << 16) | ((unsigned)n.x3 << 24);
+    w.lex = ((n.x0 * g.L[1] + n.x1) * g.L[2] + n.x2) * g.L[3] + n.x3;
+    w.pv = ((n.x0 + n.x1 + n.x2 + n.x3) & 1) ? g.Vh : 0;
+    return w;
+  }
+  template <int S> __device__ __forceinline__ void move(const Geom& g, int d) {
+    const bool f0 = d == 0, f1 = d == 1, f2 = d == 2, f3 = d == 3;
+    const int L = (f0 ? g.L[0] : 0) + (f1 ? g.L[1] : 0) + (f2 ? g.L[2] : 0) + (f3 ? g.L[3] : 0);
+    const int st = (f0 ? 2 * g.S0 : 0) + (f1 ? 2 * g.S1 : 0) + (f2 ? 2 * g.S2 : 0) + (f3 ? 1 : 0);
+    const unsigned sh = 8u * (unsigned)d;
+    int v = (int)((c >> sh) & 255u) + S;
+    // the wrap as arithmetic on a 0/1 flag: selects on it here turned into
+    // uniform-looking branch trees and the kernel spilled
+    const int w = S > 0 ? (v >= L ? 1 : 0) : (v < 0 ? 1 : 0);
+    v -= w * (S * L);
+    lex += S * st - w * (S * st * L);
+    c = (c & ~(255u << sh)) | ((unsigned)v << sh);
+    pv = g.Vh - pv;  // every move changes the parity (all extents are even)
+  }
+  __device__ __forceinline__ int index() const { return pv + (lex >> 1); }
+};
+
+// conj(a b - c d) with fused multiply-adds: one entry of the cross product
+// that rebuilds the third row (or column) of an SU(3) matrix.
+__device__ __forceinline__ C cross_conj(C a, C b, C c, C d) {
+  double re = a.re * b.re, im = a.re * b.im;
+  re = fma(-a.im, b.im, re);
+  im = fma(a.im, b.re, im);
+  re = fma(-c.re, d.re, re);
+  im = fma(-c.re, d.im, im);
+  re = fma(c.im, d.im, re);
+  im = fma(-c.im, d.re, im);
+  return {re, -im};
+}
+
+// Column 2 of an SU(3) matrix from columns 0 and 1: conj(col0 x col1).
+__device__ __forceinline__ void rebuild_col2(M& t) {
+  t.e[2] = cross_conj(t.e[3], t.e[7], t.e[6], t.e[4]);
+  t.e[5] = cross_conj(t.e[6], t.e[1], t.e[0], t.e[7]);
+  t.e[8] = cross_conj(t.e[0], t.e[4], t.e[3], t.e[1]);
+}
+
+// t <- t m and t <- t m^dag for SU(3) t (a staple path's partial product)
+// and an SU(3) link m of which rows 0 and 1 are loaded: columns 0 and 1 of
+// the product, then column 2 rebuilt from them.  t m^dag needs only rows 0
+// and 1 of m; t m needs m's (2,0) and (2,1) entries, rebuilt from its rows
+// 0 and 1.  96 and 112 FP64 instructions, against 108 for a full product
+// plus 24 to rebuild the link's third row.
+__device__ __forceinline__ void rmul_n_su3(M& t, const M& m) {
+  const C m6 = cross_conj(m.e[1], m.e[5], m.e[2], m.e[4]);
+  const C m7 = cross_conj(m.e[2], m.e[3], m.e[0], m.e[5]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const C a0 = t.e[3 * i], a1 = t.e[3 * i + 1], a2 = t.e[3 * i + 2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const C b0 = m.e[j], b1 = m.e[3 + j], b2 = j == 0 ? m6 : m7;
+      double re = a0.re * b0.re, im = a0.re * b0.im;
+      re = fma(-a0.im, b0.im, re);
+      im = fma(a0.im, b0.re, im);
+      re = fma(a1.re, b1.re, re);
+      re = fma(-a1.im, b1.im, re);
+      im = fma(a1.re, b1.im, im);
+      im = fma(a1.im, b1.re, im);
+      re = fma(a2.re, b2.re, re);
+      re = fma(-a2.im, b2.im, re);
+      im = fma(a2.re, b2.im, im);
+      im = fma(a2.im, b2.re, im);
+      t.e[3 * i + j] = {re, im};
+    }
+  }
+  rebuild_col2(t);
+}
+__device__ __forceinline__ void rmul_d_su3(M& t, const M& m) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const C a0 = t.e[3 * i], a1 = t.e[3 * i + 1], a2 = t.e[3 * i + 2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      // row j of m, conjugated
+      const C b0 = m.e[3 * j], b1 = m.e[3 * j + 1], b2 = m.e[3 * j + 2];
+      double re = a0.re * b0.re, im = -a0.re * b0.im;
+      re = fma(a0.im, b0.im, re);
+      im = fma(a0.im, b0.re, im);
+      re = fma(a1.re, b1.re, re);
+      re = fma(a1.im, b1.im, re);
+      im = fma(-a1.re, b1.im, im);
+      im = fma(a1.im, b1.re, im);
+      re = fma(a2.re, b2.re, re);
+      re = fma(a2.im, b2.im, re);
+      im = fma(-a2.re, b2.im, im);
+      im = fma(a2.im, b2.re, im);
+      t.e[3 * i + j] = {re, im};
+    }
+  }
+  rebuild_col2(t);
+}
+
+// Rows 0 and 1 of SU(3) link (i, d) (elements 0-5; the full-storage layout,
+// nine elements per direction), loaded after `dep` as ldLs_after does.
+__device__ __forceinline__ M ldLs2_after(const double2* __restrict__ U, const Geom& g, int i, int d, double dep) {
+  const size_t es = (size_t)g.V * sizeof(double2);
+  const char* base = reinterpret_cast<const char*>(U) + (size_t)(d * 9) * es;
+  unsigned off = (unsigned)i * (unsigned)sizeof(double2);
+  asm volatile("" : "+v"(off) : "v"(dep));
+  M m;
+#pragma unroll
+  for (int e = 0; e < 6; ++e) {
+    const double2 v = *reinterpret_cast<const double2*>(base + e * es + off);
+    m.e[e] = {v.x, v.y};
+  }
+  return m;
+}
+
 // The link of the step from x along direction d with sign S: U_d(x) for
 // S > 0, U_d(x - d)^dag for S < 0; x moves to x + S d.  t <- t L, or t = L
 // for the first step of a path.  The sign is a compile-time constant: a
@@ -349,45 +475,50 @@ __device__ __forceinline__ Site4 step(const Geom& g, Site4 x, int d, int s) {
 // step waits for `dep`, an element of the partial product as it stood before
 // the previous step, so the link of step k + 1 can be in flight while step k
 // multiplies, but no later link is loaded early.
-template <int S, bool R12>
-__device__ __forceinline__ void mul_step(M& t, const double2* __restrict__ U, const Geom& g, Site4& x, int d,
+// PURE: t is a product of links (SU(3) when R12), so with SU(3) links the
+// product takes the column-rebuild forms above.
+template <int S, bool R12, bool PURE = true>
+__device__ __forceinline__ void mul_step(M& t, const double2* __restrict__ U, const Geom& g, SiteW& x, int d,
                                          double2& dep) {
+  constexpr bool kSu3 = R12 && PURE;
   if constexpr (S > 0) {
-    const M m = ldLs_after<R12>(U, g, x.index(g), d, dep.y);
-    x.move_f(g, d, +1);
+    const M m = kSu3 ? ldLs2_after(U, g, x.index(), d, dep.y) : ldLs_after<R12>(U, g, x.index(), d, dep.y);
+    x.move<+1>(g, d);
     dep.y = dep.x;
     dep.x = t.e[8].im;
-    rmul_nf(t, m);
+    if constexpr (kSu3) rmul_n_su3(t, m);
+    else rmul_nf(t, m);
   } else {
-    x.move_f(g, d, -1);
-    const M m = ldLs_after<R12>(U, g, x.index(g), d, dep.y);
+    x.move<-1>(g, d);
+    const M m = kSu3 ? ldLs2_after(U, g, x.index(), d, dep.y) : ldLs_after<R12>(U, g, x.index(), d, dep.y);
     dep.y = dep.x;
     dep.x = t.e[8].im;
-    rmul_df(t, m);
+    if constexpr (kSu3) rmul_d_su3(t, m);
+    else rmul_df(t, m);
   }
 }
 // the first link of a path
 template <int S, bool R12>
-__device__ __forceinline__ M first_step(const double2* __restrict__ U, const Geom& g, Site4& x, int d, double2& dep) {
+__device__ __forceinline__ M first_step(const double2* __restrict__ U, const Geom& g, SiteW& x, int d, double2& dep) {
   if constexpr (S > 0) {
-    const M m = ldLs_after<R12>(U, g, x.index(g), d, dep.y);
-    x.move_f(g, d, +1);
+    const M m = ldLs_after<R12>(U, g, x.index(), d, dep.y);
+    x.move<+1>(g, d);
     return m;
   } else {
-    x.move_f(g, d, -1);
-    return mdag(ldLs_after<R12>(U, g, x.index(g), d, dep.y));
+    x.move<-1>(g, d);
+    return mdag(ldLs_after<R12>(U, g, x.index(), d, dep.y));
   }
 }
 
 // One half of the staple sum of plane (mu, nu) (see staples_fused), added to A.
 template <int S, bool R12>
-__device__ __forceinline__ void staple_half(M& A, const double2* __restrict__ U, const Geom& g, const Site4& nm,
+__device__ __forceinline__ void staple_half(M& A, const double2* __restrict__ U, const Geom& g, const SiteW& nm,
                                             int mu, int nu, double c11, double c12) {
   double2 dep = make_double2(A.e[8].im, A.e[8].im);
-  Site4 x = nm;
+  SiteW x = nm;
   M t = first_step<S, R12>(U, g, x, nu, dep);
   mul_step<-1, R12>(t, U, g, x, mu, dep);  // a, x = n + S nu
-  const Site4 xa = x;
+  const SiteW xa = x;
   M Z = mscale(c11, t);
   if (c12 != 0.0) {
     mul_step<-1, R12>(t, U, g, x, mu, dep);
@@ -408,7 +539,7 @@ __device__ __forceinline__ void staple_half(M& A, const double2* __restrict__ U,
     maxpy(Z, c12, t);  // 2 x 1 in nu, up to its last step
   }
   x = xa;
-  mul_step<-S, R12>(Z, U, g, x, nu, dep);
+  mul_step<-S, R12, false>(Z, U, g, x, nu, dep);  // Z is a sum of paths, not SU(3)
 #pragma unroll
   for (int k = 0; k < 9; ++k) A.e[k] = A.e[k] + Z.e[k];
 }
@@ -430,13 +561,14 @@ __device__ __forceinline__ void staple_half(M& A, const double2* __restrict__ U,
 // 11 SU(3) products and 14 link loads per half instead of 14 and 18.  Every
 // product keeps its path's left-to-right order; only the sums are regrouped,
 // so the result equals staples() to rounding.  mu is uniform over the
-// workgroup and nu over the loop, so every link base stays scalar.
+// workgroup and nu over the loop, so every link base stays scalar.  Every
+// extent must be below 256 (SiteW).
 template <bool R12 = false>
 __device__ __forceinline__ M staples_fused(const double2* __restrict__ U, const Geom& g, const Site4& n, int mu,
                                           double c11, double c12) {
   M A = mzero<double>();
-  Site4 nm = n;
-  nm.move_f(g, mu, +1);  // n + mu
+  SiteW nm = SiteW::of(g, n);
+  nm.move<+1>(g, mu);  // n + mu
   // one half per loop iteration, the two signs in loops of their own: a
   // branch between the halves inside one loop kept both halves' values live
 #pragma unroll 1

@@ -274,6 +274,20 @@ def test_gauge_force_and_action_match_oracle(ctx, dims, c11, c12):
     assert abs(g.avg_plaq() - oc.avg_plaq(dims, U)) < 1e-13
 
 
+@pytest.mark.parametrize("scale", [0.9, 1.0 + 1e-12])
+def test_gauge_force_non_su3_links_match_oracle(ctx, scale):
+    """Links off SU(3) (scaled by 0.9, or by 1 + 1e-12, above the 1e-14
+    two-row threshold of the FP64 kernels) take the force's full-row path,
+    which makes no use of unitarity, and still match the oracle."""
+    dims = (4, 6, 4, 4)
+    U = scale * dev.random_links(dims, 0.5, seed=27)
+    g = dev.Gauge(ctx, dims, U)
+    for c11, c12 in ((5.0 / 3.0, -1.0 / 12.0), (1.0, 0.0)):
+        Fg = g.force(4.0, c11, c12, which=1)
+        Fr = oc.force(dims, U, None, None, 4.0, c11, c12, which=1)
+        assert np.abs(Fg - Fr).max() < 1e-13 * max(1.0, np.abs(Fr).max())
+
+
 def test_fermion_and_full_force_match_oracle(ctx):
     dims = (4, 4, 4, 8)
     m = 0.1
