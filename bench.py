@@ -477,6 +477,45 @@ FORCE_FLOP_PER_SITE = 4 * 85 * 198
 FERMION_FORCE_BYTES_PER_SITE = (72 + 6 + 6 + 72 + 72) * 8
 
 
+def bench_latency(args, dev):
+    """Kernel times at small lattices (8^4 to 24^4): the plain hop with and
+    without its grid reduction, FP32 and FP64, and the two kernels of one
+    FP32-inner mixed iteration in both forms (fused update + hop, split), each
+    the average of 100 back-to-back launches between HIP events.  Where the
+    times barely change with the lattice, a launch is fixed cost, not data."""
+    ctx = dev.Context(0)
+    sizes = {}
+    for L in (8, 12, 16, 24):
+        dims = (L,) * 4
+        g = dev.Gauge(ctx, dims, dev.random_links(dims, 0.3, seed=1))
+        r = {"sites": L ** 4, "waves": L ** 4 // 64}
+        for prec in ("f32", "f64"):
+            r[f"hop_{prec}_us"] = round(g.time_hop(prec, False, 100), 2)
+            r[f"hop_dot_{prec}_us"] = round(g.time_hop(prec, True, 100), 2)
+        # the four-lanes-per-site hop form (four times the waves)
+        for prec in ("f32", "f16", "f64"):
+            dev.set_hop_variant(prec, 1)
+            r[f"hop4_{prec}_us"] = round(g.time_hop(prec, False, 100), 2)
+            dev.set_hop_variant(prec, 0)
+            r[f"hop1_{prec}_us"] = round(g.time_hop(prec, False, 100), 2)
+            dev.set_hop_variant(prec, -1)
+        for form, name in ((0, "fused"), (1, "split")):
+            dev.set_cg_form("f32", form)
+            a, b = g.time_cg_kernels("f32", 100)
+            r[f"mix_f32_{name}_us"] = [round(a, 2), round(b, 2)]
+        dev.set_cg_form("f32", -1)
+        sizes[f"{L}^4"] = r
+        log(f"[bench] latency {L}^4: {r}")
+    out = {"metric": "kernel time at small lattices (us per launch)", "value": sizes["16^4"]["hop_f32_us"],
+           "unit": "us (FP32 hop, 16^4)", "n_gpus": 1, "steps": 100, "warmup": 1,
+           "ms_per_step": round(sizes["16^4"]["hop_f32_us"] / 1e3, 5), "higher_is_better": False,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f32 / f64", "data": "synthetic (seeded Philox)",
+           "config": {"workload": "hop and mixed-iteration kernels at 8^4, 12^4, 16^4, 24^4, links exp(i 0.3 H)",
+                      "parallelism": "single GPU"},
+           "roofline": None, "cpu_baseline": None, "sizes": sizes}
+    print(json.dumps(out))
+
+
 def bench_force(args, dev):
     """The force kernel as the leapfrog runs it (momentum kick fused) on 32^4
     links exp(i 0.3 H), 16^4 beside it: the Symanzik gauge force (c11 = 5/3,
@@ -673,7 +712,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="32^4",
-                    choices=sorted(WORKLOADS) + ["batch", "serve", "precision", "traj32", "heatbath", "hex", "force"])
+                    choices=sorted(WORKLOADS) + ["batch", "serve", "precision", "traj32", "heatbath", "hex", "force",
+                                                          "latency"])
     ap.add_argument("--rates", default="0.5,1,2", help="serve: arrival rates as multiples of the measured "
                                                      "sequential FP64 capacity")
     ap.add_argument("--requests", type=int, default=200, help="serve: requests per arrival rate")
@@ -712,6 +752,8 @@ def main():
         return bench_hex(args, dev)
     if args.workload == "force":
         return bench_force(args, dev)
+    if args.workload == "latency":
+        return bench_latency(args, dev)
 
     wl = WORKLOADS[args.workload]
     dims, mass = wl["dims"], wl["mass"]

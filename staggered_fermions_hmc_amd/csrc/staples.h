@@ -471,9 +471,10 @@ __device__ __forceinline__ M ldLs2_after(const double2* __restrict__ U, const Ge
 // S > 0, U_d(x - d)^dag for S < 0; x moves to x + S d.  t <- t L, or t = L
 // for the first step of a path.  The sign is a compile-time constant: a
 // runtime sign would make the compiler evaluate both branches.
-// Software pipelining of a path with prefetch distance one: the load of a
-// step waits for `dep`, an element of the partial product as it stood before
-// the previous step, so the link of step k + 1 can be in flight while step k
+// Software pipelining of a path with prefetch distance two: the load of
+// step k waits for `dep.y`, an element of the partial product as it stood
+// before step k - 2 multiplied (dep.x holds the one before step k - 1), so
+// the links of steps k and k + 1 can be in flight while step k - 1
 // multiplies, but no later link is loaded early.
 // PURE: t is a product of links (SU(3) when R12), so with SU(3) links the
 // product takes the column-rebuild forms above.

@@ -19,7 +19,8 @@ def test_help_lists_workloads():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--help"], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode == 0
-    for w in ("16^4", "32^4", "batch", "serve", "precision", "traj32", "heatbath", "hex", "--kernels-only"):
+    for w in ("16^4", "32^4", "batch", "serve", "precision", "traj32", "heatbath", "hex", "force", "latency",
+              "--kernels-only", "--legs", "--no-graphs"):
         assert w in r.stdout
 
 
@@ -28,3 +29,4 @@ def test_traffic_summary_names_the_roofline_kernels():
     the bench line reports as roofline.traffic for each workload's kernel."""
     data = json.load(open(os.path.join(ROOT, "profiles", "kernel_traffic.json")))
     assert data["16^4"]["mix_b_f32"] > 0 and data["32^4"]["mix_b_f16"] > 0
+    assert data["32^4"]["force_gauge"] > 0 and data["batch"]["kb_hop"] > 0
