@@ -316,7 +316,8 @@ int sfh_service_destroy(sfh_service* s);
 /* F = F_gauge(beta, c11, c12) + F_fermion(X, Y) per link, selected by which.
  * Replaces compute_full_force_field (Fermions.cpp:640-659) and the
  * fermion-only / gauge-only fields (:1114-1134, :1278-1294).  X and Y may be
- * NULL when which excludes the fermion force. */
+ * NULL when which excludes the fermion force.  The force (and every MD call
+ * that evaluates it) takes lattice extents up to 255 (SFH_E_INVALID above). */
 int sfh_force(sfh_ctx* ctx, const sfh_gauge* g, double beta, double c11, double c12, const double* X,
               const double* Y, unsigned which, double* F);
 /* Symanzik gauge action summed per link.  Replaces compute_action
@@ -367,7 +368,7 @@ int sfh_links_load(double* const* arrays, int n, size_t doubles_each, const char
  * (beta, c11, c12) on the device-resident links.  Replaces heatbath_update
  * (Lattice.cpp:725-757) with single_link_heatbath (:696-723), the subgroup
  * updates R, S, T (:570-694) and the SU(2) sampler (SU3_Sampling.cpp:17-199).
- * Every extent must be a multiple of 4 and beta > 0.  The sweep visits the
+ * Every extent must be a multiple of 4 (at most 252) and beta > 0.  The sweep visits the
  * links direction by direction, each direction in four colour classes
  * (2 n_mu + sum of the other coordinates) mod 4 whose links are updated in
  * parallel; the reference visits sites lexicographically.  Uniforms are

@@ -869,7 +869,7 @@ int sfh_heatbath(sfh_ctx* c, sfh_gauge* g, double beta, double c11, double c12, 
       !std::isfinite(c12))
     return SFH_E_INVALID;
   if (!heatbath_dims_ok(g->g))
-    return set_err(c, SFH_E_INVALID, "heatbath needs every lattice extent to be a multiple of 4");
+    return set_err(c, SFH_E_INVALID, "heatbath needs every lattice extent to be a multiple of 4, at most 252");
   Guard gd(c);
   invalidate_lp(g);
   return heatbath(c, g->g, g->U, beta, c11, c12, seed, sweep0, nsweeps, avg_plaq, action_density);

@@ -25,7 +25,8 @@
 // a wave is a set of whole 128-byte rows.  A call converts the links into
 // that layout, runs its sweeps, and converts them back.
 //
-// About 87 SU(3) products (84 for the staples) per link.
+// About 73 SU(3) products per link: the staple paths grouped by shared
+// factors (66, and 6 more for the plaquette sum the measurement needs).
 #include <cstring>
 
 #include "internal.h"
@@ -246,19 +247,13 @@ __global__ __launch_bounds__(kHbThreads) void k_heatbath(Geom g, double2* __rest
     int lex;
     const Site4 n = class_site(g, mu, col, t, lex);
     const int i = IX::of(g, n);
-    // A1 waits in LDS (lane-interleaved, no bank conflicts) while the
-    // rectangle walks run, so the walks keep two waves per SIMD
-    M A;
-    {
-      const M A1 = staples<IX>(U, g, n, mu, 1.0, 0.0);
+    // A = c11 A1 + c12 A2 from the staple paths grouped by shared factors;
+    // the plaquette sum A1 (for the measurement) accumulates in LDS
+    // (lane-interleaved, no bank conflicts) while the walks run
 #pragma unroll
-      for (int e = 0; e < 9; ++e) {
-        sA1[2 * e][threadIdx.x] = A1.e[e].re;
-        sA1[2 * e + 1][threadIdx.x] = A1.e[e].im;
-      }
-      A = mscale(c11, A1);
-    }
-    if (c12 != 0.0) A = madd(A, mscale(c12, staples<IX>(U, g, n, mu, 0.0, 1.0)));
+    for (int e = 0; e < 18; ++e) sA1[e][threadIdx.x] = 0.0;
+    const M A = staples_fused<IX::kTwoRows, HbLinks<IX::kTwoRows>, true>(U, g, n, mu, c11, c12,
+                                                                         &sA1[0][threadIdx.x], kHbThreads);
     M u = ldLx<IX>(U, g, i, mu);
     Draws rs{Philox{k0, k1}, 4u * (uint32_t)lex + (uint32_t)mu, sweep, 0u, 0, 0.0, 0};
     subgroup<0, 1>(u, A, beta, rs);
@@ -320,8 +315,8 @@ __global__ __launch_bounds__(256) void k_heatbath_total(const double* __restrict
 }  // namespace
 
 bool heatbath_dims_ok(const Geom& g) {
-  for (int d = 0; d < 4; ++d)
-    if (g.L[d] < 4 || g.L[d] % 4) return false;
+  for (int d = 0; d < 4; ++d)  // the staple walks pack each coordinate into 8 bits
+    if (g.L[d] < 4 || g.L[d] % 4 || g.L[d] > 252) return false;
   return true;
 }
 

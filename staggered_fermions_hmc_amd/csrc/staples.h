@@ -307,10 +307,12 @@ __device__ __forceinline__ void rmul_df(M& t, const M& m) {
 // top of the loop body, which held dozens of links in registers at once and
 // spilled.
 // R12: SU(3) links, elements 0-5 read and row 2 rebuilt as conj(row0 x row1).
-template <bool R12 = false>
+// STRIDE: stored elements per link and direction (9, or 6 for a layout of
+// two-row links).
+template <bool R12 = false, int STRIDE = 9>
 __device__ __forceinline__ M ldLs_after(const double2* __restrict__ U, const Geom& g, int i, int d, double dep) {
   const size_t es = (size_t)g.V * sizeof(double2);
-  const char* base = reinterpret_cast<const char*>(U) + (size_t)(d * 9) * es;
+  const char* base = reinterpret_cast<const char*>(U) + (size_t)(d * STRIDE) * es;
   unsigned off = (unsigned)i * (unsigned)sizeof(double2);
   asm volatile("" : "+v"(off) : "v"(dep));
   M m;
@@ -341,19 +343,35 @@ __device__ __forceinline__ Site4 step(const Geom& g, Site4 x, int d, int s) {
   return x;
 }
 
-// A site of the force's staple walks: the four coordinates packed 8 bits
-// each (x_d in bits 8d .. 8d + 7; every extent is below 256), the
-// lexicographic index and the parity's offset p V/2, all carried along by
-// every move.  The direction of a move is wave-uniform, so its extent and
-// stride are scalar selects; a move then costs about ten vector
-// instructions and a link index two, where recomputing the index from four
-// coordinates after each move took about twenty-five.
+// The four coordinates of a site packed 8 bits each (x_d in bits 8d .. 8d + 7;
+// every extent is below 256), moved by S along a wave-uniform direction d:
+// the extent is a scalar select.  w = 1 when the move wrapped.  The wrap is
+// arithmetic on a 0/1 flag: selects on it turned into uniform-looking
+// branch trees, and the force kernel spilled.
+template <int S> __device__ __forceinline__ unsigned packed_move(const Geom& g, unsigned c, int d, int& w) {
+  const bool f0 = d == 0, f1 = d == 1, f2 = d == 2, f3 = d == 3;
+  const int L = (f0 ? g.L[0] : 0) + (f1 ? g.L[1] : 0) + (f2 ? g.L[2] : 0) + (f3 ? g.L[3] : 0);
+  const unsigned sh = 8u * (unsigned)d;
+  int v = (int)((c >> sh) & 255u) + S;
+  w = S > 0 ? (v >= L ? 1 : 0) : (v < 0 ? 1 : 0);
+  v -= w * (S * L);
+  return (c & ~(255u << sh)) | ((unsigned)v << sh);
+}
+__device__ __forceinline__ unsigned pack_site(const Site4& n) {
+  return (unsigned)n.x0 | ((unsigned)n.x1 << 8) | ((unsigned)n.x2 << 16) | ((unsigned)n.x3 << 24);
+}
+
+// A site of the force's staple walks in the solver's even-odd order: packed
+// coordinates, the lexicographic index and the parity's offset p V/2, all
+// carried along by every move (strides are scalar selects), so a link index
+// costs two vector instructions, where recomputing it from four coordinates
+// after each move took about twenty-five.
 struct SiteW {
   unsigned c;
   int lex, pv;
   __device__ __forceinline__ static SiteW of(const Geom& g, const Site4& n) {
     SiteW w;
-    w.c = (unsigned)n.x0 | ((unsigned)n.x1 << 8) | ((unsigned)n.x2 << 16) | ((unsigned)n.x3 << 24);
+    w.c = pack_site(n);
     w.lex = ((n.x0 * g.L[1] + n.x1) * g.L[2] + n.x2) * g.L[3] + n.x3;
     w.pv = ((n.x0 + n.x1 + n.x2 + n.x3) & 1) ? g.Vh : 0;
     return w;
@@ -362,17 +380,41 @@ struct SiteW {
     const bool f0 = d == 0, f1 = d == 1, f2 = d == 2, f3 = d == 3;
     const int L = (f0 ? g.L[0] : 0) + (f1 ? g.L[1] : 0) + (f2 ? g.L[2] : 0) + (f3 ? g.L[3] : 0);
     const int st = (f0 ? 2 * g.S0 : 0) + (f1 ? 2 * g.S1 : 0) + (f2 ? 2 * g.S2 : 0) + (f3 ? 1 : 0);
-    const unsigned sh = 8u * (unsigned)d;
-    int v = (int)((c >> sh) & 255u) + S;
-    // the wrap as arithmetic on a 0/1 flag: selects on it here turned into
-    // uniform-looking branch trees and the kernel spilled
-    const int w = S > 0 ? (v >= L ? 1 : 0) : (v < 0 ? 1 : 0);
-    v -= w * (S * L);
+    int w;
+    c = packed_move<S>(g, c, d, w);
     lex += S * st - w * (S * st * L);
-    c = (c & ~(255u << sh)) | ((unsigned)v << sh);
     pv = g.Vh - pv;  // every move changes the parity (all extents are even)
   }
-  __device__ __forceinline__ int index() const { return pv + (lex >> 1); }
+  __device__ __forceinline__ int index(const Geom&) const { return pv + (lex >> 1); }
+};
+
+// A site of the heatbath's staple walks: packed coordinates, the index in
+// the heatbath layout (x3 = 4 q + r; site (((r L0 + x0) L1 + x1) L2 + x2) Q + q,
+// Q = L3 / 4) recomputed from them at each load.
+struct SiteHb {
+  unsigned c;
+  __device__ __forceinline__ static SiteHb of(const Geom&, const Site4& n) { return SiteHb{pack_site(n)}; }
+  template <int S> __device__ __forceinline__ void move(const Geom& g, int d) {
+    int w;
+    c = packed_move<S>(g, c, d, w);
+  }
+  __device__ __forceinline__ int index(const Geom& g) const {
+    const int x0 = (int)(c & 255u), x1 = (int)((c >> 8) & 255u), x2 = (int)((c >> 16) & 255u), x3 = (int)(c >> 24);
+    return ((((x3 & 3) * g.L[0] + x0) * g.L[1] + x1) * g.L[2] + x2) * (g.L[3] >> 2) + (x3 >> 2);
+  }
+};
+
+// Link layouts of the fused staple walks: the site type and the number of
+// stored elements per link and direction.  EoLinks: the solver's even-odd
+// order, nine elements; HbLinks: the heatbath layout, six elements when its
+// links are kept as two rows.
+struct EoLinks {
+  using Site = SiteW;
+  static constexpr int kStride = 9;
+};
+template <bool TWO> struct HbLinks {
+  using Site = SiteHb;
+  static constexpr int kStride = TWO ? 6 : 9;
 };
 
 // conj(a b - c d) with fused multiply-adds: one entry of the cross product
@@ -451,11 +493,12 @@ __device__ __forceinline__ void rmul_d_su3(M& t, const M& m) {
   rebuild_col2(t);
 }
 
-// Rows 0 and 1 of SU(3) link (i, d) (elements 0-5; the full-storage layout,
-// nine elements per direction), loaded after `dep` as ldLs_after does.
+// Rows 0 and 1 of SU(3) link (i, d) (elements 0-5 of STRIDE stored per
+// direction), loaded after `dep` as ldLs_after does.
+template <int STRIDE = 9>
 __device__ __forceinline__ M ldLs2_after(const double2* __restrict__ U, const Geom& g, int i, int d, double dep) {
   const size_t es = (size_t)g.V * sizeof(double2);
-  const char* base = reinterpret_cast<const char*>(U) + (size_t)(d * 9) * es;
+  const char* base = reinterpret_cast<const char*>(U) + (size_t)(d * STRIDE) * es;
   unsigned off = (unsigned)i * (unsigned)sizeof(double2);
   asm volatile("" : "+v"(off) : "v"(dep));
   M m;
@@ -478,20 +521,21 @@ __device__ __forceinline__ M ldLs2_after(const double2* __restrict__ U, const Ge
 // multiplies, but no later link is loaded early.
 // PURE: t is a product of links (SU(3) when R12), so with SU(3) links the
 // product takes the column-rebuild forms above.
-template <int S, bool R12, bool PURE = true>
-__device__ __forceinline__ void mul_step(M& t, const double2* __restrict__ U, const Geom& g, SiteW& x, int d,
-                                         double2& dep) {
+template <int S, bool R12, bool PURE = true, class LP = EoLinks>
+__device__ __forceinline__ void mul_step(M& t, const double2* __restrict__ U, const Geom& g, typename LP::Site& x,
+                                         int d, double2& dep) {
   constexpr bool kSu3 = R12 && PURE;
+  constexpr int ST = LP::kStride;
   if constexpr (S > 0) {
-    const M m = kSu3 ? ldLs2_after(U, g, x.index(), d, dep.y) : ldLs_after<R12>(U, g, x.index(), d, dep.y);
-    x.move<+1>(g, d);
+    const M m = kSu3 ? ldLs2_after<ST>(U, g, x.index(g), d, dep.y) : ldLs_after<R12, ST>(U, g, x.index(g), d, dep.y);
+    x.template move<+1>(g, d);
     dep.y = dep.x;
     dep.x = t.e[8].im;
     if constexpr (kSu3) rmul_n_su3(t, m);
     else rmul_nf(t, m);
   } else {
-    x.move<-1>(g, d);
-    const M m = kSu3 ? ldLs2_after(U, g, x.index(), d, dep.y) : ldLs_after<R12>(U, g, x.index(), d, dep.y);
+    x.template move<-1>(g, d);
+    const M m = kSu3 ? ldLs2_after<ST>(U, g, x.index(g), d, dep.y) : ldLs_after<R12, ST>(U, g, x.index(g), d, dep.y);
     dep.y = dep.x;
     dep.x = t.e[8].im;
     if constexpr (kSu3) rmul_d_su3(t, m);
@@ -499,48 +543,68 @@ __device__ __forceinline__ void mul_step(M& t, const double2* __restrict__ U, co
   }
 }
 // the first link of a path
-template <int S, bool R12>
-__device__ __forceinline__ M first_step(const double2* __restrict__ U, const Geom& g, SiteW& x, int d, double2& dep) {
+template <int S, bool R12, class LP = EoLinks>
+__device__ __forceinline__ M first_step(const double2* __restrict__ U, const Geom& g, typename LP::Site& x, int d,
+                                        double2& dep) {
+  constexpr int ST = LP::kStride;
   if constexpr (S > 0) {
-    const M m = ldLs_after<R12>(U, g, x.index(), d, dep.y);
-    x.move<+1>(g, d);
+    const M m = ldLs_after<R12, ST>(U, g, x.index(g), d, dep.y);
+    x.template move<+1>(g, d);
     return m;
   } else {
-    x.move<-1>(g, d);
-    return mdag(ldLs_after<R12>(U, g, x.index(), d, dep.y));
+    x.template move<-1>(g, d);
+    return mdag(ldLs_after<R12, ST>(U, g, x.index(g), d, dep.y));
   }
 }
 
-// One half of the staple sum of plane (mu, nu) (see staples_fused), added to A.
-template <int S, bool R12>
-__device__ __forceinline__ void staple_half(M& A, const double2* __restrict__ U, const Geom& g, const SiteW& nm,
-                                            int mu, int nu, double c11, double c12) {
+// a[k * stride] += s b (a matrix kept lane-interleaved in LDS, 18 reals)
+__device__ __forceinline__ void lds_axpy(double* a, int stride, double s, const M& b) {
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    a[(2 * k) * stride] = fma(s, b.e[k].re, a[(2 * k) * stride]);
+    a[(2 * k + 1) * stride] = fma(s, b.e[k].im, a[(2 * k + 1) * stride]);
+  }
+}
+
+// One half of the staple sum of plane (mu, nu) (see staples_fused), added to
+// A; WA1: also the half's plaquette staple a (nu, -S), added to the matrix
+// at sA1 (LDS, stride sst).
+template <int S, bool R12, class LP, bool WA1>
+__device__ __forceinline__ void staple_half(M& A, double* sA1, int sst, const double2* __restrict__ U, const Geom& g,
+                                            const typename LP::Site& nm, int mu, int nu, double c11, double c12) {
+  using Site = typename LP::Site;
   double2 dep = make_double2(A.e[8].im, A.e[8].im);
-  SiteW x = nm;
-  M t = first_step<S, R12>(U, g, x, nu, dep);
-  mul_step<-1, R12>(t, U, g, x, mu, dep);  // a, x = n + S nu
-  const SiteW xa = x;
+  Site x = nm;
+  M t = first_step<S, R12, LP>(U, g, x, nu, dep);
+  mul_step<-1, R12, true, LP>(t, U, g, x, mu, dep);  // a, x = n + S nu
+  const Site xa = x;
+  if constexpr (WA1) {
+    Site xp = xa;
+    M P = t;
+    mul_step<-S, R12, true, LP>(P, U, g, xp, nu, dep);
+    lds_axpy(sA1, sst, 1.0, P);
+  }
   M Z = mscale(c11, t);
   if (c12 != 0.0) {
-    mul_step<-1, R12>(t, U, g, x, mu, dep);
-    mul_step<-S, R12>(t, U, g, x, nu, dep);
-    mul_step<+1, R12>(t, U, g, x, mu, dep);
+    mul_step<-1, R12, true, LP>(t, U, g, x, mu, dep);
+    mul_step<-S, R12, true, LP>(t, U, g, x, nu, dep);
+    mul_step<+1, R12, true, LP>(t, U, g, x, mu, dep);
     maxpy(A, c12, t);  // link second
     x = nm;
-    t = first_step<+1, R12>(U, g, x, mu, dep);
-    mul_step<S, R12>(t, U, g, x, nu, dep);
-    mul_step<-1, R12>(t, U, g, x, mu, dep);
-    mul_step<-1, R12>(t, U, g, x, mu, dep);
+    t = first_step<+1, R12, LP>(U, g, x, mu, dep);
+    mul_step<S, R12, true, LP>(t, U, g, x, nu, dep);
+    mul_step<-1, R12, true, LP>(t, U, g, x, mu, dep);
+    mul_step<-1, R12, true, LP>(t, U, g, x, mu, dep);
     maxpy(Z, c12, t);  // link first, up to its last step
     x = nm;
-    t = first_step<S, R12>(U, g, x, nu, dep);
-    mul_step<S, R12>(t, U, g, x, nu, dep);
-    mul_step<-1, R12>(t, U, g, x, mu, dep);
-    mul_step<-S, R12>(t, U, g, x, nu, dep);
+    t = first_step<S, R12, LP>(U, g, x, nu, dep);
+    mul_step<S, R12, true, LP>(t, U, g, x, nu, dep);
+    mul_step<-1, R12, true, LP>(t, U, g, x, mu, dep);
+    mul_step<-S, R12, true, LP>(t, U, g, x, nu, dep);
     maxpy(Z, c12, t);  // 2 x 1 in nu, up to its last step
   }
   x = xa;
-  mul_step<-S, R12, false>(Z, U, g, x, nu, dep);  // Z is a sum of paths, not SU(3)
+  mul_step<-S, R12, false, LP>(Z, U, g, x, nu, dep);  // Z is a sum of paths, not SU(3)
 #pragma unroll
   for (int k = 0; k < 9; ++k) A.e[k] = A.e[k] + Z.e[k];
 }
@@ -563,24 +627,27 @@ __device__ __forceinline__ void staple_half(M& A, const double2* __restrict__ U,
 // product keeps its path's left-to-right order; only the sums are regrouped,
 // so the result equals staples() to rounding.  mu is uniform over the
 // workgroup and nu over the loop, so every link base stays scalar.  Every
-// extent must be below 256 (SiteW).
-template <bool R12 = false>
+// extent must be below 256 (packed coordinates).  LP: the link layout; WA1:
+// the plaquette staple sum A1 (six paths, one more product per half) is
+// also accumulated into the LDS matrix at sA1 (stride sst), which the
+// caller zeroes.
+template <bool R12 = false, class LP = EoLinks, bool WA1 = false>
 __device__ __forceinline__ M staples_fused(const double2* __restrict__ U, const Geom& g, const Site4& n, int mu,
-                                          double c11, double c12) {
+                                          double c11, double c12, double* sA1 = nullptr, int sst = 0) {
   M A = mzero<double>();
-  SiteW nm = SiteW::of(g, n);
-  nm.move<+1>(g, mu);  // n + mu
+  typename LP::Site nm = LP::Site::of(g, n);
+  nm.template move<+1>(g, mu);  // n + mu
   // one half per loop iteration, the two signs in loops of their own: a
   // branch between the halves inside one loop kept both halves' values live
 #pragma unroll 1
   for (int nu = 0; nu < 4; ++nu) {
     if (nu == mu) continue;
-    staple_half<+1, R12>(A, U, g, nm, mu, nu, c11, c12);
+    staple_half<+1, R12, LP, WA1>(A, sA1, sst, U, g, nm, mu, nu, c11, c12);
   }
 #pragma unroll 1
   for (int nu = 0; nu < 4; ++nu) {
     if (nu == mu) continue;
-    staple_half<-1, R12>(A, U, g, nm, mu, nu, c11, c12);
+    staple_half<-1, R12, LP, WA1>(A, sA1, sst, U, g, nm, mu, nu, c11, c12);
   }
   return A;
 }
