@@ -805,14 +805,16 @@ int sfh_force(sfh_ctx* c, const sfh_gauge* g, double beta, double c11, double c1
 int sfh_action(sfh_ctx* c, const sfh_gauge* g, double beta, double c11, double c12, double* out) {
   if (!c || !g || !out) return SFH_E_INVALID;
   Guard gd(c);
-  *out = gauge_action(c, g->g, g->U, beta, c11, c12);
+  if (int e = ensure_su3(c, const_cast<sfh_gauge*>(g))) return e;
+  *out = gauge_action(c, g->g, g->U, beta, c11, c12, g->f64_su3);
   return std::isnan(*out) ? SFH_E_HIP : SFH_OK;
 }
 
 int sfh_avg_plaq(sfh_ctx* c, const sfh_gauge* g, double* out) {
   if (!c || !g || !out) return SFH_E_INVALID;
   Guard gd(c);
-  *out = avg_plaq(c, g->g, g->U);
+  if (int e = ensure_su3(c, const_cast<sfh_gauge*>(g))) return e;
+  *out = avg_plaq(c, g->g, g->U, g->f64_su3);
   return std::isnan(*out) ? SFH_E_HIP : SFH_OK;
 }
 
@@ -944,9 +946,9 @@ int sfh_hmc_trajectory(sfh_ctx* c, sfh_gauge* g, const sfh_md_params* p, const d
   double2* dU0 = dbuf(c, "u_save", 36 * (size_t)gg.V);
   if (!dchi || !dphi || !dX || !dP || !dU0) return SFH_E_NOMEM;
   // Fermions.cpp:812-828
-  const double sg_i = gauge_action(c, gg, g->U, p->beta, p->c11, p->c12);
-  if (int e = upload_spinor(c, gg, chi, dchi)) return e;
   if (int e = ensure_su3(c, g)) return e;
+  const double sg_i = gauge_action(c, gg, g->U, p->beta, p->c11, p->c12, g->f64_su3);
+  if (int e = upload_spinor(c, gg, chi, dchi)) return e;
   hop_f64(c->stream, gg, g->U, dchi, dphi, p->mass, 1.0, nullptr, nullptr, nullptr, 0, g->f64_su3);
   if (int e = upload_links(c, gg, P0, dP)) return e;
   const double tp_i = tr_p2(c, gg, dP);
@@ -976,7 +978,8 @@ int sfh_hmc_trajectory(sfh_ctx* c, sfh_gauge* g, const sfh_md_params* p, const d
     return rc;
   }
   const double tp_f = tr_p2(c, gg, dP);
-  const double sg_f = gauge_action(c, gg, g->U, p->beta, p->c11, p->c12);
+  if (int e = ensure_su3(c, g)) return e;
+  const double sg_f = gauge_action(c, gg, g->U, p->beta, p->c11, p->c12, g->f64_su3);
   dot_f64(c->stream, reinterpret_cast<const double*>(dphi), reinterpret_cast<const double*>(dX), 2 * n,
           c->d_partials + 2 * kPartialSlot, c->aux_ctr(), &c->d_state[1].alpha);
   hipMemcpyAsync(c->h_scalar, &c->d_state[1].alpha, sizeof(double), hipMemcpyDeviceToHost, c->stream);

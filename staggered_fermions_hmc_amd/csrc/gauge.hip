@@ -120,7 +120,11 @@ int force_field(sfh_ctx* c, const Geom& g, const double2* U, const double2* X, c
 }
 
 // ------------------------------------------------------------------ reductions over links
-// kind 0: action, 1: plaquette sum ReTr(U A1)/3, 2: ReTr(P^2)
+// kind 0: action, 1: plaquette sum ReTr(U A1)/3, 2: ReTr(P^2).  The action
+// -(beta/3) [c11 ReTr(U A1)/4 + c12 ReTr(U A2)/6] (Lattice.cpp:759-781) is
+// linear in the staple sums, so it is ReTr(U A) of one shared-factor staple
+// sum with coefficients c11/4 and c12/6.
+template <bool R12>
 __global__ __launch_bounds__(128) void k_link_sum(Geom g, const double2* __restrict__ U, double beta, double c11,
                                                   double c12, int kind, double* partials, unsigned* cnt,
                                                   double* out, int nb) {
@@ -138,36 +142,41 @@ __global__ __launch_bounds__(128) void k_link_sum(Geom g, const double2* __restr
       v += retr_mul(u, u);
     } else {
       const Site4 n = site_of(g, i);
-      const double pl = retr_mul(u, staples(U, g, n, mu, 1.0, 0.0));
-      if (kind == 1) {
-        v += pl / 3.0;
-      } else {
-        const double rc = c12 != 0.0 ? retr_mul(u, staples(U, g, n, mu, 0.0, 1.0)) : 0.0;
-        v += -(beta / 3.0) * (c11 * (pl / 4.0) + c12 * (rc / 6.0));
-      }
+      if (kind == 1) v += retr_mul(u, staples_fused<R12>(U, g, n, mu, 1.0, 0.0)) / 3.0;
+      else v += -(beta / 3.0) * retr_mul(u, staples_fused<R12>(U, g, n, mu, c11 / 4.0, c12 / 6.0));
     }
   }
   grid_reduce(v, partials, cnt, nb, lb, FinStore{out});
 }
 
-static double link_sum(sfh_ctx* c, const Geom& g, const double2* U, double beta, double c11, double c12, int kind) {
+static double link_sum(sfh_ctx* c, const Geom& g, const double2* U, double beta, double c11, double c12, int kind,
+                       bool r12) {
+  for (int d = 0; d < 4; ++d)  // the staple walks pack each coordinate into 8 bits
+    if (kind != 2 && g.L[d] > 255) {
+      set_err(c, SFH_E_INVALID, "action and plaquette: lattice extents above 255");
+      return NAN;
+    }
   int nb = (4 * g.V + 127) / 128;
   if (nb > kPartialSlot - kRedGroups) nb = kPartialSlot - kRedGroups;
   KTimer kt(c, "force");
-  hipLaunchKernelGGL(k_link_sum, dim3(nb), dim3(128), 0, c->stream, g, U, beta, c11, c12, kind,
-                     c->d_partials + 2 * kPartialSlot, c->aux_ctr(), &c->d_state[1].pv, nb);
+  if (r12)
+    hipLaunchKernelGGL(k_link_sum<true>, dim3(nb), dim3(128), 0, c->stream, g, U, beta, c11, c12, kind,
+                       c->d_partials + 2 * kPartialSlot, c->aux_ctr(), &c->d_state[1].pv, nb);
+  else
+    hipLaunchKernelGGL(k_link_sum<false>, dim3(nb), dim3(128), 0, c->stream, g, U, beta, c11, c12, kind,
+                       c->d_partials + 2 * kPartialSlot, c->aux_ctr(), &c->d_state[1].pv, nb);
   hipMemcpyAsync(c->h_scalar, &c->d_state[1].pv, sizeof(double), hipMemcpyDeviceToHost, c->stream);
   if (hip_check(c, hipStreamSynchronize(c->stream), "link reduction")) return NAN;
   return *c->h_scalar;
 }
 
-double gauge_action(sfh_ctx* c, const Geom& g, const double2* U, double beta, double c11, double c12) {
-  return link_sum(c, g, U, beta, c11, c12, 0);
+double gauge_action(sfh_ctx* c, const Geom& g, const double2* U, double beta, double c11, double c12, bool r12) {
+  return link_sum(c, g, U, beta, c11, c12, 0, r12);
 }
-double avg_plaq(sfh_ctx* c, const Geom& g, const double2* U) {
-  return link_sum(c, g, U, 0, 0, 0, 1) / (4.0 * g.V);
+double avg_plaq(sfh_ctx* c, const Geom& g, const double2* U, bool r12) {
+  return link_sum(c, g, U, 0, 0, 0, 1, r12) / (4.0 * g.V);
 }
-double tr_p2(sfh_ctx* c, const Geom& g, const double2* P) { return link_sum(c, g, P, 0, 0, 0, 2); }
+double tr_p2(sfh_ctx* c, const Geom& g, const double2* P) { return link_sum(c, g, P, 0, 0, 0, 2, false); }
 
 // ------------------------------------------------------------------ Polyakov loops
 // Tr prod_t U_3(x, y, z, t), right-multiplied in t order from t = 0

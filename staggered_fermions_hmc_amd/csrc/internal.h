@@ -199,8 +199,9 @@ void ru_sub(hipStream_t s, const Geom& g, double2* r, const double2* phi, const 
 // r12: the links are SU(3) to kSu3TolF64; the gauge force reads two rows.
 int force_field(sfh_ctx* c, const Geom& g, const double2* U, const double2* X, const double2* Y, double beta,
                 double c11, double c12, unsigned which, double2* F, double2* P, double coef, bool r12);
-double gauge_action(sfh_ctx* c, const Geom& g, const double2* U, double beta, double c11, double c12);
-double avg_plaq(sfh_ctx* c, const Geom& g, const double2* U);
+// r12: the links pass the FP64 SU(3) check (ensure_su3), so the staple walks read two rows
+double gauge_action(sfh_ctx* c, const Geom& g, const double2* U, double beta, double c11, double c12, bool r12);
+double avg_plaq(sfh_ctx* c, const Geom& g, const double2* U, bool r12);
 double tr_p2(sfh_ctx* c, const Geom& g, const double2* P);
 int link_update(sfh_ctx* c, const Geom& g, double2* U, const double2* P, double eps, int check);
 int exp_iq_one(sfh_ctx* c, const double* Q18, double* out18);
