@@ -69,6 +69,27 @@ __device__ __forceinline__ void stW(double2* __restrict__ W, const Geom& g, int 
 #pragma unroll
   for (int e = 0; e < n; ++e) W[(size_t)(k * n + e) * g.V + i] = make_double2(m.e[e].re, m.e[e].im);
 }
+// Rows 0 and 1 only of an SU(3) link of a two-row level array (the SU(3)
+// product forms need no third row).
+__device__ __forceinline__ M ldW2(const double2* __restrict__ W, const Geom& g, int i, int k) {
+  M m;
+#pragma unroll
+  for (int e = 0; e < 6; ++e) {
+    const double2 v = W[(size_t)(k * 6 + e) * g.V + i];
+    m.e[e] = {v.x, v.y};
+  }
+  return m;
+}
+// Rows 0 and 1 only of an SU(3) thin link.
+__device__ __forceinline__ M ldT2(const double2* __restrict__ U, const Geom& g, int i, int mu) {
+  M m;
+#pragma unroll
+  for (int e = 0; e < 6; ++e) {
+    const double2 v = U[(size_t)(mu * 9 + e) * g.V + i];
+    m.e[e] = {v.x, v.y};
+  }
+  return m;
+}
 // Thin link (all nine elements stored; two rows read when TWO).
 template <bool TWO>
 __device__ __forceinline__ M ldT(const double2* __restrict__ U, const Geom& g, int i, int mu) {
@@ -86,23 +107,37 @@ __device__ __forceinline__ M ldT(const double2* __restrict__ U, const Geom& g, i
   }
 }
 
-// Up plus down staple of link (x, mu) in the (mu, sig) plane; L(i, d) is the
-// link of direction d at device site i for the level at hand.
-template <class F>
-__device__ __forceinline__ M staple_pair(const Geom& g, const Site4& x, int mu, int sig, F L) {
-  Site4 xs = x, xm = x, xd = x;
-  xs.move(g, sig, +1);
-  xm.move(g, mu, +1);
-  xd.move(g, sig, -1);
-  Site4 xdm = xd;
-  xdm.move(g, mu, +1);
-  M up = L(x.index(g), sig);
-  rmul_n(up, L(xs.index(g), mu));
-  rmul_d(up, L(xm.index(g), sig));
-  M dn = mdag(L(xd.index(g), sig));
-  rmul_n(dn, L(xd.index(g), mu));
-  rmul_n(dn, L(xdm.index(g), sig));
-  return madd(up, dn);
+// Up plus down staple of link (x, mu) in the (mu, sig) plane; L(i, d, full)
+// is the link of direction d at device site i for the level at hand, with
+// all three rows (full) or, for TWO, rows 0 and 1 only.  The site walks
+// carry their index (SiteW).  TWO: every link is SU(3), so the products take
+// the column-rebuild forms (rmul_n_su3 / rmul_d_su3), which need the third
+// row of no link but the first of each path.
+template <bool TWO, class F>
+__device__ __forceinline__ M staple_pair(const Geom& g, const SiteW& x, int mu, int sig, F L) {
+  SiteW xs = x, xm = x, xd = x;
+  xs.move<+1>(g, sig);
+  xm.move<+1>(g, mu);
+  xd.move<-1>(g, sig);
+  SiteW xdm = xd;
+  xdm.move<+1>(g, mu);
+  if constexpr (TWO) {
+    M up = L(x.index(g), sig, true);
+    rmul_n_su3(up, L(xs.index(g), mu, false));
+    rmul_d_su3(up, L(xm.index(g), sig, false));
+    M dn = mdag(L(xd.index(g), sig, true));
+    rmul_n_su3(dn, L(xd.index(g), mu, false));
+    rmul_n_su3(dn, L(xdm.index(g), sig, false));
+    return madd(up, dn);
+  } else {
+    M up = L(x.index(g), sig, true);
+    rmul_n(up, L(xs.index(g), mu, true));
+    rmul_d(up, L(xm.index(g), sig, true));
+    M dn = mdag(L(xd.index(g), sig, true));
+    rmul_n(dn, L(xd.index(g), mu, true));
+    rmul_n(dn, L(xdm.index(g), sig, true));
+    return madd(up, dn);
+  }
 }
 
 // exp(proj(alpha/2 G U^dagger)) U (Smearing.cpp:126-129, :143-145, :223-225, :291-293).
@@ -148,8 +183,9 @@ __global__ __launch_bounds__(kHexThreads) void k_hex1(Geom g, const double2* __r
   const int i = hex_lane(g, 12, nb, k);
   if (i < 0) return;
   const int mu = k / 3, r = k % 3, sig = r < mu ? r : r + 1;
-  const Site4 x = site_of(g, i);
-  const M G = staple_pair(g, x, mu, sig, [&](int j, int d) { return ldT<TWO>(U, g, j, d); });
+  const SiteW x = SiteW::of(g, site_of(g, i));
+  const M G = staple_pair<TWO>(g, x, mu, sig,
+                               [&](int j, int d, bool full) { return full ? ldT<TWO>(U, g, j, d) : ldT2(U, g, j, d); });
   M out;
   const int st = stout(G, ldT<TWO>(U, g, i, mu), alpha, out);
   if (st != SFH_OK) atomicMax(status, st);
@@ -164,13 +200,15 @@ __global__ __launch_bounds__(kHexThreads) void k_hex2(Geom g, const double2* __r
   const int i = hex_lane(g, 12, nb, k);
   if (i < 0) return;
   const int mu = k / 3, r = k % 3, nu = r < mu ? r : r + 1;
-  const Site4 x = site_of(g, i);
+  const SiteW x = SiteW::of(g, site_of(g, i));
   M G = mzero<double>();
 #pragma unroll 1
   for (int sig = 0; sig < 4; ++sig) {
     if (sig == mu || sig == nu) continue;
     const int rho = 6 - mu - nu - sig;
-    G = madd(G, staple_pair(g, x, mu, sig, [&](int j, int d) { return ldW<TWO>(V1, g, j, pair_slot(d, rho)); }));
+    G = madd(G, staple_pair<TWO>(g, x, mu, sig, [&](int j, int d, bool full) {
+               return full ? ldW<TWO>(V1, g, j, pair_slot(d, rho)) : ldW2(V1, g, j, pair_slot(d, rho));
+             }));
   }
   M out;
   const int st = stout(G, ldT<TWO>(U, g, i, mu), alpha, out);
@@ -187,13 +225,15 @@ __global__ __launch_bounds__(kHexThreads) void k_hex3(Geom g, double2* __restric
   int mu;
   const int i = hex_lane(g, 4, nb, mu);
   if (i < 0) return;
-  const Site4 x = site_of(g, i);
+  const SiteW x = SiteW::of(g, site_of(g, i));
   M G = mzero<double>();
 #pragma unroll 1
   for (int nu = 0; nu < 4; ++nu) {
     if (nu == mu) continue;
-    G = madd(G, staple_pair(g, x, mu, nu,
-                            [&](int j, int d) { return ldW<TWO>(V2, g, j, pair_slot(d, d == mu ? nu : mu)); }));
+    G = madd(G, staple_pair<TWO>(g, x, mu, nu, [&](int j, int d, bool full) {
+               const int k = pair_slot(d, d == mu ? nu : mu);
+               return full ? ldW<TWO>(V2, g, j, k) : ldW2(V2, g, j, k);
+             }));
   }
   M out;
   const int st = stout(G, ldT<TWO>(U, g, i, mu), alpha, out);
@@ -236,6 +276,8 @@ static int hex_steps(sfh_ctx* c, const Geom& g, double2* U, double2* V1, double2
 
 int hex_smear(sfh_ctx* c, const Geom& g, double2* U, const double alpha[3], int nsteps) {
   if (nsteps == 0) return SFH_OK;
+  for (int d = 0; d < 4; ++d)  // the staple walks pack each coordinate into 8 bits
+    if (g.L[d] > 255) return set_err(c, SFH_E_INVALID, "hex smearing: lattice extents above 255");
   double2* V1 = static_cast<double2*>(scratch(c, "hex_v1", 108 * (size_t)g.V * sizeof(double2)));
   double2* V2 = static_cast<double2*>(scratch(c, "hex_v2", 108 * (size_t)g.V * sizeof(double2)));
   unsigned* dev = static_cast<unsigned*>(scratch(c, "su3_dev", sizeof(unsigned)));

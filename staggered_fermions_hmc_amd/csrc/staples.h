@@ -1,7 +1,7 @@
 // Closed-path staples of the Symanzik gauge action, shared by the force,
-// action and heatbath kernels: a site as four coordinates in registers, link
-// loads and stores in the device layout, row-wise SU(3) products, and the
-// plaquette and rectangle paths walked from n + mu back to n.
+// action, heatbath and HEX kernels: sites as coordinates in registers, link
+// loads and stores in the device layouts, SU(3) products, and the plaquette
+// and rectangle paths walked from n + mu back to n with their shared factors.
 #pragma once
 #include "internal.h"
 
@@ -169,68 +169,6 @@ __device__ __forceinline__ void rmul_d(M& t, const M& m) {
       t.e[3 * i + j] = s;
     }
   }
-}
-
-// Closed paths through link U_mu(n), as moves from n + mu back to n: d = 0
-// moves along mu, d = 1 along the plane's other direction nu; a + move from
-// x takes U_d(x), a - move takes U_d(x - d)^dag.  The product of a path is
-// the staple whose ReTr(U_mu(n) staple) is the loop.
-struct Move {
-  int d, s;
-};
-// the two plaquettes of the (mu, nu) plane (Lattice.cpp:249-313): below, above
-static __constant__ Move kPlaqPath[2][3] = {{{1, -1}, {0, -1}, {1, +1}}, {{1, +1}, {0, -1}, {1, -1}}};
-// the six rectangles (Lattice.cpp:315-499): 2 x 1 in nu below and above
-// (short side on the link), then 1 x 2 with the link first or second along
-// the long side, below and above
-static __constant__ Move kRectPath[6][5] = {
-    {{1, -1}, {1, -1}, {0, -1}, {1, +1}, {1, +1}},
-    {{1, +1}, {1, +1}, {0, -1}, {1, -1}, {1, -1}},
-    {{0, +1}, {1, -1}, {0, -1}, {0, -1}, {1, +1}},
-    {{1, +1}, {0, -1}, {0, -1}, {1, -1}, {0, +1}},
-    {{1, -1}, {0, -1}, {0, -1}, {1, +1}, {0, +1}},
-    {{0, +1}, {1, +1}, {0, -1}, {0, -1}, {1, -1}},
-};
-
-template <class IX, int N>
-__device__ __forceinline__ M walk(const double2* __restrict__ U, const Geom& g, Site4 x, int mu, int nu,
-                                  const Move (&path)[N]) {
-  M t;
-#pragma unroll
-  for (int k = 0; k < N; ++k) {
-    const int d = path[k].d ? nu : mu;
-    const int s = path[k].s;
-    if (s < 0) x.move(g, d, -1);
-    const M m = ldLx<IX>(U, g, IX::of(g, x), d);
-    if (s > 0) x.move(g, d, +1);
-    if (k == 0) t = s > 0 ? m : mdag(m);
-    else if (s > 0) rmul_n(t, m);
-    else rmul_d(t, m);
-  }
-  return t;
-}
-
-// c11 A1 + c12 A2 for link (n, mu): A1 the six plaquette staples, A2 the
-// eighteen rectangle staples (Lattice.cpp:249-507); IX places sites in U.
-template <class IX = EoIndex>
-static __device__ M staples(const double2* __restrict__ U, const Geom& g, const Site4& n, int mu, double c11,
-                            double c12) {
-  M A = mzero<double>();
-  Site4 x = n;
-  x.move(g, mu, +1);
-#pragma unroll 1
-  for (int nu = 0; nu < 4; ++nu) {
-    if (nu == mu) continue;
-    if (c11 != 0.0) {
-#pragma unroll 1
-      for (int k = 0; k < 2; ++k) A = madd(A, mscale(c11, walk<IX>(U, g, x, mu, nu, kPlaqPath[k])));
-    }
-    if (c12 != 0.0) {
-#pragma unroll 1
-      for (int k = 0; k < 6; ++k) A = madd(A, mscale(c12, walk<IX>(U, g, x, mu, nu, kRectPath[k])));
-    }
-  }
-  return A;
 }
 
 // Link (i, d) with a wave-uniform base per element (d uniform, V a kernel
@@ -609,8 +547,9 @@ __device__ __forceinline__ void staple_half(M& A, double* sA1, int sst, const do
   for (int k = 0; k < 9; ++k) A.e[k] = A.e[k] + Z.e[k];
 }
 
-// c11 A1 + c12 A2 for link (n, mu), the staple sums of staples() with the
-// paths of each plane grouped by shared factors (Lattice.cpp:249-507).  The
+// c11 A1 + c12 A2 for link (n, mu), A1 the six plaquette staples and A2 the
+// eighteen rectangle staples (Lattice.cpp:249-507), with the paths of each
+// plane grouped by shared factors.  The
 // eight paths of plane (mu, nu) split into the four that leave n + mu
 // towards +nu or end on U_nu(n)^dag (the plaquette above and rectangle types
 // 2, 4 and 6) and the four mirrored ones towards -nu (below, types 1, 5 and
@@ -625,7 +564,7 @@ __device__ __forceinline__ void staple_half(M& A, double* sA1, int sst, const do
 //   + c12 a (mu, -) (nu, -S) (mu, +):
 // 11 SU(3) products and 14 link loads per half instead of 14 and 18.  Every
 // product keeps its path's left-to-right order; only the sums are regrouped,
-// so the result equals staples() to rounding.  mu is uniform over the
+// so the result equals the path-by-path sum to rounding.  mu is uniform over the
 // workgroup and nu over the loop, so every link base stays scalar.  Every
 // extent must be below 256 (packed coordinates).  LP: the link layout; WA1:
 // the plaquette staple sum A1 (six paths, one more product per half) is
