@@ -39,7 +39,9 @@ __device__ __forceinline__ M mi_antiherm(const M& A, double s) {
 }
 
 // Total force on link (i, mu) and optional P -= coef F (Fermions.cpp:372-384,
-// :443-515, :640-659, :690-733).
+// :443-515, :640-659, :690-733).  R12: the links passed the FP64 SU(3)
+// check, so the gauge part reads two rows per link and its staple paths take
+// the SU(3) product forms; otherwise every link is read whole.
 template <bool R12>
 __global__ __launch_bounds__(128) void k_force(Geom g, const double2* __restrict__ U, const double2* __restrict__ X,
                                                const double2* __restrict__ Y, double beta, double c11, double c12,
