@@ -805,6 +805,7 @@ int sfh_force(sfh_ctx* c, const sfh_gauge* g, double beta, double c11, double c1
 int sfh_action(sfh_ctx* c, const sfh_gauge* g, double beta, double c11, double c12, double* out) {
   if (!c || !g || !out) return SFH_E_INVALID;
   Guard gd(c);
+  if (!staple_dims_ok(g->g)) return set_err(c, SFH_E_INVALID, "action: lattice extents above 255");
   if (int e = ensure_su3(c, const_cast<sfh_gauge*>(g))) return e;
   *out = gauge_action(c, g->g, g->U, beta, c11, c12, g->f64_su3);
   return std::isnan(*out) ? SFH_E_HIP : SFH_OK;
@@ -813,6 +814,7 @@ int sfh_action(sfh_ctx* c, const sfh_gauge* g, double beta, double c11, double c
 int sfh_avg_plaq(sfh_ctx* c, const sfh_gauge* g, double* out) {
   if (!c || !g || !out) return SFH_E_INVALID;
   Guard gd(c);
+  if (!staple_dims_ok(g->g)) return set_err(c, SFH_E_INVALID, "plaquette: lattice extents above 255");
   if (int e = ensure_su3(c, const_cast<sfh_gauge*>(g))) return e;
   *out = avg_plaq(c, g->g, g->U, g->f64_su3);
   return std::isnan(*out) ? SFH_E_HIP : SFH_OK;

@@ -106,10 +106,15 @@ __global__ __launch_bounds__(128) void k_force(Geom g, const double2* __restrict
   }
 }
 
+bool staple_dims_ok(const Geom& g) {
+  for (int d = 0; d < 4; ++d)  // the staple walks pack each coordinate into 8 bits
+    if (g.L[d] > 255) return false;
+  return true;
+}
+
 int force_field(sfh_ctx* c, const Geom& g, const double2* U, const double2* X, const double2* Y, double beta,
                 double c11, double c12, unsigned which, double2* F, double2* P, double coef, bool r12) {
-  for (int d = 0; d < 4; ++d)  // the staple walks pack each coordinate into 8 bits
-    if (g.L[d] > 255) return set_err(c, SFH_E_INVALID, "force: lattice extents above 255");
+  if (!staple_dims_ok(g)) return set_err(c, SFH_E_INVALID, "force: lattice extents above 255");
   KTimer kt(c, "force");
   const int nb = 4 * ((g.Vh + kRangeHalf - 1) / kRangeHalf);
   if (r12)
@@ -153,11 +158,10 @@ __global__ __launch_bounds__(128) void k_link_sum(Geom g, const double2* __restr
 
 static double link_sum(sfh_ctx* c, const Geom& g, const double2* U, double beta, double c11, double c12, int kind,
                        bool r12) {
-  for (int d = 0; d < 4; ++d)  // the staple walks pack each coordinate into 8 bits
-    if (kind != 2 && g.L[d] > 255) {
-      set_err(c, SFH_E_INVALID, "action and plaquette: lattice extents above 255");
-      return NAN;
-    }
+  if (kind != 2 && !staple_dims_ok(g)) {
+    set_err(c, SFH_E_INVALID, "action and plaquette: lattice extents above 255");
+    return NAN;
+  }
   int nb = (4 * g.V + 127) / 128;
   if (nb > kPartialSlot - kRedGroups) nb = kPartialSlot - kRedGroups;
   KTimer kt(c, "force");

@@ -199,6 +199,9 @@ void ru_sub(hipStream_t s, const Geom& g, double2* r, const double2* phi, const 
 // r12: the links are SU(3) to kSu3TolF64; the gauge force reads two rows.
 int force_field(sfh_ctx* c, const Geom& g, const double2* U, const double2* X, const double2* Y, double beta,
                 double c11, double c12, unsigned which, double2* F, double2* P, double coef, bool r12);
+// Every extent below 256: the staple walks (force, action, plaquette, HEX)
+// pack each coordinate into 8 bits.
+bool staple_dims_ok(const Geom& g);
 // r12: the links pass the FP64 SU(3) check (ensure_su3), so the staple walks read two rows
 double gauge_action(sfh_ctx* c, const Geom& g, const double2* U, double beta, double c11, double c12, bool r12);
 double avg_plaq(sfh_ctx* c, const Geom& g, const double2* U, bool r12);

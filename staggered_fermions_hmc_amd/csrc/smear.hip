@@ -276,8 +276,7 @@ static int hex_steps(sfh_ctx* c, const Geom& g, double2* U, double2* V1, double2
 
 int hex_smear(sfh_ctx* c, const Geom& g, double2* U, const double alpha[3], int nsteps) {
   if (nsteps == 0) return SFH_OK;
-  for (int d = 0; d < 4; ++d)  // the staple walks pack each coordinate into 8 bits
-    if (g.L[d] > 255) return set_err(c, SFH_E_INVALID, "hex smearing: lattice extents above 255");
+  if (!staple_dims_ok(g)) return set_err(c, SFH_E_INVALID, "hex smearing: lattice extents above 255");
   double2* V1 = static_cast<double2*>(scratch(c, "hex_v1", 108 * (size_t)g.V * sizeof(double2)));
   double2* V2 = static_cast<double2*>(scratch(c, "hex_v2", 108 * (size_t)g.V * sizeof(double2)));
   unsigned* dev = static_cast<unsigned*>(scratch(c, "su3_dev", sizeof(unsigned)));

@@ -288,6 +288,28 @@ def test_gauge_force_non_su3_links_match_oracle(ctx, scale):
         assert np.abs(Fg - Fr).max() < 1e-13 * max(1.0, np.abs(Fr).max())
 
 
+def test_staple_kernels_refuse_extents_above_their_packing(ctx):
+    """The staple walks pack each coordinate into 8 bits: the force, action,
+    plaquette and HEX smearing refuse an extent of 256 and the heatbath one
+    above 252 (SFH_E_INVALID), instead of walking wrong sites; the hop and
+    the solver take such lattices."""
+    dims = (256, 2, 2, 2)
+    U = dev.random_links(dims, 0.3, seed=28)
+    g = dev.Gauge(ctx, dims, U)
+    for call in (lambda: g.force(4.0, 5 / 3, -1 / 12, which=1), lambda: g.action(4.0, 5 / 3, -1 / 12),
+                 lambda: g.avg_plaq(), lambda: g.hex_smear((0.95, 0.76, 0.38), 1)):
+        with pytest.raises(dev.SfhError):
+            call()
+    b = dev.random_spinor(dims, seed=29)
+    x, st = g.cg(b, 0.5, tol=1e-10, precision="f64")
+    r = b - oc.apply_ddag(dims, 0.5, U, x)
+    assert np.sqrt(r @ r / (b @ b)) < 1e-10
+    dims = (256, 4, 4, 4)
+    g = dev.Gauge(ctx, dims, dev.random_links(dims, 0.3, seed=30))
+    with pytest.raises(dev.SfhError):
+        g.heatbath(4.1, 5 / 3, -1 / 12, seed=1, sweep0=0, nsweeps=1)
+
+
 def test_fermion_and_full_force_match_oracle(ctx):
     dims = (4, 4, 4, 8)
     m = 0.1
