@@ -271,7 +271,12 @@ __global__ __launch_bounds__(kBlk) void k_mix_a(Geom g, const typename Prec<T>::
   store_spinor<T>(b.t, b.tn, V, i, res);
 }
 
-template <class T, bool DEFER, bool R12>
+// DM (how the four dots leave the kernel): kDotsFinish, a grid reduction
+// whose finisher advances the state; kDotsPartials (small grids), plain
+// per-workgroup partials that every workgroup of the next kernel A sums;
+// kDotsGroups (split form), the group level of the grid reduction only, whose
+// group sums every workgroup of the next update kernel adds up.
+template <class T, int DM, bool R12>
 __global__ __launch_bounds__(kBlk) void k_mix_b(Geom g, const typename Prec<T>::L* __restrict__ U, MixBufs<T> b,
                                                 float mass, CGState* st, double* partials, int nb, int cur) {
   if (st->done) return;
@@ -320,28 +325,33 @@ __global__ __launch_bounds__(kBlk) void k_mix_b(Geom g, const typename Prec<T>::
       d[3] += rr * rr + ri * ri;
     }
   }
-  if constexpr (DEFER) {
+  if constexpr (DM == kDotsPartials) {
     // per-workgroup partials only: the next kernel A sums them
     block_sum_n<4>(d);
     if (threadIdx.x == 0) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) partials[(size_t)j * nb + s.lb] = d[j];
     }
+  } else if constexpr (DM == kDotsGroups) {
+    group_reduce_n<4>(d, partials, ctr_of(st->ctr, CNT_PV), nb, s.lb);
   } else {
     grid_reduce_n<4>(d, partials, ctr_of(st->ctr, CNT_PV), nb, s.lb, FinFused{st});
   }
 }
 
-// Finisher of the last B of a chunk (DEFER): one workgroup, in place.
+// Finisher of the last B of a chunk when B defers (kDotsPartials or
+// kDotsGroups): one workgroup, in place.
+template <int DM>
 __global__ __launch_bounds__(kBlk) void k_mix_fin(CGState* st, const double* __restrict__ partials, int nb) {
   __shared__ CGState sst;
   Partials4<kPartIters> pl;
-  load_partials4(partials, nb, pl);
+  double d[4];
+  if constexpr (DM == kDotsGroups) load_groups_n<4>(partials, nb, d);
+  else load_partials4(partials, nb, pl);
   state_to_lds(sst, state_word(st));
   __syncthreads();
   if (!sst.fin_pending) return;  // uniform
-  double d[4];
-  add_partials4(pl, d);
+  if constexpr (DM != kDotsGroups) add_partials4(pl, d);
   block_sum_n<4>(d);
   if (threadIdx.x == 0) {
     finish_fused(&sst, d);
@@ -352,24 +362,62 @@ __global__ __launch_bounds__(kBlk) void k_mix_fin(CGState* st, const double* __r
 
 // Split form, own-site part of kernel A: r_k, p_k and x updated in place
 // (r and p single-buffered: both slots point to one array).  The hop
-// t = D^dag p_k then runs as a separate launch.
+// t = D^dag p_k then runs as a separate launch.  The previous B left the
+// group sums of its dots (kDotsGroups): every workgroup adds them up in one
+// fixed order and runs the finisher itself, reading the state from st_in and
+// (workgroup 0) writing the advanced state to st_out, a different copy; the
+// own site's r, v, p are loaded before the state is waited for.  nbb: the
+// workgroups of kernel B.
 template <class T>
-__global__ __launch_bounds__(kBlk) void k_mix_upd(int V, MixBufs<T> b, const CGState* st) {
-  if (st->done) return;
-  const int i = blockIdx.x * kBlk + threadIdx.x;
-  if (i >= V) return;
-  const float al = (float)st->alpha, be = (float)st->beta;
+__global__ __launch_bounds__(kBlk) void k_mix_upd(int V, MixBufs<T> b, const CGState* st_in, CGState* st_out,
+                                                  const double* __restrict__ partials, int nbb) {
+  static_assert(kRedGroups <= 64, "the group sums fit one wave");
+  double d[4];
+  load_groups_n<4>(partials, nbb, d);
+  const unsigned sw = state_word(st_in);
+  const int i0 = blockIdx.x * kBlk + threadIdx.x;
+  const bool valid = i0 < V;
+  const int i = valid ? i0 : 0;
+  SpinorWords<T> rw, vw, pw;
+  rw.load(b.r[0], V, i);
+  vw.load(b.v, V, i);
+  pw.load(b.p[0], V, i);
+  float2 xx[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) xx[c] = b.x[(size_t)c * V + i];
+  __shared__ CGState sst;
+  state_to_lds(sst, sw);
+  __syncthreads();
+  if (sst.fin_pending) {  // uniform over the block
+    // the group sums sit in wave 0; its sum is block_sum_n's (the other
+    // waves would add zeros)
+    if (threadIdx.x < 64) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) d[j] = wave_sum(d[j]);
+      if (threadIdx.x == 0) {
+        finish_fused(&sst, d);
+        sst.fin_pending = 0;
+      }
+    }
+    __syncthreads();
+  }
+  const bool stop = sst.done;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    CGState o = sst;
+    o.fin_pending = stop ? 0 : 1;  // the B that follows leaves group sums
+    *st_out = o;
+  }
+  if (stop || !valid) return;
+  const float al = (float)sst.alpha, be = (float)sst.beta;
   vec3<float> ro, vo, po;
-  load_spinor<T>(b.r[0], b.rn[0], V, i, ro);
-  load_spinor<T>(b.v, b.vn, V, i, vo);
-  load_spinor<T>(b.p[0], b.pn[0], V, i, po);
+  rw.decode(ro);
+  vw.decode(vo);
+  pw.decode(po);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    const size_t j = (size_t)c * V + i;
-    float2 xx = b.x[j];
-    xx.x = __fmaf_rn(al, po.c[c].re, xx.x);
-    xx.y = __fmaf_rn(al, po.c[c].im, xx.y);
-    b.x[j] = xx;
+    xx[c].x = __fmaf_rn(al, po.c[c].re, xx[c].x);
+    xx[c].y = __fmaf_rn(al, po.c[c].im, xx[c].y);
+    b.x[(size_t)c * V + i] = xx[c];
   }
   Q16 qr, qp;
   advance<T>(ro, vo, po, al, be, qr, qp);
@@ -446,24 +494,31 @@ void mix_a(hipStream_t s, const Geom& g, const typename Prec<T>::L* U, const Mix
 }
 template <class T>
 void mix_b(hipStream_t s, const Geom& g, const typename Prec<T>::L* U, const MixBufs<T>& b, float mass, CGState* st,
-           double* partials, bool defer, bool r12, int cur) {
+           double* partials, int dm, bool r12, int cur) {
   const int nb = mix_blocks(g);
   auto go = [&](auto kernel) {
     hipLaunchKernelGGL(kernel, dim3(nb), dim3(kBlk), 0, s, g, U, b, mass, st, partials, nb, cur);
   };
-  if (defer) {
-    if (r12) go(k_mix_b<T, true, true>);
-    else go(k_mix_b<T, true, false>);
+  if (dm == kDotsPartials) {
+    if (r12) go(k_mix_b<T, kDotsPartials, true>);
+    else go(k_mix_b<T, kDotsPartials, false>);
+  } else if (dm == kDotsGroups) {
+    if (r12) go(k_mix_b<T, kDotsGroups, true>);
+    else go(k_mix_b<T, kDotsGroups, false>);
   } else {
-    if (r12) go(k_mix_b<T, false, true>);
-    else go(k_mix_b<T, false, false>);
+    if (r12) go(k_mix_b<T, kDotsFinish, true>);
+    else go(k_mix_b<T, kDotsFinish, false>);
   }
 }
-void mix_fin(hipStream_t s, const Geom& g, CGState* st, const double* partials) {
-  hipLaunchKernelGGL(k_mix_fin, dim3(1), dim3(kBlk), 0, s, st, partials, mix_blocks(g));
+void mix_fin(hipStream_t s, const Geom& g, CGState* st, const double* partials, int dm) {
+  if (dm == kDotsGroups) hipLaunchKernelGGL(k_mix_fin<kDotsGroups>, dim3(1), dim3(kBlk), 0, s, st, partials, mix_blocks(g));
+  else hipLaunchKernelGGL(k_mix_fin<kDotsPartials>, dim3(1), dim3(kBlk), 0, s, st, partials, mix_blocks(g));
 }
-template <class T> void mix_upd(hipStream_t s, const Geom& g, const MixBufs<T>& b, CGState* st) {
-  hipLaunchKernelGGL(k_mix_upd<T>, dim3((g.V + kBlk - 1) / kBlk), dim3(kBlk), 0, s, g.V, b, st);
+template <class T>
+void mix_upd(hipStream_t s, const Geom& g, const MixBufs<T>& b, const CGState* st_in, CGState* st_out,
+             const double* partials) {
+  hipLaunchKernelGGL(k_mix_upd<T>, dim3((g.V + kBlk - 1) / kBlk), dim3(kBlk), 0, s, g.V, b, st_in, st_out, partials,
+                     mix_blocks(g));
 }
 template <class T> void mix_acc(hipStream_t s, const Geom& g, double2* x64, const MixBufs<T>& b, CGState* st) {
   hipLaunchKernelGGL(k_mix_acc<T>, dim3((g.V + kBlk - 1) / kBlk), dim3(kBlk), 0, s, g.V, x64, b, st);
@@ -477,11 +532,12 @@ template void mix_a<float>(hipStream_t, const Geom&, const float2*, const MixBuf
 template void mix_a<Half>(hipStream_t, const Geom&, const __half2*, const MixBufs<Half>&, float, const CGState*,
                           CGState*, const double*, bool, int);
 template void mix_b<float>(hipStream_t, const Geom&, const float2*, const MixBufs<float>&, float, CGState*, double*,
-                           bool, bool, int);
+                           int, bool, int);
 template void mix_b<Half>(hipStream_t, const Geom&, const __half2*, const MixBufs<Half>&, float, CGState*, double*,
-                          bool, bool, int);
-template void mix_upd<float>(hipStream_t, const Geom&, const MixBufs<float>&, CGState*);
-template void mix_upd<Half>(hipStream_t, const Geom&, const MixBufs<Half>&, CGState*);
+                          int, bool, int);
+template void mix_upd<float>(hipStream_t, const Geom&, const MixBufs<float>&, const CGState*, CGState*,
+                             const double*);
+template void mix_upd<Half>(hipStream_t, const Geom&, const MixBufs<Half>&, const CGState*, CGState*, const double*);
 template void mix_acc<float>(hipStream_t, const Geom&, double2*, const MixBufs<float>&, CGState*);
 template void mix_acc<Half>(hipStream_t, const Geom&, double2*, const MixBufs<Half>&, CGState*);
 template void mix_apply<float>(hipStream_t, const Geom&, const double2*, const MixBufs<float>&, CGState*);

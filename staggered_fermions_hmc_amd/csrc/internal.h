@@ -30,7 +30,7 @@ struct CGState {
   int ru_apply;      // mixed precision: restart the sloppy residual and direction
   int rupd;          // reliable updates done
   int converged;     // mixed precision: the FP64 true residual met the target
-  int fin_pending;   // mixed iteration, small grids: the last B's dots still await their finisher
+  int fin_pending;   // mixed iteration, deferred dots: the last B's dots still await their finisher
   int pbuf;          // mixed iteration, fused form: buffer of the latest r_k and p_k
   unsigned* ctr;     // grid-reduction arrival counters (kCtrKinds x kCtrPerKind, device)
 };
@@ -84,7 +84,8 @@ struct sfh_ctx {
   // pinned host staging for scalars
   sfh::CGState* h_state = nullptr;
   // d_state[0]: solves; [1]: reductions, timings; [2]: second copy of a
-  // mixed solve's state (the small-lattice iteration alternates copies);
+  // mixed solve's state (iterations whose dots are finished by the next
+  // kernel alternate copies);
   // [3]: scratch of the mixed-kernel timings
   sfh::CGState* d_state = nullptr;
   double* d_partials = nullptr;  // reduction partials (8 slots of kPartialSlot)

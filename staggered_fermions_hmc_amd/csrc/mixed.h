@@ -28,23 +28,34 @@ template <class T> struct MixBufs {
 template <class T>
 void mix_a(hipStream_t s, const Geom& g, const typename Prec<T>::L* U, const MixBufs<T>& b, float mass,
            const CGState* st_in, CGState* st_out, const double* partials, bool r12, int cur);
-// Whether B leaves its dots as per-workgroup partials for the next A (and
-// mix_fin) to finish, instead of a grid reduction with its own finisher:
-// true when the grid is small enough for every workgroup to read them all.
+// How kernel B's four dots reach the finisher: a grid reduction in B with
+// its own finisher (kDotsFinish); plain per-workgroup partials that every
+// workgroup of the next fused kernel A sums (kDotsPartials: small grids);
+// group sums of the grid reduction's first level that every workgroup of the
+// next split-form update kernel adds up (kDotsGroups).  With either deferred
+// form, iteration j of a chunk reads state copy j & 1 and writes copy
+// (j + 1) & 1.
+constexpr int kDotsFinish = 0, kDotsPartials = 1, kDotsGroups = 2;
+// Whether the fused form's B can leave per-workgroup partials: true when the
+// grid is small enough for every workgroup of A to read them all.
 bool mix_defers(const Geom& g);
-// Finisher of the last B of a chunk when B defers (one workgroup).
-void mix_fin(hipStream_t s, const Geom& g, CGState* st, const double* partials);
+// Finisher of the last B of a chunk when B defers (one workgroup, in place).
+void mix_fin(hipStream_t s, const Geom& g, CGState* st, const double* partials, int dm);
 // Split form of kernel A (for lattices whose neighbour vectors do not stay
 // in cache): the own-site update alone, in place; the caller then runs the
-// plain hop t = D^dag p_k.
-template <class T> void mix_upd(hipStream_t s, const Geom& g, const MixBufs<T>& b, CGState* st);
+// plain hop t = D^dag p_k.  It first finishes the previous B's dots from
+// their group sums (kDotsGroups), reading the state from st_in and writing
+// the advanced state to st_out.
+template <class T>
+void mix_upd(hipStream_t s, const Geom& g, const MixBufs<T>& b, const CGState* st_in, CGState* st_out,
+             const double* partials);
 // Fused form chosen below this many sites; the split form at and above.
 constexpr int kMixFusedMaxV = 1 << 18;
 // Kernel B of iteration k: v = D t and the four dots of FinFused, with the
 // r_k and p_k of buffer cur (the same as the A before it).
 template <class T>
 void mix_b(hipStream_t s, const Geom& g, const typename Prec<T>::L* U, const MixBufs<T>& b, float mass, CGState* st,
-           double* partials, bool defer, bool r12, int cur);
+           double* partials, int dm, bool r12, int cur);
 // Reliable update, first and last steps (gated on the solve state):
 // x64 += x + alpha p_k (p_k from buffer st->pbuf) and x = 0; then r = r64 in
 // sloppy storage, into buffer 1 with p_k moved there, where the next chunk

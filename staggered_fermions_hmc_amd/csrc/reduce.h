@@ -117,6 +117,63 @@ __device__ __forceinline__ void grid_reduce_n(double (&v)[N], double* partials, 
   }
 }
 
+// The group level of grid_reduce_n alone: the last arrival of each group sums
+// its group's partials (in slot order, the same sums as grid_reduce_n's) into
+// partials[j * (nblocks + kRedGroups) + nblocks + grp] and resets the group's
+// counter; the min(nblocks, kRedGroups) group partials are summed by a later
+// kernel (load_groups_n + block_sum_n gives grid_reduce_n's sums bit for bit
+// when nblocks > kDirectSum).  No workgroup waits on a grid-wide counter.
+template <int N>
+__device__ __forceinline__ void group_reduce_n(double (&v)[N], double* partials, unsigned* ctr, int nblocks,
+                                               int slot) {
+  block_sum_n<N>(v);
+  const int ng = nblocks < kRedGroups ? nblocks : kRedGroups;
+  const int grp = slot % ng;
+  const size_t stride = (size_t)nblocks + kRedGroups;
+  __shared__ int group_last;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+      __hip_atomic_store(&partials[j * stride + slot], v[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_s_waitcnt(0);  // the partials' stores have completed before the arrival is counted
+    const unsigned gsize = unsigned(nblocks / ng + (grp < nblocks % ng ? 1 : 0));
+    const unsigned t = __hip_atomic_fetch_add(&ctr[grp * kCtrStride], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    group_last = (t == gsize - 1);
+  }
+  __syncthreads();
+  if (!group_last) return;
+  double s[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) s[j] = 0.0;
+  for (int k = grp + ng * (int)threadIdx.x; k < nblocks; k += ng * (int)blockDim.x) {
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+      s[j] += __hip_atomic_load(&partials[j * stride + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  block_sum_n<N>(s);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+      __hip_atomic_store(&partials[j * stride + nblocks + grp], s[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&ctr[grp * kCtrStride], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// The group partials group_reduce_n left, one group per lane (lanes past the
+// group count hold zeros), for block_sum_n in a kernel after it.
+template <int N>
+__device__ __forceinline__ void load_groups_n(const double* __restrict__ partials, int nblocks, double (&v)[N]) {
+  const int ng = nblocks < kRedGroups ? nblocks : kRedGroups;
+  const size_t stride = (size_t)nblocks + kRedGroups;
+  const int k = (int)threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < N; ++j) v[j] = 0.0;
+  if (k < ng) {  // whole waves past the group count load nothing
+#pragma unroll
+    for (int j = 0; j < N; ++j) v[j] = partials[j * stride + nblocks + k];
+  }
+}
+
 template <class Fin> struct FinOne {
   Fin f;
   __device__ void operator()(const double* s) const { f(s[0]); }

@@ -230,6 +230,14 @@ bool mix_fused(int prec, const Geom& g) {
   return f < 0 ? g.V < kMixFusedMaxV : f == 0;
 }
 
+// How kernel B hands its dots on: the fused form defers them to the next
+// kernel A when every workgroup can read all partials, the split form always
+// defers the last level of the reduction to the next update kernel.
+int dots_mode(bool fused, const Geom& g) {
+  if (!fused) return kDotsGroups;
+  return mix_defers(g) ? kDotsPartials : kDotsFinish;
+}
+
 // Sloppy vectors of a mixed-precision solve on lattice g.  Fused form:
 // kernel A recomputes r_k and p_k at the neighbours, so r and p are double
 // buffers; split form: in-place update, then the plain hop.
@@ -255,7 +263,7 @@ template <class T> int mix_bufs(sfh_ctx* c, const Geom& g, bool fused, MixBufs<T
 // Step A of one mixed iteration: the fused kernel, or the in-place update
 // and the plain hop t = D^dag p_k.
 // st_in / st_out: the state copies the step reads and writes (different
-// only for the fused form on small grids, see mix_defers).
+// whenever the previous B deferred its finisher, see dots_mode).
 template <class T>
 void mix_step_a(sfh_ctx* c, const sfh_gauge* gauge, bool fused, const MixBufs<T>& b, float m, const CGState* st_in,
                 CGState* st_out, const double* partials, int cur) {
@@ -271,7 +279,7 @@ void mix_step_a(sfh_ctx* c, const sfh_gauge* gauge, bool fused, const MixBufs<T>
   CGState* st = st_out;
   {
     KTimer kt(c, "mix_upd");
-    mix_upd<T>(c->stream, g, b, st);
+    mix_upd<T>(c->stream, g, b, st_in, st_out, partials);
   }
   KTimer kt(c, half ? "hop_f16" : "hop_f32");
   if (half)
@@ -421,19 +429,20 @@ static int cg_mixed(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* prm, cons
   const float m = (float)prm->mass;
   const typename Prec<T>::L* U = reinterpret_cast<const typename Prec<T>::L*>(HALF ? (const void*)gauge->lp.u16
                                                                                   : (const void*)gauge->lp.u32);
-  // small grids: iteration j of a chunk reads state copy j & 1 and writes
+  // deferred dots: iteration j of a chunk reads state copy j & 1 and writes
   // copy (j + 1) & 1; the chunk length is even, so the chunk ends in copy 0
-  const bool defer = fused && mix_defers(g);
+  const int dm = dots_mode(fused, g);
+  const bool defer = dm != kDotsFinish;
   static_assert(kChunkMixed<HALF> % 2 == 0, "state copies alternate per iteration");
   CGState* sts[2] = {st, defer ? c->d_state + 2 : st};
   auto iter = [&](int j) {
     mix_step_a<T>(c, gauge, fused, b, m, sts[j & 1], sts[(j + 1) & 1], part, j & 1);
     KTimer kt(c, HALF ? "mix_b_f16" : "mix_b_f32");
-    mix_b<T>(c->stream, g, U, b, m, sts[(j + 1) & 1], part, defer, gauge->lp_su3, j & 1);
+    mix_b<T>(c->stream, g, U, b, m, sts[(j + 1) & 1], part, dm, gauge->lp_su3, j & 1);
   };
   auto tail = [&]() {
     KTimer kt(c, "reliable_update");
-    if (defer) mix_fin(c->stream, g, st, part);
+    if (defer) mix_fin(c->stream, g, st, part, dm);
     mix_acc<T>(c->stream, g, x, b, st);
     hop_f64(c->stream, g, gauge->U, x, t64, prm->mass, -1.0, nullptr, st, nullptr, 1, gauge->f64_su3);
     hop_f64(c->stream, g, gauge->U, t64, ax, prm->mass, +1.0, nullptr, st, nullptr, 1, gauge->f64_su3);
@@ -551,12 +560,12 @@ static int time_mixed_t(sfh_ctx* c, sfh_gauge* gauge, int n, double* us_a, doubl
     *us = 1e3 * ms / n;
     return (int)SFH_OK;
   };
-  // A reads st and (small grids) writes a scratch copy, so every launch
+  // A reads st and (deferred dots) writes a scratch copy, so every launch
   // sees the same state; B changes alpha and beta or only leaves partials
-  const bool defer = fused && mix_defers(g);
-  CGState* st_out = defer ? &c->d_state[3] : st;
+  const int dm = dots_mode(fused, g);
+  CGState* st_out = dm != kDotsFinish ? &c->d_state[3] : st;
   int rc = timed([&]() { mix_step_a<T>(c, gauge, fused, b, m, st, st_out, part, 0); }, us_a);
-  if (!rc) rc = timed([&]() { mix_b<T>(c->stream, g, U, b, m, st, part, defer, gauge->lp_su3, 0); }, us_b);
+  if (!rc) rc = timed([&]() { mix_b<T>(c->stream, g, U, b, m, st, part, dm, gauge->lp_su3, 0); }, us_b);
   reset_aux_state(c);
   return rc;
 }
