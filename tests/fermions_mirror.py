@@ -13,7 +13,7 @@ import dataclasses
 
 import numpy as np
 
-from . import device as _dev
+from staggered_fermions_hmc_amd import device as _dev
 
 
 @dataclasses.dataclass

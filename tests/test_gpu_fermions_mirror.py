@@ -1,4 +1,4 @@
-"""The Python mirror of the reference's field-level API (fermions.py) against
+"""The reference-named calls of tests/fermions_mirror.py against
 the oracle on the reference's default 4^4 lattice: each function returns what
 the reference function of the same name returns (Fermions.h:31-70, :105,
 :130; Lattice.h:75-77), through the HIP library."""
@@ -7,7 +7,7 @@ import pytest
 
 from oracle import oracle as oc
 from staggered_fermions_hmc_amd import device as dev
-from staggered_fermions_hmc_amd import fermions as fm
+import fermions_mirror as fm
 
 pytestmark = pytest.mark.gpu
 
