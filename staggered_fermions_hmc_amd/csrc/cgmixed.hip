@@ -294,21 +294,38 @@ __global__ __launch_bounds__(kBlk) void k_mix_b(Geom g, const typename Prec<T>::
       int fw, bw;
       float sf, sb;
       neighbours<float>(g, s.cb, n, mu, fw, bw, sf, sb);
-      mat3<float> Uf, Ub;
-      vec3<float> tf, tb;
-      load_link<T, R12>(U, V, mu, s.i, Uf);
-      load_spinor<T>(b.t, b.tn, V, s.q + fw, tf);
-      load_link<T, R12>(U, V, mu, s.q + bw, Ub);
-      load_spinor<T>(b.t, b.tn, V, s.q + bw, tb);
-      acc_mv(acc, Uf, tf, sf);
-      acc_mdv(acc, Ub, tb, -sb);
+      if constexpr (kHalfR12<T, R12>) {
+        hop_dir_h2(acc, U, b.t, V, mu, s.i, s.q + fw, s.q + bw, sf, sb);
+      } else {
+        mat3<float> Uf, Ub;
+        vec3<float> tf, tb;
+        load_link<T, R12>(U, V, mu, s.i, Uf);
+        load_spinor<T>(b.t, b.tn, V, s.q + fw, tf);
+        load_link<T, R12>(U, V, mu, s.q + bw, Ub);
+        load_spinor<T>(b.t, b.tn, V, s.q + bw, tb);
+        acc_mv(acc, Uf, tf, sf);
+        acc_mdv(acc, Ub, tb, -sb);
+      }
     }
-    vec3<float> self, v, pk, rk;
-    load_spinor<T>(b.t, b.tn, V, s.i, self);
+    vec3<float> v, pk, rk;
+    if constexpr (kHalfR12<T, R12>) {
+      // m n h + acc / 2 from the record's halves
+      const H16 sr = b.t[s.i];
+      const float mn = mass * sr.n;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      v.c[c].re = mass * self.c[c].re + 0.5f * acc.c[c].re;
-      v.c[c].im = mass * self.c[c].im + 0.5f * acc.c[c].im;
+      for (int c = 0; c < 3; ++c) {
+        const f16x2 h = __builtin_bit_cast(f16x2, sr.c[c]);
+        v.c[c].re = __builtin_fmaf(mn, (float)h.x, 0.5f * acc.c[c].re);
+        v.c[c].im = __builtin_fmaf(mn, (float)h.y, 0.5f * acc.c[c].im);
+      }
+    } else {
+      vec3<float> self;
+      load_spinor<T>(b.t, b.tn, V, s.i, self);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        v.c[c].re = mass * self.c[c].re + 0.5f * acc.c[c].re;
+        v.c[c].im = mass * self.c[c].im + 0.5f * acc.c[c].im;
+      }
     }
     Q16 qv;
     if constexpr (is_half<T>()) quant16(v, qv);

@@ -68,21 +68,38 @@ __global__ __launch_bounds__(256) void k_hop(Geom g, const typename Prec<T>::L* 
         fw = cb + (nm + 1 == L ? -(L - 1) * S : S);
         bw = cb + (nm == 0 ? (L - 1) * S : -S);
       }
-      mat3<R> Uf, Ub;
-      vec3<R> vf, vb;
-      load_link<T, R12>(U, V, mu, i, Uf);
-      load_spinor<T>(in, in_norm, V, q + fw, vf);
-      load_link<T, R12>(U, V, mu, q + bw, Ub);
-      load_spinor<T>(in, in_norm, V, q + bw, vb);
-      acc_mv(acc, Uf, vf, sf);
-      acc_mdv(acc, Ub, vb, -sb);
+      if constexpr (kHalfR12<T, R12>) {
+        hop_dir_h2(acc, U, in, V, mu, i, q + fw, q + bw, sf, sb);
+      } else {
+        mat3<R> Uf, Ub;
+        vec3<R> vf, vb;
+        load_link<T, R12>(U, V, mu, i, Uf);
+        load_spinor<T>(in, in_norm, V, q + fw, vf);
+        load_link<T, R12>(U, V, mu, q + bw, Ub);
+        load_spinor<T>(in, in_norm, V, q + bw, vb);
+        acc_mv(acc, Uf, vf, sf);
+        acc_mdv(acc, Ub, vb, -sb);
+      }
     }
-    vec3<R> self, res;
-    load_spinor<T>(in, in_norm, V, i, self);
+    vec3<R> res;
+    if constexpr (kHalfR12<T, R12>) {
+      // a n h + hb acc from the record's halves
+      const H16 sr = in[i];
+      const R an = a * sr.n;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      res.c[c].re = a * self.c[c].re + hb * acc.c[c].re;
-      res.c[c].im = a * self.c[c].im + hb * acc.c[c].im;
+      for (int c = 0; c < 3; ++c) {
+        const f16x2 h = __builtin_bit_cast(f16x2, sr.c[c]);
+        res.c[c].re = __builtin_fmaf(an, (float)h.x, hb * acc.c[c].re);
+        res.c[c].im = __builtin_fmaf(an, (float)h.y, hb * acc.c[c].im);
+      }
+    } else {
+      vec3<R> self;
+      load_spinor<T>(in, in_norm, V, i, self);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        res.c[c].re = a * self.c[c].re + hb * acc.c[c].re;
+        res.c[c].im = a * self.c[c].im + hb * acc.c[c].im;
+      }
     }
     store_spinor<T>(out, out_norm, V, i, res);
     if constexpr (DOT) {

@@ -27,6 +27,9 @@ template <> struct Prec<float> { using R = float; using L = float2; using S = fl
 // in four arrays, so a two-row read is two loads and a full read four.
 template <> struct Prec<Half> { using R = float; using L = __half2; using S = H16; };
 
+// FP16 records with two-row FP16 links: the hop terms of hop_dir_h2.
+template <class T, bool R12> constexpr bool kHalfR12 = std::is_same<T, Half>::value && R12;
+
 // Spinor storage elements per site: three complex, or one FP16 record.
 template <class T> constexpr int kSpinorElems = std::is_same<T, Half>::value ? 1 : 3;
 
@@ -79,6 +82,133 @@ __device__ __forceinline__ void load_link(const typename Prec<T>::L* __restrict_
     m.e[7] = conj(m.e[2] * m.e[3] - m.e[0] * m.e[5]);
     m.e[8] = conj(m.e[0] * m.e[4] - m.e[1] * m.e[3]);
   }
+}
+
+// ---- FP16 hop terms without conversion instructions.
+// In FP16 storage the hop is bound by vector-ALU issue, not bytes: with each
+// half converted to FP32 before it is multiplied, and every neighbour spinor
+// scaled by its norm component by component, one site cost 1105 VALU
+// instructions per wave.  Here the stored halves go straight into
+// mixed-precision fused multiply-adds (v_fma_mix_f32: FP16 operands, FP32
+// product and accumulation; a product of two halves is exact in FP32), and
+// a neighbour's norm multiplies the 3-vector U h once per link.
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ f16x2 as_h2(unsigned w) { return __builtin_bit_cast(f16x2, w); }
+
+// A two-row FP16 link as stored (rows 0 and 1, six complex halves) and its
+// third row rebuilt in FP32, conj(row0 x row1).
+struct LinkH2 {
+  f16x2 u[6];
+  cpx<float> r2[3];
+};
+
+// conj(a b - c d) for complex halves, in FP32 (the zero addend keeps the
+// first product a fused multiply-add of halves)
+__device__ __forceinline__ cpx<float> cross_conj_h(f16x2 a, f16x2 b, f16x2 c, f16x2 d) {
+  float re = __builtin_fmaf((float)a.x, (float)b.x, 0.0f);
+  re = __builtin_fmaf(-(float)a.y, (float)b.y, re);
+  re = __builtin_fmaf(-(float)c.x, (float)d.x, re);
+  re = __builtin_fmaf((float)c.y, (float)d.y, re);
+  float im = __builtin_fmaf((float)a.x, (float)b.y, 0.0f);
+  im = __builtin_fmaf((float)a.y, (float)b.x, im);
+  im = __builtin_fmaf(-(float)c.x, (float)d.y, im);
+  im = __builtin_fmaf(-(float)c.y, (float)d.x, im);
+  return {re, -im};
+}
+
+__device__ __forceinline__ void load_link_h2(const __half2* __restrict__ U, int V, int mu, int i, LinkH2& m) {
+  const char* base = reinterpret_cast<const char*>(U) + (size_t)mu * 36 * V;
+  const uint4 a = reinterpret_cast<const uint4*>(base)[i];
+  const uint2 b = reinterpret_cast<const uint2*>(base + (size_t)16 * V)[i];
+  m.u[0] = as_h2(a.x);
+  m.u[1] = as_h2(a.y);
+  m.u[2] = as_h2(a.z);
+  m.u[3] = as_h2(a.w);
+  m.u[4] = as_h2(b.x);
+  m.u[5] = as_h2(b.y);
+  m.r2[0] = cross_conj_h(m.u[1], m.u[5], m.u[2], m.u[4]);
+  m.r2[1] = cross_conj_h(m.u[2], m.u[3], m.u[0], m.u[5]);
+  m.r2[2] = cross_conj_h(m.u[0], m.u[4], m.u[1], m.u[3]);
+}
+
+// t += a b and t += conj(a) b, a an FP16 or FP32 complex, b a complex half
+__device__ __forceinline__ void mac_h(cpx<float>& t, f16x2 a, f16x2 b) {
+  t.re = __builtin_fmaf((float)a.x, (float)b.x, t.re);
+  t.re = __builtin_fmaf(-(float)a.y, (float)b.y, t.re);
+  t.im = __builtin_fmaf((float)a.x, (float)b.y, t.im);
+  t.im = __builtin_fmaf((float)a.y, (float)b.x, t.im);
+}
+__device__ __forceinline__ void mac_h(cpx<float>& t, cpx<float> a, f16x2 b) {
+  t.re = __builtin_fmaf(a.re, (float)b.x, t.re);
+  t.re = __builtin_fmaf(-a.im, (float)b.y, t.re);
+  t.im = __builtin_fmaf(a.re, (float)b.y, t.im);
+  t.im = __builtin_fmaf(a.im, (float)b.x, t.im);
+}
+__device__ __forceinline__ void macc_h(cpx<float>& t, f16x2 a, f16x2 b) {
+  t.re = __builtin_fmaf((float)a.x, (float)b.x, t.re);
+  t.re = __builtin_fmaf((float)a.y, (float)b.y, t.re);
+  t.im = __builtin_fmaf((float)a.x, (float)b.y, t.im);
+  t.im = __builtin_fmaf(-(float)a.y, (float)b.x, t.im);
+}
+__device__ __forceinline__ void macc_h(cpx<float>& t, cpx<float> a, f16x2 b) {
+  t.re = __builtin_fmaf(a.re, (float)b.x, t.re);
+  t.re = __builtin_fmaf(a.im, (float)b.y, t.re);
+  t.im = __builtin_fmaf(a.re, (float)b.y, t.im);
+  t.im = __builtin_fmaf(-a.im, (float)b.x, t.im);
+}
+
+__device__ __forceinline__ void acc_add(vec3<float>& acc, int a, float sn, cpx<float> t) {
+  acc.c[a].re = __builtin_fmaf(sn, t.re, acc.c[a].re);
+  acc.c[a].im = __builtin_fmaf(sn, t.im, acc.c[a].im);
+}
+
+// acc += s U v for the spinor v = n h of FP16 record r (h its halves, n its
+// norm): U h is formed from the halves, then scaled by s n.
+__device__ __forceinline__ void acc_mv_h(vec3<float>& acc, const LinkH2& U, const H16& r, float s) {
+  const f16x2 h0 = __builtin_bit_cast(f16x2, r.c[0]), h1 = __builtin_bit_cast(f16x2, r.c[1]),
+              h2 = __builtin_bit_cast(f16x2, r.c[2]);
+  const float sn = s * r.n;
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    cpx<float> t = {0.0f, 0.0f};
+    mac_h(t, U.u[3 * a], h0);
+    mac_h(t, U.u[3 * a + 1], h1);
+    mac_h(t, U.u[3 * a + 2], h2);
+    acc_add(acc, a, sn, t);
+  }
+  cpx<float> t = {0.0f, 0.0f};
+  mac_h(t, U.r2[0], h0);
+  mac_h(t, U.r2[1], h1);
+  mac_h(t, U.r2[2], h2);
+  acc_add(acc, 2, sn, t);
+}
+// acc += s U^dag v, v = n h as above: (U^dag h)_a = sum_b conj(U_ba) h_b.
+__device__ __forceinline__ void acc_mdv_h(vec3<float>& acc, const LinkH2& U, const H16& r, float s) {
+  const f16x2 h0 = __builtin_bit_cast(f16x2, r.c[0]), h1 = __builtin_bit_cast(f16x2, r.c[1]),
+              h2 = __builtin_bit_cast(f16x2, r.c[2]);
+  const float sn = s * r.n;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    cpx<float> t = {0.0f, 0.0f};
+    macc_h(t, U.u[a], h0);
+    macc_h(t, U.u[3 + a], h1);
+    macc_h(t, U.r2[a], h2);
+    acc_add(acc, a, sn, t);
+  }
+}
+
+// One direction of the hop for FP16 records and two-row FP16 links:
+// acc += sf U_mu(i) in(f) - sb U_mu(b)^dag in(b), the loads issued first.
+__device__ __forceinline__ void hop_dir_h2(vec3<float>& acc, const __half2* __restrict__ U,
+                                           const H16* __restrict__ in, int V, int mu, int i, int f, int b, float sf,
+                                           float sb) {
+  LinkH2 Uf, Ub;
+  load_link_h2(U, V, mu, i, Uf);
+  const H16 vf = in[f];
+  load_link_h2(U, V, mu, b, Ub);
+  const H16 vb = in[b];
+  acc_mv_h(acc, Uf, vf, sf);
+  acc_mdv_h(acc, Ub, vb, -sb);
 }
 
 // nrm: unused (the FP16 record carries its norm); kept so one call serves
