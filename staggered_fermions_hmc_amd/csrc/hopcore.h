@@ -51,6 +51,37 @@ __device__ __forceinline__ cpx<float> h2c(unsigned w) {
   return {v.x, v.y};
 }
 
+// FP32 complex arithmetic as chains of fused multiply-adds: the plain
+// expressions compile to separate multiplies, adds and subtracts (252, 102
+// and 54 of the FP32 hop's 944 VALU instructions per wave), and at small
+// lattices, one wave per SIMD, a wave's own instruction stream is on the
+// critical path.
+// conj(a b - c d)
+__device__ __forceinline__ cpx<float> cross_conj_f(cpx<float> a, cpx<float> b, cpx<float> c, cpx<float> d) {
+  float re = __builtin_fmaf(a.re, b.re, 0.0f);
+  re = __builtin_fmaf(-a.im, b.im, re);
+  re = __builtin_fmaf(-c.re, d.re, re);
+  re = __builtin_fmaf(c.im, d.im, re);
+  float im = __builtin_fmaf(a.re, b.im, 0.0f);
+  im = __builtin_fmaf(a.im, b.re, im);
+  im = __builtin_fmaf(-c.re, d.im, im);
+  im = __builtin_fmaf(-c.im, d.re, im);
+  return {re, -im};
+}
+// t += a b, t += conj(a) b
+__device__ __forceinline__ void mac_f(cpx<float>& t, cpx<float> a, cpx<float> b) {
+  t.re = __builtin_fmaf(a.re, b.re, t.re);
+  t.re = __builtin_fmaf(-a.im, b.im, t.re);
+  t.im = __builtin_fmaf(a.re, b.im, t.im);
+  t.im = __builtin_fmaf(a.im, b.re, t.im);
+}
+__device__ __forceinline__ void macc_f(cpx<float>& t, cpx<float> a, cpx<float> b) {
+  t.re = __builtin_fmaf(a.re, b.re, t.re);
+  t.re = __builtin_fmaf(a.im, b.im, t.re);
+  t.im = __builtin_fmaf(a.re, b.im, t.im);
+  t.im = __builtin_fmaf(-a.im, b.re, t.im);
+}
+
 // R12: read the first two rows only and rebuild the third as
 // conj(row0 x row1), exact for SU(3) links (two thirds of the link bytes).
 template <class T, bool R12 = false>
@@ -78,9 +109,15 @@ __device__ __forceinline__ void load_link(const typename Prec<T>::L* __restrict_
     for (int e = 0; e < (R12 ? 6 : 9); ++e) m.e[e] = ldc(U + (size_t)(mu * 9 + e) * V + i);
   }
   if constexpr (R12) {
-    m.e[6] = conj(m.e[1] * m.e[5] - m.e[2] * m.e[4]);
-    m.e[7] = conj(m.e[2] * m.e[3] - m.e[0] * m.e[5]);
-    m.e[8] = conj(m.e[0] * m.e[4] - m.e[1] * m.e[3]);
+    if constexpr (std::is_same<typename Prec<T>::R, float>::value) {
+      m.e[6] = cross_conj_f(m.e[1], m.e[5], m.e[2], m.e[4]);
+      m.e[7] = cross_conj_f(m.e[2], m.e[3], m.e[0], m.e[5]);
+      m.e[8] = cross_conj_f(m.e[0], m.e[4], m.e[1], m.e[3]);
+    } else {
+      m.e[6] = conj(m.e[1] * m.e[5] - m.e[2] * m.e[4]);
+      m.e[7] = conj(m.e[2] * m.e[3] - m.e[0] * m.e[5]);
+      m.e[8] = conj(m.e[0] * m.e[4] - m.e[1] * m.e[3]);
+    }
   }
 }
 
@@ -286,11 +323,20 @@ template <class R>
 __device__ __forceinline__ void acc_mv(vec3<R>& acc, const mat3<R>& U, const vec3<R>& v, R s) {
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    cpx<R> t = U.e[3 * a] * v.c[0];
-    cmac(t, U.e[3 * a + 1], v.c[1]);
-    cmac(t, U.e[3 * a + 2], v.c[2]);
-    acc.c[a].re += s * t.re;
-    acc.c[a].im += s * t.im;
+    if constexpr (std::is_same<R, float>::value) {
+      cpx<float> t = {0.0f, 0.0f};
+      mac_f(t, U.e[3 * a], v.c[0]);
+      mac_f(t, U.e[3 * a + 1], v.c[1]);
+      mac_f(t, U.e[3 * a + 2], v.c[2]);
+      acc.c[a].re = __builtin_fmaf(s, t.re, acc.c[a].re);
+      acc.c[a].im = __builtin_fmaf(s, t.im, acc.c[a].im);
+    } else {
+      cpx<R> t = U.e[3 * a] * v.c[0];
+      cmac(t, U.e[3 * a + 1], v.c[1]);
+      cmac(t, U.e[3 * a + 2], v.c[2]);
+      acc.c[a].re += s * t.re;
+      acc.c[a].im += s * t.im;
+    }
   }
 }
 // acc += s * U^dag v
@@ -298,11 +344,20 @@ template <class R>
 __device__ __forceinline__ void acc_mdv(vec3<R>& acc, const mat3<R>& U, const vec3<R>& v, R s) {
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    cpx<R> t = conj(U.e[a]) * v.c[0];
-    cmacc(t, U.e[3 + a], v.c[1]);
-    cmacc(t, U.e[6 + a], v.c[2]);
-    acc.c[a].re += s * t.re;
-    acc.c[a].im += s * t.im;
+    if constexpr (std::is_same<R, float>::value) {
+      cpx<float> t = {0.0f, 0.0f};
+      macc_f(t, U.e[a], v.c[0]);
+      macc_f(t, U.e[3 + a], v.c[1]);
+      macc_f(t, U.e[6 + a], v.c[2]);
+      acc.c[a].re = __builtin_fmaf(s, t.re, acc.c[a].re);
+      acc.c[a].im = __builtin_fmaf(s, t.im, acc.c[a].im);
+    } else {
+      cpx<R> t = conj(U.e[a]) * v.c[0];
+      cmacc(t, U.e[3 + a], v.c[1]);
+      cmacc(t, U.e[6 + a], v.c[2]);
+      acc.c[a].re += s * t.re;
+      acc.c[a].im += s * t.im;
+    }
   }
 }
 
