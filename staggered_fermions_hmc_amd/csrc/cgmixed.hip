@@ -409,8 +409,7 @@ __global__ __launch_bounds__(kBlk) void k_mix_upd(int V, MixBufs<T> b, const CGS
     // the group sums sit in wave 0; its sum is block_sum_n's (the other
     // waves would add zeros)
     if (threadIdx.x < 64) {
-#pragma unroll
-      for (int j = 0; j < 4; ++j) d[j] = wave_sum(d[j]);
+      wave_sum4_all(d);
       if (threadIdx.x == 0) {
         finish_fused(&sst, d);
         sst.fin_pending = 0;

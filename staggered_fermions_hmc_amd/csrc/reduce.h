@@ -35,15 +35,50 @@ namespace sfh {
 // N * (nblocks + kRedGroups) doubles.  fin receives the N sums.
 constexpr int kDirectSum = 1024;
 
+// Four sums over the 64 lanes of a wave: the lanes first exchange half of
+// their values with the partner across lane bit 0 and then across bit 1, so
+// each lane carries one value into the four butterfly steps that remain:
+// 7 exchanges of a double instead of 24 (each a cross-lane permute through
+// the LDS crossbar).  Every lane gets the total of value
+// slot = 2 (lane & 1) + ((lane >> 1) & 1), the same bits in every lane of
+// that slot; the order of the additions is fixed.
+__device__ __forceinline__ double wave_sum4(const double (&v)[4], int& slot) {
+  const int l = threadIdx.x & 63;
+  const bool b0 = l & 1, b1 = (l >> 1) & 1;
+  const double w0 = (b0 ? v[2] : v[0]) + __shfl_xor(b0 ? v[0] : v[2], 1, 64);
+  const double w1 = (b0 ? v[3] : v[1]) + __shfl_xor(b0 ? v[1] : v[3], 1, 64);
+  double x = (b1 ? w1 : w0) + __shfl_xor(b1 ? w0 : w1, 2, 64);
+#pragma unroll
+  for (int off = 4; off < 64; off <<= 1) x += __shfl_xor(x, off, 64);
+  slot = (b0 ? 2 : 0) + (b1 ? 1 : 0);
+  return x;
+}
+// The four totals of wave_sum4 in every lane of the wave (lane 2 a + b
+// holds slot 2 b + a).
+__device__ __forceinline__ void wave_sum4_all(double (&v)[4]) {
+  int slot;
+  const double x = wave_sum4(v, slot);
+  v[0] = __shfl(x, 0, 64);
+  v[1] = __shfl(x, 2, 64);
+  v[2] = __shfl(x, 1, 64);
+  v[3] = __shfl(x, 3, 64);
+}
+
 template <int N>
 __device__ __forceinline__ void block_sum_n(double (&v)[N]) {
   __shared__ double ws[16][N];
-#pragma unroll
-  for (int j = 0; j < N; ++j) v[j] = wave_sum(v[j]);
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0) {
+  if constexpr (N == 4) {
+    int slot;
+    const double x = wave_sum4(v, slot);
+    if (lane < 4) ws[w][slot] = x;
+  } else {
 #pragma unroll
-    for (int j = 0; j < N; ++j) ws[w][j] = v[j];
+    for (int j = 0; j < N; ++j) v[j] = wave_sum(v[j]);
+    if (lane == 0) {
+#pragma unroll
+      for (int j = 0; j < N; ++j) ws[w][j] = v[j];
+    }
   }
   __syncthreads();
   const int nw = blockDim.x >> 6;
