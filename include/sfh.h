@@ -224,6 +224,15 @@ int sfh_field_download(sfh_ctx* ctx, const sfh_field* f, double* host);
  * apply_dirac_dagger_at_site_s (Fermions.cpp:218-288) as used by
  * generate_phi_field (:302-314) and solve_Y_field (:623-633). */
 int sfh_apply_d(sfh_ctx* ctx, const sfh_gauge* g, double mass, int dagger, const double* in, double* out);
+/* out = D in or D^dagger in applied in a storage precision of the mixed CG's
+ * inner iteration: SFH_PREC_F32 (FP32 spinors and links) or SFH_PREC_F16
+ * (FP16 records with per-site norms, FP16 links; FP32 arithmetic), the same
+ * kernels the inner iterations run, with in converted from FP64 and out
+ * converted back; SFH_PREC_F64 is sfh_apply_d.  No counterpart in the
+ * reference (which works in FP64 only): it exposes the low-precision
+ * operator for checks against the FP64 one and for callers that want it. */
+int sfh_apply_d_prec(sfh_ctx* ctx, const sfh_gauge* g, double mass, int dagger, int precision, const double* in,
+                     double* out);
 /* out = D (D^dagger in).  Replaces apply_DDdagger (Fermions.cpp:530-547). */
 int sfh_apply_ddag(sfh_ctx* ctx, const sfh_gauge* g, double mass, const double* in, double* out);
 /* Device-resident variants of the two above. */

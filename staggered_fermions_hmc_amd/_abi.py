@@ -87,6 +87,7 @@ SIGNATURES = {
     "sfh_field_upload": (_I, [_P, _P, _P]),
     "sfh_field_download": (_I, [_P, _P, _P]),
     "sfh_apply_d": (_I, [_P, _P, _D, _I, _P, _P]),
+    "sfh_apply_d_prec": (_I, [_P, _P, _D, _I, _I, _P, _P]),
     "sfh_apply_ddag": (_I, [_P, _P, _D, _P, _P]),
     "sfh_apply_d_field": (_I, [_P, _P, _D, _I, _P, _P]),
     "sfh_inner": (_I, [_P, _SZ, _P, _P, ctypes.POINTER(_D)]),

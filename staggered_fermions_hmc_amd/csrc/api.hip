@@ -660,6 +660,41 @@ int sfh_apply_d(sfh_ctx* c, const sfh_gauge* g, double mass, int dagger, const d
   return download_spinor(c, g->g, b, out);
 }
 
+int sfh_apply_d_prec(sfh_ctx* c, const sfh_gauge* g, double mass, int dagger, int precision, const double* in,
+                     double* out) {
+  if (!c || !g || !in || !out) return SFH_E_INVALID;
+  if (precision == SFH_PREC_F64) return sfh_apply_d(c, g, mass, dagger, in, out);
+  if (precision != SFH_PREC_F32 && precision != SFH_PREC_F16) return set_err(c, SFH_E_INVALID, "unknown precision");
+  Guard gd(c);
+  const Geom& gg = g->g;
+  const size_t n = 3 * (size_t)gg.V;
+  double2 *a = dbuf(c, "io_a", n), *b = dbuf(c, "io_b", n);
+  if (!a || !b) return SFH_E_NOMEM;
+  sfh_gauge* gm = const_cast<sfh_gauge*>(g);
+  if (int e = upload_spinor(c, gg, in, a)) return e;
+  if (int e = ensure_lp(c, gm, precision)) return e;
+  const float fb = dagger ? -1.0f : 1.0f;
+  if (precision == SFH_PREC_F32) {
+    float2* s0 = static_cast<float2*>(scratch(c, "lp_io_a", n * sizeof(float2)));
+    float2* s1 = static_cast<float2*>(scratch(c, "lp_io_b", n * sizeof(float2)));
+    if (!s0 || !s1) return SFH_E_NOMEM;
+    f64_to_f32(c->stream, a, s0, n);
+    KTimer kt(c, "hop_f32");
+    hop_f32(c->stream, gg, gm->lp.u32, s0, s1, (float)mass, fb, nullptr, nullptr, nullptr, gm->lp_su3);
+    f32_to_f64(c->stream, s1, b, n);
+  } else {
+    H16* s0 = static_cast<H16*>(scratch(c, "lp_io_a16", (size_t)gg.V * sizeof(H16)));
+    H16* s1 = static_cast<H16*>(scratch(c, "lp_io_b16", (size_t)gg.V * sizeof(H16)));
+    if (!s0 || !s1) return SFH_E_NOMEM;
+    f64_to_f16(c->stream, gg, a, s0);
+    KTimer kt(c, "hop_f16");
+    hop_f16(c->stream, gg, gm->lp.u16, s0, s1, (float)mass, fb, nullptr, nullptr, nullptr, gm->lp_su3);
+    f16_to_f64(c->stream, gg, s1, b);
+  }
+  if (int e = hip_check(c, hipGetLastError(), "low-precision hop")) return e;
+  return download_spinor(c, gg, b, out);
+}
+
 int sfh_apply_ddag(sfh_ctx* c, const sfh_gauge* g, double mass, const double* in, double* out) {
   if (!c || !g || !in || !out) return SFH_E_INVALID;
   Guard gd(c);

@@ -128,6 +128,28 @@ void f64_to_f16(hipStream_t s, const Geom& g, const double2* in, H16* out) {
   hipLaunchKernelGGL(k_f64_to_f16, dim3(nblk(g.V)), dim3(kB), 0, s, g.V, in, out);
 }
 
+// Back to FP64: FP32 spinors and FP16 records (halves times the site's norm).
+__global__ void k_f32_to_f64(const float2* __restrict__ in, double2* __restrict__ out, size_t n) {
+  const size_t i = (size_t)blockIdx.x * kB + threadIdx.x;
+  if (i < n) out[i] = make_double2((double)in[i].x, (double)in[i].y);
+}
+void f32_to_f64(hipStream_t s, const float2* in, double2* out, size_t n) {
+  hipLaunchKernelGGL(k_f32_to_f64, dim3(nblk(n)), dim3(kB), 0, s, in, out, n);
+}
+__global__ void k_f16_to_f64(int V, const H16* __restrict__ in, double2* __restrict__ out) {
+  const int i = blockIdx.x * kB + threadIdx.x;
+  if (i >= V) return;
+  const H16 r = in[i];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float2 t = __half22float2(r.c[c]);
+    out[(size_t)c * V + i] = make_double2((double)(r.n * t.x), (double)(r.n * t.y));
+  }
+}
+void f16_to_f64(hipStream_t s, const Geom& g, const H16* in, double2* out) {
+  hipLaunchKernelGGL(k_f16_to_f64, dim3(nblk(g.V)), dim3(kB), 0, s, g.V, in, out);
+}
+
 // FP16 records of (1 + i) in every colour (halves 1.0, norm 1.0).
 __global__ void k_h16_ones(H16* __restrict__ f, int V) {
   const int i = blockIdx.x * kB + threadIdx.x;

@@ -185,6 +185,8 @@ void cg_p_f32(hipStream_t s, const Geom& g, float2* p, const float2* r, CGState*
 // mixed-precision support
 void f64_to_f32(hipStream_t s, const double2* in, float2* out, size_t n);
 void f64_to_f16(hipStream_t s, const Geom& g, const double2* in, H16* out);
+void f32_to_f64(hipStream_t s, const float2* in, double2* out, size_t n);
+void f16_to_f64(hipStream_t s, const Geom& g, const H16* in, double2* out);  // halves times the record's norm
 void h16_ones(hipStream_t s, H16* f, int V);  // every colour (1 + i)
 // *h_state (pinned host memory) = *d_state, on the stream
 void state_out(hipStream_t s, const CGState* d_state, CGState* h_state);

@@ -136,10 +136,16 @@ class Gauge:
         assert a.size == self.V * 6, (a.size, self.V * 6)
         return a
 
-    def apply_d(self, mass, psi, dagger=False):
+    def apply_d(self, mass, psi, dagger=False, precision="f64"):
+        """D psi (or D^dagger psi); precision "f32" / "f16" applies the mixed
+        CG's inner-iteration operator (storage in that precision)."""
         a = self._spinor(psi)
         out = np.empty_like(a)
-        self.ctx.check(self.ctx._lib.sfh_apply_d(self.ctx.h, self.h, mass, int(bool(dagger)), ptr(a), ptr(out)))
+        if precision == "f64":
+            self.ctx.check(self.ctx._lib.sfh_apply_d(self.ctx.h, self.h, mass, int(bool(dagger)), ptr(a), ptr(out)))
+        else:
+            self.ctx.check(self.ctx._lib.sfh_apply_d_prec(self.ctx.h, self.h, mass, int(bool(dagger)),
+                                                          _abi.PREC[precision], ptr(a), ptr(out)))
         return out
 
     def apply_ddag(self, mass, psi):

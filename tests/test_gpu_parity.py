@@ -85,6 +85,68 @@ def test_hop_forms_agree(ctx, dims, prec):
             assert rel(sol[k], sol[0]) < 1e-8, k
 
 
+def _stored_links(U, dtype, two_rows):
+    """The low-precision link copy as the library stores and reads it: each
+    real rounded to nearest from double to float or half (the compiled
+    conversion to half rounds once, from double), and with SU(3) links (the
+    two-row read) the third row of every link rebuilt as conj(row0 x row1)
+    from the stored rows."""
+    q = U.astype(dtype).astype(np.float64)
+    if two_rows:
+        M = (q[0::2] + 1j * q[1::2]).reshape(-1, 3, 3)
+        M[:, 2, :] = np.conj(np.cross(M[:, 0, :], M[:, 1, :]))
+        q = np.empty_like(U)
+        q[0::2], q[1::2] = M.real.reshape(-1), M.imag.reshape(-1)
+    return q
+
+
+def _stored_f16_spinor(psi):
+    """FP16 records as the library writes them: FP32 components, the site's
+    max-norm n (FP32), halves of v * (1 / n) rounded to nearest; returns
+    n * h per component."""
+    v = psi.astype(np.float32).reshape(-1, 6)
+    n = np.abs(v).max(axis=1)
+    inv = np.where(n > 0, np.float32(1) / np.where(n > 0, n, np.float32(1)), np.float32(0)).astype(np.float32)
+    h = (v * inv[:, None]).astype(np.float16)
+    return (n[:, None].astype(np.float64) * h.astype(np.float64)).reshape(-1)
+
+
+@pytest.mark.parametrize("dims", [(4, 4, 4, 4), (8, 6, 4, 4), (16, 16, 16, 16)])
+@pytest.mark.parametrize("su3", [True, False])
+def test_low_precision_hop_matches_oracle(ctx, dims, su3):
+    """D and D^dag in the inner iteration's storage (the kernels the mixed
+    CG runs: FP16 products from the stored halves, FP32 arithmetic).
+    Against the FP64 oracle on the exact inputs: FP32 within 1e-6 and FP16
+    within 4e-3 of max|D psi| (2^-11 relative rounding of every stored link
+    element and spinor component, a few dozen terms per site).  Against the
+    FP64 oracle applied to the stored (quantised) links and spinor, FP16 to
+    within half an FP16 unit of the output's own storage (2^-12 of each
+    site's norm) plus FP32 accumulation (measured: 2.443e-4 at most, 16^4),
+    and FP32 to 5e-7 (measured 1.9e-7): this pins the arithmetic on the
+    stored values, norms, signs and rebuilt rows, not only the precision.
+    su3 False: links scaled by 0.9, which take the three-row reads."""
+    m = 0.1
+    U = dev.random_links(dims, 0.3, seed=61)
+    if not su3:
+        U = 0.9 * U
+    psi = dev.random_spinor(dims, seed=62)
+    g = dev.Gauge(ctx, dims, U)
+    for dg in (False, True):
+        exact = oc.apply_d(dims, m, U, psi, dagger=dg)
+        d32 = g.apply_d(m, psi, dagger=dg, precision="f32")
+        assert rel(d32, exact) < 1e-6, dg
+        d16 = g.apply_d(m, psi, dagger=dg, precision="f16")
+        assert rel(d16, exact) < 4e-3, dg
+        emul = oc.apply_d(dims, m, _stored_links(U, np.float16, su3), _stored_f16_spinor(psi), dagger=dg)
+        e = emul.reshape(-1, 6)
+        site_norm = np.abs(e).max(axis=1)
+        diff = np.abs(d16.reshape(-1, 6) - e).max(axis=1)
+        assert np.all(diff <= (2.0 ** -12 + 2e-5) * site_norm), (dg, (diff / site_norm).max())
+        emul32 = oc.apply_d(dims, m, _stored_links(U, np.float32, su3), psi.astype(np.float32).astype(np.float64),
+                            dagger=dg)
+        assert rel(d32, emul32) < 5e-7, dg
+
+
 def test_dslash_delta_source_stencil(ctx):
     """The oracle's hand-derived stencil cases, on the device (to 4e-15: the
     FP64 hop rebuilds the third row of SU(3) links from the first two, which
