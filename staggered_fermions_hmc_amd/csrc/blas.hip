@@ -225,32 +225,4 @@ void sub_f64(hipStream_t s, double2* out, const double2* a, const double2* b, si
   hipLaunchKernelGGL(k_sub, dim3(nblk(n)), dim3(kB), 0, s, out, a, b, n);
 }
 
-// ------------------------------------------------------------------ device-side reliable update
-// The FP64 true residual of a reliable update, r64 = phi - D D^dag x, and its
-// norm (finisher FinRU); returns at once unless an update is pending, so the
-// mixed solver can run it at the end of every chunk with no host round trip.
-__global__ __launch_bounds__(kB) void k_ru_sub(int V, double2* __restrict__ r, const double2* __restrict__ phi,
-                                               const double2* __restrict__ ax, CGState* st, double* partials, int nb) {
-  if (!st->ru_pending) return;
-  const int i = blockIdx.x * kB + threadIdx.x;
-  double rr = 0.0;
-  if (i < V) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const size_t j = (size_t)c * V + i;
-      const double2 a = phi[j], b = ax[j];
-      const double2 d = make_double2(a.x - b.x, a.y - b.y);
-      r[j] = d;
-      rr += d.x * d.x + d.y * d.y;
-    }
-  }
-  grid_reduce(rr, partials, ctr_of(st->ctr, CNT_RU), nb, blockIdx.x, FinRU{st});
-}
-
-void ru_sub(hipStream_t s, const Geom& g, double2* r, const double2* phi, const double2* ax, CGState* st,
-            double* partials) {
-  const unsigned nb = nblk(g.V);
-  hipLaunchKernelGGL(k_ru_sub, dim3(nb), dim3(kB), 0, s, g.V, r, phi, ax, st, partials, (int)nb);
-}
-
 }  // namespace sfh

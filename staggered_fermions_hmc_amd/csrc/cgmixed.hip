@@ -486,6 +486,55 @@ __global__ __launch_bounds__(kBlk) void k_mix_apply(int V, const double2* __rest
 
 inline int mix_blocks(const Geom& g) { return (g.Vh + kBlk / 2 - 1) / (kBlk / 2); }
 
+// Second FP64 hop of the reliable update with its residual: r = phi - D t
+// (D = m + the hop, t = D^dag x) at every site, and |r|^2 reduced with
+// FinRU (converged, out of budget, or restart), so D t never goes to
+// memory.  Same site map and per-site arithmetic as the FP64 hop; gated on
+// a pending update like the other steps of the chunk's tail.
+template <bool R12>
+__global__ __launch_bounds__(kBlk) void k_ru_resid(Geom g, const double2* __restrict__ U,
+                                                   const double2* __restrict__ t, const double2* __restrict__ phi,
+                                                   double2* __restrict__ r, double mass, CGState* st,
+                                                   double* partials, int nb) {
+  if (!st->ru_pending) return;
+  const Site s = site_of(g, nb);
+  double rr = 0.0;
+  if (s.cb < g.Vh) {
+    const int V = g.V;
+    int n[4];
+    cb_coords(g, s.cb, s.p, n);
+    vec3<double> acc;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc.c[c] = {0.0, 0.0};
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) {
+      int fw, bw;
+      double sf, sb;
+      neighbours<double>(g, s.cb, n, mu, fw, bw, sf, sb);
+      mat3<double> Uf, Ub;
+      vec3<double> vf, vb;
+      load_link<double, R12>(U, V, mu, s.i, Uf);
+      load_spinor<double>(t, nullptr, V, s.q + fw, vf);
+      load_link<double, R12>(U, V, mu, s.q + bw, Ub);
+      load_spinor<double>(t, nullptr, V, s.q + bw, vb);
+      acc_mv(acc, Uf, vf, sf);
+      acc_mdv(acc, Ub, vb, -sb);
+    }
+    vec3<double> self;
+    load_spinor<double>(t, nullptr, V, s.i, self);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const size_t j = (size_t)c * V + s.i;
+      const double2 f = phi[j];
+      const double2 d = make_double2(f.x - (mass * self.c[c].re + 0.5 * acc.c[c].re),
+                                     f.y - (mass * self.c[c].im + 0.5 * acc.c[c].im));
+      r[j] = d;
+      rr += d.x * d.x + d.y * d.y;
+    }
+  }
+  grid_reduce(rr, partials, ctr_of(st->ctr, CNT_RU), nb, s.lb, FinRU{st});
+}
+
 }  // namespace
 
 bool mix_defers(const Geom& g) { return mix_blocks(g) <= kDirectSum; }
@@ -525,6 +574,14 @@ void mix_b(hipStream_t s, const Geom& g, const typename Prec<T>::L* U, const Mix
     if (r12) go(k_mix_b<T, kDotsFinish, true>);
     else go(k_mix_b<T, kDotsFinish, false>);
   }
+}
+void ru_resid(hipStream_t s, const Geom& g, const double2* U, const double2* t, const double2* phi, double2* r,
+              double mass, CGState* st, double* partials, bool r12) {
+  const int nb = mix_blocks(g);
+  if (r12)
+    hipLaunchKernelGGL(k_ru_resid<true>, dim3(nb), dim3(kBlk), 0, s, g, U, t, phi, r, mass, st, partials, nb);
+  else
+    hipLaunchKernelGGL(k_ru_resid<false>, dim3(nb), dim3(kBlk), 0, s, g, U, t, phi, r, mass, st, partials, nb);
 }
 void mix_fin(hipStream_t s, const Geom& g, CGState* st, const double* partials, int dm) {
   if (dm == kDotsGroups) hipLaunchKernelGGL(k_mix_fin<kDotsGroups>, dim3(1), dim3(kBlk), 0, s, st, partials, mix_blocks(g));

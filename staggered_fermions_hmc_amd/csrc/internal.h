@@ -194,9 +194,6 @@ void acc_f32_into_f64(hipStream_t s, double2* xh, float2* xl, size_t n);  // xh 
 void xpay_f64(hipStream_t s, double2* y, const double2* x, double a, size_t n);  // y = x + a*y
 void axpy_f64(hipStream_t s, double2* y, const double2* x, double a, size_t n);  // y += a*x
 void sub_f64(hipStream_t s, double2* out, const double2* a, const double2* b, size_t n);  // out = a - b
-// Device-side reliable update, gated on the solve state (blas.hip).
-void ru_sub(hipStream_t s, const Geom& g, double2* r, const double2* phi, const double2* ax, CGState* st,
-            double* partials);
 
 // ------------------------------------------------------------------ gauge (gauge.hip)
 // r12: the links are SU(3) to kSu3TolF64; the gauge force reads two rows.

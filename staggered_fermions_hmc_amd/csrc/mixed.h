@@ -61,6 +61,12 @@ void mix_b(hipStream_t s, const Geom& g, const typename Prec<T>::L* U, const Mix
 // sloppy storage, into buffer 1 with p_k moved there, where the next chunk
 // starts.
 template <class T> void mix_acc(hipStream_t s, const Geom& g, double2* x64, const MixBufs<T>& b, CGState* st);
+// Reliable update, residual: r = phi - D t with the FP64 links (t = D^dag x
+// from the first FP64 hop) and |r|^2 to the update's finisher, in one
+// launch gated on the pending update.  r12: the links pass the FP64 SU(3)
+// check (two rows read).
+void ru_resid(hipStream_t s, const Geom& g, const double2* U, const double2* t, const double2* phi, double2* r,
+              double mass, CGState* st, double* partials, bool r12);
 template <class T> void mix_apply(hipStream_t s, const Geom& g, const double2* r64, const MixBufs<T>& b, CGState* st);
 
 }  // namespace sfh

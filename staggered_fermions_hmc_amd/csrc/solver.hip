@@ -384,9 +384,10 @@ static int cg_f64(sfh_ctx* c, const sfh_gauge* gauge, const sfh_cg_params* prm, 
 // FP16 storage, with the FP64 reliable update (Clark et al.) run on the
 // device at the end of every chunk, gated on the solve state:
 //   mix_acc   : x += x_lo + alpha p_k, x_lo = 0
-//   hop_f64 x2: ax = D (D^dag x)                  (gate 1)
-//   ru_sub    : r64 = phi - ax, |r64|^2 -> FinRU (converged, out of budget,
-//               or restart with beta = |r_true|^2 / |r_k|^2 and alpha = 0)
+//   hop_f64   : t64 = D^dag x                     (gate 1)
+//   ru_resid  : r64 = phi - D t64, |r64|^2 -> FinRU (converged, out of
+//               budget, or restart with beta = |r_true|^2 / |r_k|^2 and
+//               alpha = 0)
 //   mix_apply : r_k = r64 in sloppy storage
 template <bool HALF>
 static int cg_mixed(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* prm, const double2* phi, double2* x,
@@ -398,11 +399,10 @@ static int cg_mixed(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* prm, cons
   if (int e = ensure_lp(c, gauge, HALF ? SFH_PREC_F16 : SFH_PREC_F32)) return e;
   double2* r64 = buf<double2>(c, "cg_r", n);
   double2* t64 = buf<double2>(c, "cg_t", n);
-  double2* ax = buf<double2>(c, "cg_v", n);
   const bool fused = mix_fused(HALF ? SFH_PREC_F16 : SFH_PREC_F32, g);
   MixBufs<T> b{};
   if (int e = mix_bufs<T>(c, g, fused, b)) return e;
-  if (!r64 || !t64 || !ax || !b.x) return SFH_E_NOMEM;
+  if (!r64 || !t64 || !b.x) return SFH_E_NOMEM;
   // iteration 0 reads r_{-1} = phi from buffer 1, p_{-1} = v_{-1} = 0, alpha = beta = 0
   const size_t ns = (size_t)kSpinorElems<T> * g.V;
   hipMemsetAsync(b.x, 0, n * sizeof(float2), c->stream);
@@ -445,8 +445,7 @@ static int cg_mixed(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* prm, cons
     if (defer) mix_fin(c->stream, g, st, part, dm);
     mix_acc<T>(c->stream, g, x, b, st);
     hop_f64(c->stream, g, gauge->U, x, t64, prm->mass, -1.0, nullptr, st, nullptr, 1, gauge->f64_su3);
-    hop_f64(c->stream, g, gauge->U, t64, ax, prm->mass, +1.0, nullptr, st, nullptr, 1, gauge->f64_su3);
-    ru_sub(c->stream, g, r64, phi, ax, st, part_ru);
+    ru_resid(c->stream, g, gauge->U, t64, phi, r64, prm->mass, st, part_ru, gauge->f64_su3);
     mix_apply<T>(c->stream, g, r64, b, st);
   };
   auto direct = [&]() {
@@ -454,7 +453,7 @@ static int cg_mixed(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* prm, cons
     tail();
   };
   const std::string key = graph_key(HALF ? (fused ? "mix16f" : "mix16s") : (fused ? "mix32f" : "mix32s"), g,
-                                    {gauge->U, (const void*)U, x, r64, phi, t64, ax, b.r[0], b.r[1], b.p[0], b.p[1],
+                                    {gauge->U, (const void*)U, x, r64, phi, t64, b.r[0], b.r[1], b.p[0], b.p[1],
                                      b.v, b.t, b.rn[0], b.rn[1], b.pn[0], b.pn[1], b.vn, b.tn, b.x, st, sts[1], part},
                                     prm->mass,
                                     ((get_hop_variant(0, g) * 4 + get_hop_variant(HALF ? 2 : 1, g)) * 2 +
