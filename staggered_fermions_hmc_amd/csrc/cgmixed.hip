@@ -356,27 +356,6 @@ __global__ __launch_bounds__(kBlk) void k_mix_b(Geom g, const typename Prec<T>::
   }
 }
 
-// Finisher of the last B of a chunk when B defers (kDotsPartials or
-// kDotsGroups): one workgroup, in place.
-template <int DM>
-__global__ __launch_bounds__(kBlk) void k_mix_fin(CGState* st, const double* __restrict__ partials, int nb) {
-  __shared__ CGState sst;
-  Partials4<kPartIters> pl;
-  double d[4];
-  if constexpr (DM == kDotsGroups) load_groups_n<4>(partials, nb, d);
-  else load_partials4(partials, nb, pl);
-  state_to_lds(sst, state_word(st));
-  __syncthreads();
-  if (!sst.fin_pending) return;  // uniform
-  if constexpr (DM != kDotsGroups) add_partials4(pl, d);
-  block_sum_n<4>(d);
-  if (threadIdx.x == 0) {
-    finish_fused(&sst, d);
-    sst.fin_pending = 0;
-    *st = sst;
-  }
-}
-
 // Split form, own-site part of kernel A: r_k, p_k and x updated in place
 // (r and p single-buffered: both slots point to one array).  The hop
 // t = D^dag p_k then runs as a separate launch.  The previous B left the
@@ -441,14 +420,63 @@ __global__ __launch_bounds__(kBlk) void k_mix_upd(int V, MixBufs<T> b, const CGS
   store_q<T>(b.p[0], b.pn[0], V, i, po, qp);
 }
 
-template <class T>
-__global__ __launch_bounds__(kBlk) void k_mix_acc(int V, double2* __restrict__ xh, MixBufs<T> b, CGState* st) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->ru_apply = 0;  // read only by the apply kernel of this chunk
-  if (!st->ru_pending) return;
+// First step of the reliable update at a chunk's end: x64 += x + alpha p_k
+// and x = 0 when an update is pending.  With deferred dots (DM not
+// kDotsFinish) the chunk's last B left its dots for a finisher: every
+// workgroup sums them in one fixed order (as kernel A or the update kernel
+// would) and runs it, reading the state from st_in and (workgroup 0)
+// writing the advanced state to st_out, the copy the rest of the update
+// works on; otherwise st_in == st_out.  nbb: the workgroups of kernel B.
+template <class T, int DM>
+__global__ __launch_bounds__(kBlk) void k_mix_acc(int V, double2* __restrict__ xh, MixBufs<T> b, const CGState* st_in,
+                                                  CGState* st_out, const double* __restrict__ partials, int nbb) {
+  __shared__ CGState sst;
+  if constexpr (DM == kDotsFinish) {
+    if (threadIdx.x == 0) sst = *st_in;
+    __syncthreads();
+  } else {
+    static_assert(kRedGroups <= 64, "the group sums fit one wave");
+    Partials4<kPartIters> pl;
+    double d[4];
+    if constexpr (DM == kDotsGroups) load_groups_n<4>(partials, nbb, d);
+    else load_partials4(partials, nbb, pl);
+    state_to_lds(sst, state_word(st_in));
+    __syncthreads();
+    if (sst.fin_pending) {  // uniform over the block
+      if constexpr (DM == kDotsGroups) {
+        if (threadIdx.x < 64) {  // the group sums sit in wave 0
+          wave_sum4_all(d);
+          if (threadIdx.x == 0) {
+            finish_fused(&sst, d);
+            sst.fin_pending = 0;
+          }
+        }
+      } else {
+        add_partials4(pl, d);
+        block_sum_n<4>(d);
+        if (threadIdx.x == 0) {
+          finish_fused(&sst, d);
+          sst.fin_pending = 0;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // ru_apply is read only by the apply kernel of this chunk
+    if constexpr (DM == kDotsFinish) {
+      st_out->ru_apply = 0;  // in place: the only field that changes
+    } else {
+      CGState o = sst;
+      o.ru_apply = 0;
+      *st_out = o;
+    }
+  }
+  if (!sst.ru_pending) return;  // uniform
   const int i = blockIdx.x * kBlk + threadIdx.x;
   if (i >= V) return;
-  const int k = st->pbuf;  // buffers of p_k, the direction of the last iteration
-  const double al = st->alpha;
+  const int k = sst.pbuf;  // buffers of p_k, the direction of the last iteration
+  const double al = sst.alpha;
   vec3<float> pk;
   load_spinor<T>(b.p[k], b.pn[k], V, i, pk);
 #pragma unroll
@@ -463,9 +491,13 @@ __global__ __launch_bounds__(kBlk) void k_mix_acc(int V, double2* __restrict__ x
   }
 }
 
+// Last step of the reliable update: r_k = r64 in sloppy storage when the
+// solve restarts; workgroup 0 first copies the update's state copy st to
+// st_out (the copy the next chunk starts from) when they differ.
 template <class T>
 __global__ __launch_bounds__(kBlk) void k_mix_apply(int V, const double2* __restrict__ r64, MixBufs<T> b,
-                                                    const CGState* st) {
+                                                    const CGState* st, CGState* st_out) {
+  if (st_out != st && blockIdx.x == 0 && threadIdx.x == 0) *st_out = *st;
   if (!st->ru_apply) return;
   const int i = blockIdx.x * kBlk + threadIdx.x;
   if (i >= V) return;
@@ -583,21 +615,28 @@ void ru_resid(hipStream_t s, const Geom& g, const double2* U, const double2* t, 
   else
     hipLaunchKernelGGL(k_ru_resid<false>, dim3(nb), dim3(kBlk), 0, s, g, U, t, phi, r, mass, st, partials, nb);
 }
-void mix_fin(hipStream_t s, const Geom& g, CGState* st, const double* partials, int dm) {
-  if (dm == kDotsGroups) hipLaunchKernelGGL(k_mix_fin<kDotsGroups>, dim3(1), dim3(kBlk), 0, s, st, partials, mix_blocks(g));
-  else hipLaunchKernelGGL(k_mix_fin<kDotsPartials>, dim3(1), dim3(kBlk), 0, s, st, partials, mix_blocks(g));
-}
 template <class T>
 void mix_upd(hipStream_t s, const Geom& g, const MixBufs<T>& b, const CGState* st_in, CGState* st_out,
              const double* partials) {
   hipLaunchKernelGGL(k_mix_upd<T>, dim3((g.V + kBlk - 1) / kBlk), dim3(kBlk), 0, s, g.V, b, st_in, st_out, partials,
                      mix_blocks(g));
 }
-template <class T> void mix_acc(hipStream_t s, const Geom& g, double2* x64, const MixBufs<T>& b, CGState* st) {
-  hipLaunchKernelGGL(k_mix_acc<T>, dim3((g.V + kBlk - 1) / kBlk), dim3(kBlk), 0, s, g.V, x64, b, st);
+template <class T>
+void mix_acc(hipStream_t s, const Geom& g, double2* x64, const MixBufs<T>& b, const CGState* st_in, CGState* st_out,
+             const double* partials, int dm) {
+  const dim3 grid((g.V + kBlk - 1) / kBlk);
+  const int nbb = mix_blocks(g);
+  if (dm == kDotsGroups)
+    hipLaunchKernelGGL((k_mix_acc<T, kDotsGroups>), grid, dim3(kBlk), 0, s, g.V, x64, b, st_in, st_out, partials, nbb);
+  else if (dm == kDotsPartials)
+    hipLaunchKernelGGL((k_mix_acc<T, kDotsPartials>), grid, dim3(kBlk), 0, s, g.V, x64, b, st_in, st_out, partials, nbb);
+  else
+    hipLaunchKernelGGL((k_mix_acc<T, kDotsFinish>), grid, dim3(kBlk), 0, s, g.V, x64, b, st_in, st_out, partials, nbb);
 }
-template <class T> void mix_apply(hipStream_t s, const Geom& g, const double2* r64, const MixBufs<T>& b, CGState* st) {
-  hipLaunchKernelGGL(k_mix_apply<T>, dim3((g.V + kBlk - 1) / kBlk), dim3(kBlk), 0, s, g.V, r64, b, st);
+template <class T>
+void mix_apply(hipStream_t s, const Geom& g, const double2* r64, const MixBufs<T>& b, const CGState* st,
+               CGState* st_out) {
+  hipLaunchKernelGGL(k_mix_apply<T>, dim3((g.V + kBlk - 1) / kBlk), dim3(kBlk), 0, s, g.V, r64, b, st, st_out);
 }
 
 template void mix_a<float>(hipStream_t, const Geom&, const float2*, const MixBufs<float>&, float, const CGState*,
@@ -611,9 +650,13 @@ template void mix_b<Half>(hipStream_t, const Geom&, const __half2*, const MixBuf
 template void mix_upd<float>(hipStream_t, const Geom&, const MixBufs<float>&, const CGState*, CGState*,
                              const double*);
 template void mix_upd<Half>(hipStream_t, const Geom&, const MixBufs<Half>&, const CGState*, CGState*, const double*);
-template void mix_acc<float>(hipStream_t, const Geom&, double2*, const MixBufs<float>&, CGState*);
-template void mix_acc<Half>(hipStream_t, const Geom&, double2*, const MixBufs<Half>&, CGState*);
-template void mix_apply<float>(hipStream_t, const Geom&, const double2*, const MixBufs<float>&, CGState*);
-template void mix_apply<Half>(hipStream_t, const Geom&, const double2*, const MixBufs<Half>&, CGState*);
+template void mix_acc<float>(hipStream_t, const Geom&, double2*, const MixBufs<float>&, const CGState*, CGState*,
+                             const double*, int);
+template void mix_acc<Half>(hipStream_t, const Geom&, double2*, const MixBufs<Half>&, const CGState*, CGState*,
+                            const double*, int);
+template void mix_apply<float>(hipStream_t, const Geom&, const double2*, const MixBufs<float>&, const CGState*,
+                               CGState*);
+template void mix_apply<Half>(hipStream_t, const Geom&, const double2*, const MixBufs<Half>&, const CGState*,
+                              CGState*);
 
 }  // namespace sfh

@@ -39,8 +39,6 @@ constexpr int kDotsFinish = 0, kDotsPartials = 1, kDotsGroups = 2;
 // Whether the fused form's B can leave per-workgroup partials: true when the
 // grid is small enough for every workgroup of A to read them all.
 bool mix_defers(const Geom& g);
-// Finisher of the last B of a chunk when B defers (one workgroup, in place).
-void mix_fin(hipStream_t s, const Geom& g, CGState* st, const double* partials, int dm);
 // Split form of kernel A (for lattices whose neighbour vectors do not stay
 // in cache): the own-site update alone, in place; the caller then runs the
 // plain hop t = D^dag p_k.  It first finishes the previous B's dots from
@@ -59,14 +57,21 @@ void mix_b(hipStream_t s, const Geom& g, const typename Prec<T>::L* U, const Mix
 // Reliable update, first and last steps (gated on the solve state):
 // x64 += x + alpha p_k (p_k from buffer st->pbuf) and x = 0; then r = r64 in
 // sloppy storage, into buffer 1 with p_k moved there, where the next chunk
-// starts.
-template <class T> void mix_acc(hipStream_t s, const Geom& g, double2* x64, const MixBufs<T>& b, CGState* st);
+// starts.  With deferred dots (dm not kDotsFinish) mix_acc first finishes
+// the chunk's last B from `partials`, reading the state from st_in and
+// writing it to st_out, the copy the rest of the update uses; mix_apply
+// then copies that state back to st_out, where the next chunk starts.
+template <class T>
+void mix_acc(hipStream_t s, const Geom& g, double2* x64, const MixBufs<T>& b, const CGState* st_in, CGState* st_out,
+             const double* partials, int dm);
 // Reliable update, residual: r = phi - D t with the FP64 links (t = D^dag x
 // from the first FP64 hop) and |r|^2 to the update's finisher, in one
 // launch gated on the pending update.  r12: the links pass the FP64 SU(3)
 // check (two rows read).
 void ru_resid(hipStream_t s, const Geom& g, const double2* U, const double2* t, const double2* phi, double2* r,
               double mass, CGState* st, double* partials, bool r12);
-template <class T> void mix_apply(hipStream_t s, const Geom& g, const double2* r64, const MixBufs<T>& b, CGState* st);
+template <class T>
+void mix_apply(hipStream_t s, const Geom& g, const double2* r64, const MixBufs<T>& b, const CGState* st,
+               CGState* st_out);
 
 }  // namespace sfh

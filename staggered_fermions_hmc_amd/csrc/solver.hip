@@ -442,11 +442,13 @@ static int cg_mixed(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* prm, cons
   };
   auto tail = [&]() {
     KTimer kt(c, "reliable_update");
-    if (defer) mix_fin(c->stream, g, st, part, dm);
-    mix_acc<T>(c->stream, g, x, b, st);
-    hop_f64(c->stream, g, gauge->U, x, t64, prm->mass, -1.0, nullptr, st, nullptr, 1, gauge->f64_su3);
-    ru_resid(c->stream, g, gauge->U, t64, phi, r64, prm->mass, st, part_ru, gauge->f64_su3);
-    mix_apply<T>(c->stream, g, r64, b, st);
+    // with deferred dots the update works on the second state copy (mix_acc
+    // finishes the chunk's last B into it) and mix_apply copies it back
+    CGState* sru = sts[1];
+    mix_acc<T>(c->stream, g, x, b, st, sru, part, dm);
+    hop_f64(c->stream, g, gauge->U, x, t64, prm->mass, -1.0, nullptr, sru, nullptr, 1, gauge->f64_su3);
+    ru_resid(c->stream, g, gauge->U, t64, phi, r64, prm->mass, sru, part_ru, gauge->f64_su3);
+    mix_apply<T>(c->stream, g, r64, b, sru, st);
   };
   auto direct = [&]() {
     for (int j = 0; j < kChunkMixed<HALF>; ++j) iter(j);
