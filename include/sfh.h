@@ -92,7 +92,8 @@ typedef struct {
   int precision;         /* sfh_precision                                     */
   double reliable_delta; /* mixed precision: reliable update due when |r|
                             drops by this factor since the last one, run at
-                            the end of that chunk (0 -> 0.1)                  */
+                            the end of that chunk (0 -> 0.1 for F16, 0.003
+                            for F32 and the batched solves)                   */
   int compute_true_residual; /* F64: 1 also report the FP64 true residual; 2
                                 also stop on it (once the iterated residual
                                 meets tol, r = phi - D D^dag x is formed and

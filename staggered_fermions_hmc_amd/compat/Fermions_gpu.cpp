@@ -161,7 +161,7 @@ sfh_cg_params cg_params() {
   p.tol = g_cfg.tol;
   p.max_iter = g_cfg.max_iter;
   p.precision = g_cfg.precision;
-  p.reliable_delta = 0.1;
+  p.reliable_delta = 0.0;  // the library default for the precision
   return p;
 }
 

@@ -432,7 +432,7 @@ size_t Engine<T>::slot_bytes(int V) {
 
 template <class T>
 Engine<T>::Engine(sfh_ctx* c, double tol, int max_iter, double delta, int chunk, bool use_graph)
-    : c_(c), tol_(tol), max_iter_(max_iter), delta_(delta > 0 ? delta : 0.1), chunk_(chunk), use_graph_(use_graph) {}
+    : c_(c), tol_(tol), max_iter_(max_iter), delta_(delta > 0 ? delta : kDeltaF32), chunk_(chunk), use_graph_(use_graph) {}
 
 template <class T>
 void Engine<T>::launch_chunk() {

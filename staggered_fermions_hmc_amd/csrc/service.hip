@@ -199,7 +199,7 @@ int sfh_service_create(int device, size_t mem_budget, double tol, int max_iter, 
   // when requests are admitted (or many have finished), and replayed while
   // the set of solves in flight holds; one launch per chunk keeps a lightly
   // loaded service from being bound by kernel launches on the host
-  s->eng.reset(new sfh::Engine<float>(s->ctx, tol, max_iter, 0.1, sfh::kBatchChunk, true));
+  s->eng.reset(new sfh::Engine<float>(s->ctx, tol, max_iter, 0.0, sfh::kBatchChunk, true));
   s->worker = std::thread(worker_loop, s);
   *out = s;
   return SFH_OK;

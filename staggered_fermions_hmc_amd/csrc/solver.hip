@@ -418,7 +418,7 @@ static int cg_mixed(sfh_ctx* c, sfh_gauge* gauge, const sfh_cg_params* prm, cons
   s.target = prm->tol * prm->tol * phi_sq;
   s.max_iter = prm->max_iter;
   s.mixed = 1;
-  s.delta = prm->reliable_delta > 0 ? prm->reliable_delta : 0.1;
+  s.delta = prm->reliable_delta > 0 ? prm->reliable_delta : (HALF ? kDeltaF16 : kDeltaF32);
   s.rmax = std::sqrt(phi_sq);
   s.rsq_true = phi_sq;
   s.pbuf = 1;

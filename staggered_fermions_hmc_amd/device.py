@@ -154,7 +154,7 @@ class Gauge:
         self.ctx.check(self.ctx._lib.sfh_apply_ddag(self.ctx.h, self.h, mass, ptr(a), ptr(out)))
         return out
 
-    def cg(self, phi, mass, tol=1e-12, max_iter=50000, precision="f64", reliable_delta=0.1,
+    def cg(self, phi, mass, tol=1e-12, max_iter=50000, precision="f64", reliable_delta=0.0,
            true_residual=False):
         a = self._spinor(phi)
         x = np.empty_like(a)
@@ -164,7 +164,7 @@ class Gauge:
                                             ctypes.byref(stats)))
         return x, stats.as_dict()
 
-    def cg_field(self, phi_field, x_field, mass, tol=1e-12, max_iter=50000, precision="f64", reliable_delta=0.1,
+    def cg_field(self, phi_field, x_field, mass, tol=1e-12, max_iter=50000, precision="f64", reliable_delta=0.0,
                  true_residual=False):
         prm = _abi.CGParams(mass, tol, max_iter, _abi.PREC[precision], reliable_delta, _true_mode(true_residual))
         stats = _abi.CGStats()
@@ -378,7 +378,7 @@ def _batch_array(systems):
     return arr, keep, total
 
 
-def cg_batch(ctx, systems, tol=1e-10, max_iter=100000, precision="f32", reliable_delta=0.1):
+def cg_batch(ctx, systems, tol=1e-10, max_iter=100000, precision="f32", reliable_delta=0.0):
     """Solve many systems together.  systems: list of (gauge, mass, [rhs
     Field, ...], [x Field, ...]).  Returns one stats dict per right-hand side,
     systems in order."""
@@ -389,7 +389,7 @@ def cg_batch(ctx, systems, tol=1e-10, max_iter=100000, precision="f32", reliable
     return [s.as_dict() for s in stats]
 
 
-def tune_batch(ctx, systems, tol=1e-10, max_iter=100000, precision="f32", reliable_delta=0.1, shape=None):
+def tune_batch(ctx, systems, tol=1e-10, max_iter=100000, precision="f32", reliable_delta=0.0, shape=None):
     """Autotune the batched hop's launch shape for this batch (kept in the
     context for batches of the same shape); the x fields are work space.
     shape=(sites per block, RHS per block) records that shape instead.
