@@ -383,7 +383,8 @@ static int cg_f64(sfh_ctx* c, const sfh_gauge* gauge, const sfh_cg_params* prm, 
 // Mixed-precision CG: the two-kernel iteration of cgmixed.hip in FP32 or
 // FP16 storage, with the FP64 reliable update (Clark et al.) run on the
 // device at the end of every chunk, gated on the solve state:
-//   mix_acc   : x += x_lo + alpha p_k, x_lo = 0
+//   mix_acc   : (deferred dots: finish the chunk's last iteration first)
+//               x += x_lo + alpha p_k, x_lo = 0
 //   hop_f64   : t64 = D^dag x                     (gate 1)
 //   ru_resid  : r64 = phi - D t64, |r64|^2 -> FinRU (converged, out of
 //               budget, or restart with beta = |r_true|^2 / |r_k|^2 and
