@@ -41,6 +41,16 @@ __global__ __launch_bounds__(256) void k_hop(Geom g, const typename Prec<T>::L* 
     const int V = g.V;
     const int i = p * g.Vh + cb;
     const int q = (1 - p) * g.Vh;
+    // FP64: the own site's input (and the dot's other operand) are loaded
+    // first, in flight with the links and neighbours (16^4 FP64 CG 242 ->
+    // 252 solves/s); the low-precision hops load them last, which keeps
+    // their register count and measured the same or faster
+    constexpr bool kEarly = std::is_same<T, double>::value;
+    SpinorWords<T> sw, ww;
+    if constexpr (kEarly) {
+      sw.load(in, V, i);
+      if constexpr (DOT) ww.load(dw, V, i);
+    }
     int n[4];
     cb_coords(g, cb, p, n);
     vec3<R> acc;
@@ -82,9 +92,10 @@ __global__ __launch_bounds__(256) void k_hop(Geom g, const typename Prec<T>::L* 
       }
     }
     vec3<R> res;
+    if constexpr (!kEarly) sw.load(in, V, i);
     if constexpr (kHalfR12<T, R12>) {
       // a n h + hb acc from the record's halves
-      const H16 sr = in[i];
+      const H16& sr = sw.s;
       const R an = a * sr.n;
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -94,7 +105,7 @@ __global__ __launch_bounds__(256) void k_hop(Geom g, const typename Prec<T>::L* 
       }
     } else {
       vec3<R> self;
-      load_spinor<T>(in, in_norm, V, i, self);
+      sw.decode(self);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         res.c[c].re = a * self.c[c].re + hb * acc.c[c].re;
@@ -103,8 +114,9 @@ __global__ __launch_bounds__(256) void k_hop(Geom g, const typename Prec<T>::L* 
     }
     store_spinor<T>(out, out_norm, V, i, res);
     if constexpr (DOT) {
+      if constexpr (!kEarly) ww.load(dw, V, i);
       vec3<R> w;
-      load_spinor<T>(dw, dw_norm, V, i, w);
+      ww.decode(w);
 #pragma unroll
       for (int c = 0; c < 3; ++c) dotv += (double)w.c[c].re * (double)res.c[c].re + (double)w.c[c].im * (double)res.c[c].im;
     }
