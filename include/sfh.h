@@ -69,7 +69,7 @@ typedef enum {
  * recurrence (Fermions.cpp:553-617).  F32 and F16 run the inner iterations in
  * low precision with FP64 reliable updates (true residual recomputed in FP64
  * once the iterated residual has dropped by reliable_delta, at the end of the
- * chunk of iterations in which that happens: up to 15 FP32 or 7 FP16
+ * chunk of iterations in which that happens: up to 23 FP32 or 9 FP16
  * iterations later), FP64 global reductions, and stop on the FP64 true
  * residual.  F16 stores spinors as
  * 16-bit floats with a per-site FP32 max-norm and links as 16-bit floats,

@@ -88,14 +88,15 @@ namespace {
 // Iterations per chunk.  Mixed precision: a reliable update that falls due
 // runs at the chunk's end, so the chunk length bounds how long the sloppy
 // residual drifts past the update point: FP16 storage drifts fastest and
-// takes short chunks (32^4, m = 0.01: 227 iterations at 8, 252 at 16);
-// FP32 takes longer ones, which pay the per-chunk launches less often
-// (16^4: 16 instead of 8 is 3.06 -> 2.93 ms per solve at 129 iterations;
-// 24 was 4 % faster again, with 6 updates instead of 8, but let the
-// residual fall about two decades before an update that fell due after
-// one: 16 keeps updates close to the factor they are set to).
+// takes short chunks (32^4, m = 0.01, three right-hand sides and 24^4:
+// 10 iterations per chunk 1 % faster than 8 on average, 12 mixed, 14 and
+// 16 up to 9 % slower from the drift); FP32 holds its recursion over the
+// 300-fold drop of its update threshold and takes longer chunks, which pay
+// the chunk's tail and launch less often (24 against 16: the same
+// iteration counts from 16^4 to 32^4, 16^4 3 % faster; 48 let a 24^4
+// solve drift by 7 iterations).
 constexpr int kChunk = 16;
-template <bool HALF> constexpr int kChunkMixed = HALF ? 8 : 16;
+template <bool HALF> constexpr int kChunkMixed = HALF ? 10 : 24;
 
 struct Bufs64 {
   double2 *r, *p, *t, *v;
