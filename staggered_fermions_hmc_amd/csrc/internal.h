@@ -19,7 +19,6 @@ namespace sfh {
 // 16^4 to 32^4 as at 0.1 with half the updates, 16^4 2.63 -> 2.53 ms).
 constexpr double kDeltaF16 = 0.1, kDeltaF32 = 0.003;
 
-
 // Per-solve device scalars.  Written only by the last workgroup of a grid
 // reduction (or by the host between launches), read by later launches, so
 // kernels of one stream never race on them.
