@@ -195,7 +195,8 @@ int sfh_service_create(int device, size_t mem_budget, double tol, int max_iter, 
     return e;
   }
   s->budget = mem_budget;
-  // chunks of 8 iterations replayed as HIP graphs: a graph is recaptured
+  // chunks of kBatchChunk iterations replayed as HIP graphs (reliable-update
+  // threshold: the engine's FP32 default): a graph is recaptured
   // when requests are admitted (or many have finished), and replayed while
   // the set of solves in flight holds; one launch per chunk keeps a lightly
   // loaded service from being bound by kernel launches on the host
