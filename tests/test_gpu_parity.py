@@ -322,6 +322,27 @@ def test_cg_16_4_both_precisions(ctx):
     assert rel(x32, x64) < 1e-8 and rel(x16, x64) < 1e-8
 
 
+@pytest.mark.parametrize("dims", [(24, 24, 24, 24), (12, 12, 12, 20)])
+def test_mixed_solves_on_non_power_of_two_lattices(ctx, dims):
+    """Mixed-precision solves where V is not a power of two: 24^4 (331776
+    sites, above the fused form's 2^18, so the split form with group sums)
+    and 12^3 x 20 (34560 sites, the fused form with per-workgroup partials),
+    in both storage precisions at their default update thresholds and chunk
+    lengths.  The true residual is the oracle's FP64 D D^dag of the returned
+    solution, not the solver's own report."""
+    m = 0.02
+    U = dev.random_links(dims, 0.3, seed=61)
+    g = dev.Gauge(ctx, dims, U)
+    phi = dev.random_spinor(dims, seed=62)
+    for prec in ("f32", "f16"):
+        x, st = g.cg(phi, m, tol=1e-10, precision=prec)
+        r = phi - oc.apply_ddag(dims, m, U, x)
+        true_res = np.sqrt(r @ r / (phi @ phi))
+        assert true_res < 1e-10, (prec, st)
+        assert abs(st["true_rel_resid"] - true_res) < 1e-12, (prec, st, true_res)
+        assert st["reliable_updates"] >= 1, (prec, st)
+
+
 # ---------------------------------------------------------------- forces / actions
 @pytest.mark.parametrize("dims", [(4, 4, 4, 4), (6, 4, 4, 2), (4, 6, 8, 4)])
 @pytest.mark.parametrize("c11,c12", [(1.0, 0.0), (5.0 / 3.0, -1.0 / 12.0), (0.0, 1.0)])
